@@ -1,0 +1,397 @@
+"""Python mirror of the l() / rk3_ssp() path of colinhu123/3T_RH on MI355X.
+
+The package loads ``librh3t.so`` (host Field/GhostGrid construction + gfx950
+HIP kernels, built by ``build.py``) through ctypes and mirrors the
+reference's data model:
+
+=======================  ==================================================
+reference (Rust)         here
+=======================  ==================================================
+``BCType``               :class:`BCType` factories (src/bc1.rs:16-28)
+``GridInfo``             :class:`GridInfo` (src/field1.rs:5-73)
+``Field`` + ``GhostGrid``:class:`Field` (src/field1.rs:75-160,
+                         src/ghost.rs:29-118): geometry, fluid mask,
+                         ghost tables; ``build_ghosts()``
+``l(u, ghosts, s)``      :meth:`Solver.l` (src/main.rs:135-305)
+``rk3_ssp``              :meth:`Solver.rk3_ssp` (src/main.rs:351-373)
+``calc_global_dt``       :meth:`Solver.calc_global_dt` (src/main.rs:375-408)
+time loop body           :meth:`Solver.advance` (src/main.rs:459-467)
+=======================  ==================================================
+
+Errors the reference turns into panics raise :class:`RH3TError` carrying the
+first failure (kind, RK stage, grid index, state).  All state arrays are
+float64 numpy arrays of shape (nx, ny, 6) in the reference's AoS order.
+There is no CPU fallback: without the built library every entry point
+raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librh3t.so")
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_NONPHYSICAL, ERR_GHOST, ERR_GEOMETRY = 1, 2, 3, 4, 5
+ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_SINGULAR, ERR_DT = 6, 7, 8, 9
+STATUS_NAMES = {
+    0: "ok", 1: "bad argument", 2: "HIP error", 3: "non-physical state", 4: "non-finite ghost state",
+    5: "geometry/ghost build error", 6: "unsupported", 7: "no HIP device", 8: "singular ILW system",
+    9: "non-positive dt",
+}
+
+BC_REFLECTIVE_WALL, BC_WALL, BC_PERIODIC, BC_CONSTANT = 0, 1, 2, 3
+BC_TIME_DEPENDENT, BC_OUTFLOW, BC_ZEROTH_ORDER, BC_FARFIELD = 4, 5, 6, 7
+ARITH_EXACT, ARITH_FAST = 0, 1
+
+
+class RH3TError(RuntimeError):
+    def __init__(self, status, message="", error=None):
+        self.status = status
+        self.error = error
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+
+
+# ----------------------------------------------------------------- C types
+
+class c_bc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("state", ctypes.c_double * 6),
+                ("p_inf", ctypes.c_double), ("sigma", ctypes.c_double), ("l_domain", ctypes.c_double)]
+
+
+class c_physics(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "gamma_e", "gamma_i", "gamma_r", "kappa_e", "kappa_i", "kappa_r", "cve", "cvi", "a_rad",
+        "omega_ei", "omega_er", "lambda_", "weno_q", "eps")]
+
+
+class c_grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("x0", ctypes.c_double), ("y0", ctypes.c_double)]
+
+
+class c_ghost(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "i", "j", "boundary", "side_id", "bc_kind", "heavy_id", "src_i", "src_j", "nearest_i",
+        "nearest_j")] + [(n, ctypes.c_double) for n in ("p0x", "p0y", "nx", "ny", "distance")]
+
+
+class c_ghost_bc(ctypes.Structure):
+    _fields_ = [("nearest_i", ctypes.c_int32), ("nearest_j", ctypes.c_int32), ("normal", ctypes.c_double * 2),
+                ("d", ctypes.c_double * 5), ("stencil", ctypes.c_uint32 * 55), ("s", ctypes.c_double * 979),
+                ("w", ctypes.c_double * 225)]
+
+
+class c_error(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("stage", ctypes.c_int32), ("i", ctypes.c_int32),
+                ("j", ctypes.c_int32), ("ghost_id", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("state", ctypes.c_double * 6), ("message", ctypes.c_char * 256)]
+
+
+_P = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); the complete C ABI of include/rh3t.h
+SIGNATURES = {
+    "rh3t_physics_default": (None, [ctypes.POINTER(c_physics)]),
+    "rh3t_problem_create": (ctypes.c_int, [ctypes.POINTER(c_grid), _DP, ctypes.c_int32, ctypes.POINTER(c_bc),
+                                           _DP, ctypes.c_int32, ctypes.POINTER(c_bc), ctypes.POINTER(_P)]),
+    "rh3t_problem_destroy": (None, [_P]),
+    "rh3t_problem_add_arc": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                            _DP, _DP, _DP]),
+    "rh3t_problem_build_ghosts": (ctypes.c_int, [_P]),
+    "rh3t_problem_init_cylinder": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.POINTER(_P)]),
+    "rh3t_problem_cylinder_state": (ctypes.c_int, [_P, ctypes.c_double, _DP]),
+    "rh3t_problem_grid": (ctypes.c_int, [_P, ctypes.POINTER(c_grid)]),
+    "rh3t_problem_counts": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_int64)]),
+    "rh3t_problem_fluid_mask": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint8)]),
+    "rh3t_problem_ghosts": (ctypes.c_int, [_P, ctypes.POINTER(c_ghost)]),
+    "rh3t_problem_ghost_bc": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(c_ghost_bc)]),
+    "rh3t_problem_bc": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(c_bc)]),
+    "rh3t_problem_access_table": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32)]),
+    "rh3t_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int32)]),
+    "rh3t_ctx_create": (ctypes.c_int, [_P, ctypes.POINTER(c_physics), ctypes.c_int32, ctypes.POINTER(_P)]),
+    "rh3t_ctx_destroy": (None, [_P]),
+    "rh3t_set_state": (ctypes.c_int, [_P, _DP, ctypes.c_double]),
+    "rh3t_get_state": (ctypes.c_int, [_P, _DP, _DP]),
+    "rh3t_l": (ctypes.c_int, [_P, _DP, _DP]),
+    "rh3t_rk3_step": (ctypes.c_int, [_P, ctypes.c_double]),
+    "rh3t_calc_global_dt": (ctypes.c_int, [_P, _DP]),
+    "rh3t_advance": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _DP]),
+    "rh3t_ctx_set_arith": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "rh3t_last_error": (ctypes.c_int, [_P, ctypes.POINTER(c_error)]),
+    "rh3t_profile": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "rh3t_profile_read": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, _DP,
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded librh3t.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RH3TError(ERR_UNSUPPORTED, f"{LIB_PATH} is not built; run 3t_rh_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status, what=""):
+    if status != OK:
+        raise RH3TError(status, what)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+# ------------------------------------------------------------- data model
+
+class Physics:
+    """constant.rs as a runtime struct (src/constant.rs:1-25)."""
+
+    FIELDS = ("gamma_e", "gamma_i", "gamma_r", "kappa_e", "kappa_i", "kappa_r", "cve", "cvi", "a_rad",
+              "omega_ei", "omega_er", "lambda_", "weno_q", "eps")
+
+    def __init__(self, **kw):
+        c = c_physics()
+        lib().rh3t_physics_default(ctypes.byref(c))
+        for k, v in kw.items():
+            if k not in self.FIELDS:
+                raise TypeError(f"unknown physics constant {k}")
+            setattr(c, k, float(v))
+        self.c = c
+
+    def __getattr__(self, k):
+        if k in Physics.FIELDS:
+            return getattr(self.c, k)
+        raise AttributeError(k)
+
+    def as_tuple(self):
+        return tuple(getattr(self.c, k) for k in self.FIELDS)
+
+
+class BCType:
+    """Factories for BCType (src/bc1.rs:16-28)."""
+
+    @staticmethod
+    def _mk(kind, state=None, p_inf=0.0, sigma=0.0, l_domain=0.0):
+        b = c_bc()
+        b.kind = kind
+        if state is not None:
+            for k in range(6):
+                b.state[k] = float(state[k])
+        b.p_inf, b.sigma, b.l_domain = p_inf, sigma, l_domain
+        return b
+
+    reflective_wall = staticmethod(lambda: BCType._mk(BC_REFLECTIVE_WALL))
+    wall = staticmethod(lambda: BCType._mk(BC_WALL))
+    periodic = staticmethod(lambda: BCType._mk(BC_PERIODIC))
+    zeroth_order = staticmethod(lambda: BCType._mk(BC_ZEROTH_ORDER))
+    constant = staticmethod(lambda state: BCType._mk(BC_CONSTANT, state))
+    farfield = staticmethod(lambda state: BCType._mk(BC_FARFIELD, state))
+    outflow = staticmethod(lambda p_inf, sigma, l_domain: BCType._mk(BC_OUTFLOW, None, p_inf, sigma, l_domain))
+
+
+class GridInfo:
+    def __init__(self, nx, ny, dx, dy, x0, y0):
+        self.nx, self.ny, self.dx, self.dy, self.x0, self.y0 = int(nx), int(ny), float(dx), float(dy), \
+            float(x0), float(y0)
+
+    def x(self, i):
+        return self.x0 + float(i) * self.dx
+
+    def y(self, j):
+        return self.y0 + float(j) * self.dy
+
+
+_DUMMY_INNER = [(-1002.0, -1002.0), (-1001.0, -1002.0), (-1001.0, -1001.0), (-1002.0, -1001.0)]
+
+
+class Field:
+    """Field + GhostGrid of the reference: grid, polygons, BC lists, arcs,
+    fluid mask and (after :meth:`build_ghosts`) the ghost tables."""
+
+    def __init__(self, grid, outer, bc_outer, inner=None, bc_inner=None, _handle=None):
+        self._h = ctypes.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            inner = inner if inner is not None else _DUMMY_INNER
+            bc_inner = bc_inner if bc_inner is not None else [BCType.wall() for _ in inner]
+            g = c_grid(grid.nx, grid.ny, grid.dx, grid.dy, grid.x0, grid.y0)
+            o = np.ascontiguousarray(np.asarray(outer, dtype=np.float64).reshape(-1))
+            n = np.ascontiguousarray(np.asarray(inner, dtype=np.float64).reshape(-1))
+            bo = (c_bc * len(bc_outer))(*bc_outer)
+            bi = (c_bc * len(bc_inner))(*bc_inner)
+            _check(lib().rh3t_problem_create(ctypes.byref(g), _dptr(o), len(bc_outer), bo, _dptr(n),
+                                             len(bc_inner), bi, ctypes.byref(self._h)), "Field::new")
+        gi = c_grid()
+        _check(lib().rh3t_problem_grid(self._h, ctypes.byref(gi)))
+        self.grid = GridInfo(gi.nx, gi.ny, gi.dx, gi.dy, gi.x0, gi.y0)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.rh3t_problem_destroy(h)
+            self._h = None
+
+    @classmethod
+    def init_cylinder(cls, nx=121, mach=3.0):
+        """init::init_cylinder (src/init.rs:116-510); ghosts built."""
+        h = ctypes.c_void_p()
+        _check(lib().rh3t_problem_init_cylinder(nx, mach, ctypes.byref(h)), "init_cylinder")
+        f = cls(None, None, None, _handle=h)
+        f.mach = mach
+        return f
+
+    def cylinder_state(self, mach=None):
+        u = np.zeros((self.grid.nx, self.grid.ny, 6))
+        _check(lib().rh3t_problem_cylinder_state(self._h, self.mach if mach is None else mach, _dptr(u)))
+        return u
+
+    def add_arc(self, boundary, side_start, side_end, start, mid, end, fluid_outside=True):
+        s, m, e = (np.asarray(p, dtype=np.float64) for p in (start, mid, end))
+        _check(lib().rh3t_problem_add_arc(self._h, boundary, side_start, side_end, int(fluid_outside),
+                                          _dptr(s), _dptr(m), _dptr(e)), "add_arc")
+
+    def build_ghosts(self):
+        _check(lib().rh3t_problem_build_ghosts(self._h), "GhostGrid::build")
+        return self
+
+    def counts(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().rh3t_problem_counts(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    @property
+    def fluid(self):
+        m = np.zeros(self.grid.nx * self.grid.ny, dtype=np.uint8)
+        _check(lib().rh3t_problem_fluid_mask(self._h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return m.reshape(self.grid.nx, self.grid.ny).astype(bool)
+
+    def ghosts(self):
+        """Structured array of the sorted ghost list (GhostInfo)."""
+        n = self.counts()[1]
+        arr = (c_ghost * max(n, 1))()
+        _check(lib().rh3t_problem_ghosts(self._h, arr))
+        dt = np.dtype([(f, np.int32) for f, _ in c_ghost._fields_[:10]] +
+                      [(f, np.float64) for f, _ in c_ghost._fields_[10:]])
+        return np.frombuffer(bytes(arr), dtype=dt, count=n).copy()
+
+    def ghost_bc(self, heavy_id):
+        g = c_ghost_bc()
+        _check(lib().rh3t_problem_ghost_bc(self._h, heavy_id, ctypes.byref(g)))
+        return g
+
+    def bc(self, boundary, side):
+        b = c_bc()
+        _check(lib().rh3t_problem_bc(self._h, boundary, side, ctypes.byref(b)))
+        return b
+
+    def access_table(self):
+        a = np.zeros(self.grid.nx * self.grid.ny * 17, dtype=np.uint32)
+        _check(lib().rh3t_problem_access_table(self._h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+               "access table")
+        return a
+
+    def solver(self, physics=None, device=0, arith=ARITH_FAST):
+        return Solver(self, physics, device, arith)
+
+
+def device_count():
+    n = ctypes.c_int32()
+    lib().rh3t_device_count(ctypes.byref(n))
+    return n.value
+
+
+class Solver:
+    """Device context: the l() / rk3_ssp() path on one GPU."""
+
+    def __init__(self, field, physics=None, device=0, arith=ARITH_FAST):
+        self.field = field
+        self.physics = physics if physics is not None else Physics()
+        self._h = ctypes.c_void_p()
+        st = lib().rh3t_ctx_create(field._h, ctypes.byref(self.physics.c), device, ctypes.byref(self._h))
+        if st != OK:
+            raise RH3TError(st, "rh3t_ctx_create")
+        _check(lib().rh3t_ctx_set_arith(self._h, arith))
+        self.nx, self.ny = field.grid.nx, field.grid.ny
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rh3t_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def _raise(self, st, what):
+        e = c_error()
+        lib().rh3t_last_error(self._h, ctypes.byref(e))
+        info = {"kind": e.kind, "stage": e.stage, "i": e.i, "j": e.j, "ghost_id": e.ghost_id,
+                "state": list(e.state), "message": e.message.decode(errors="replace")}
+        raise RH3TError(st, f"{what}: {info['message']}", info)
+
+    def set_arith(self, mode):
+        _check(lib().rh3t_ctx_set_arith(self._h, mode))
+
+    def set_state(self, u, time=0.0):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != (self.nx, self.ny, 6):
+            raise ValueError(f"state must have shape {(self.nx, self.ny, 6)}")
+        _check(lib().rh3t_set_state(self._h, _dptr(u), float(time)), "set_state")
+
+    def get_state(self):
+        u = np.zeros((self.nx, self.ny, 6))
+        t = ctypes.c_double()
+        _check(lib().rh3t_get_state(self._h, _dptr(u), ctypes.byref(t)), "get_state")
+        return u, t.value
+
+    def l(self, with_ghosts=False):
+        rhs = np.zeros((self.nx, self.ny, 6))
+        ng = self.field.counts()[1]
+        gv = np.zeros((max(ng, 1), 6)) if with_ghosts else None
+        st = lib().rh3t_l(self._h, _dptr(rhs), _dptr(gv) if with_ghosts else None)
+        if st != OK:
+            self._raise(st, "l()")
+        return (rhs, gv[:ng]) if with_ghosts else rhs
+
+    def rk3_ssp(self, dt):
+        st = lib().rh3t_rk3_step(self._h, float(dt))
+        if st != OK:
+            self._raise(st, "rk3_ssp")
+
+    def calc_global_dt(self):
+        d = ctypes.c_double()
+        _check(lib().rh3t_calc_global_dt(self._h, ctypes.byref(d)), "calc_global_dt")
+        return d.value
+
+    def advance(self, nsteps, cfl_factor=0.05, dt_fixed=0.0):
+        t = ctypes.c_double()
+        st = lib().rh3t_advance(self._h, int(nsteps), float(cfl_factor), float(dt_fixed), ctypes.byref(t))
+        if st != OK:
+            self._raise(st, "advance")
+        return t.value
+
+    def profile(self, enable=True):
+        _check(lib().rh3t_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        names = ctypes.create_string_buffer(1024)
+        avg = (ctypes.c_double * 32)()
+        cnt = (ctypes.c_int64 * 32)()
+        n = ctypes.c_int32()
+        _check(lib().rh3t_profile_read(self._h, 32, names, 1024, avg, cnt, ctypes.byref(n)))
+        labels = names.raw.split(b"\0")[: n.value]
+        return {labels[k].decode(): (avg[k], cnt[k]) for k in range(n.value)}

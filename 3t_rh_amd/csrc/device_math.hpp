@@ -1,0 +1,304 @@
+// Pointwise FP64 numerics of the l() path for gfx950, written against the
+// reference's formulas (colinhu123/3T_RH src/state.rs, src/weno.rs,
+// src/noncon.rs, src/diffusion.rs, src/source.rs, src/dt.rs).  Expressions
+// keep the reference's association and literal divisions; the translation
+// unit is compiled with -ffp-contract=off so products and sums round as the
+// reference's do.  Structural zeros and ones of the 6x6 eigenvector
+// matrices are skipped (x*0 + y == y and x*1 == x exactly, signed zeros
+// aside), everything else is evaluated as written.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace rh3t_dev {
+
+struct Phys {
+  double gamma_e, gamma_i, gamma_r;
+  double ge, gi, gr;  // gamma_k - 1.0
+  double gt;          // gi + ge + gr (reference order)
+  double kappa_e, kappa_i, kappa_r;
+  double cve, cvi, a_rad;
+  double omega_ei, omega_er;
+  double lambda, weno_q, eps;
+  int diffusion, source;
+};
+
+// Derived planes of the padded derived array.
+enum { D_U = 0, D_V, D_EE, D_EI, D_ER, D_PE, D_PI, D_PR, D_TE, D_TI, D_TR, D_N };
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+// x.powi(3) = x * (x*x), x.powi(4) = (x*x)*(x*x)
+__device__ __forceinline__ double cube(double x) { return x * (x * x); }
+__device__ __forceinline__ double pow4(double x) { double x2 = x * x; return x2 * x2; }
+
+// Derived::from_state, src/state.rs:241-275.  te/ti/tr only when asked.
+__device__ __forceinline__ void derived_from_state(const Phys& ph, const double s[6], double d[D_N],
+                                                   bool temps) {
+  double rho = s[0];
+  double u = s[1] / rho;
+  double v = s[2] / rho;
+  double w2 = sq(u) + sq(v);
+  d[D_U] = u;
+  d[D_V] = v;
+  d[D_EE] = s[3] / rho - w2 / 6.0;
+  d[D_EI] = s[4] / rho - w2 / 6.0;
+  d[D_ER] = s[5] / rho - w2 / 6.0;
+  d[D_PE] = ph.ge * (s[3] - rho * w2 / 6.0);
+  d[D_PI] = ph.gi * (s[4] - rho * w2 / 6.0);
+  d[D_PR] = ph.gr * (s[5] - rho * w2 / 6.0);
+  if (temps) {
+    d[D_TE] = s[3] / (rho * ph.cve) - w2 / (6.0 * ph.cve);
+    d[D_TI] = s[4] / (rho * ph.cvi) - w2 / (6.0 * ph.cvi);
+    double er_term = s[5] / rho - w2 / 6.0;
+    d[D_TR] = pow(er_term * rho / ph.a_rad, 0.25);
+  }
+}
+
+// weno5, src/weno.rs:846-886
+__device__ __forceinline__ double weno5(double u0, double u1, double u2, double u3, double u4,
+                                        double eps) {
+  double beta2 = 13.0 / 12.0 * sq(u2 - 2.0 * u3 + u4) + 0.25 * sq(3.0 * u2 - 4.0 * u3 + u4);
+  double beta1 = 13.0 / 12.0 * sq(u1 - 2.0 * u2 + u3) + 0.25 * sq(u1 - u3);
+  double beta0 = 13.0 / 12.0 * sq(u0 - 2.0 * u1 + u2) + 0.25 * sq(u0 - 4.0 * u1 + 3.0 * u2);
+  double a0 = 0.1 / sq(eps + beta0);
+  double a1 = 0.6 / sq(eps + beta1);
+  double a2 = 0.3 / sq(eps + beta2);
+  double sum_o = a0 + a1 + a2;
+  double w0 = a0 / sum_o, w1 = a1 / sum_o, w2 = a2 / sum_o;
+  double p0 = u0 / 3.0 - 7.0 / 6.0 * u1 + 11.0 / 6.0 * u2;
+  double p1 = -u1 / 6.0 + 5.0 / 6.0 * u2 + u3 / 3.0;
+  double p2 = u2 / 3.0 + 5.0 / 6.0 * u3 - u4 / 6.0;
+  return w0 * p0 + w1 * p1 + w2 * p2;
+}
+
+// Roe state and the closed-form 6x6 left/right eigenvectors of
+// Stencil6::build_l_plain / build_r_plain (src/weno.rs:28-548).  DIR 0 = X
+// (rows: u-c, electron, ion, radiation, transverse, u+c), DIR 1 = Y (rows:
+// v-c, transverse, electron, ion, radiation, v+c).
+struct Eigen {
+  double L[6][6];
+  double R[6][6];
+  double lam[6];
+};
+
+// Energy-mode row of L, src/weno.rs:110-233 (identical in X and Y).
+__device__ __forceinline__ void energy_row(int mode, double hk, double u, double v, double w2,
+                                           double cs2, double gt, double b, double ge, double gi,
+                                           double gr, double row[6]) {
+  double gk = mode == 0 ? ge : (mode == 1 ? gi : gr);
+  row[0] = gk / gt - gk * w2 / (6.0 * cs2) - gk * gt * w2 / b * (hk - cs2 * w2);
+  row[1] = gk * u / (3.0 * cs2) + 2.0 * gk * gt * u / b * (hk - cs2 * w2);
+  row[2] = gk * v / (3.0 * cs2) + 2.0 * gk * gt * v / b * (hk - cs2 * w2);
+  if (mode == 0) {
+    row[3] = -sq(ge) / (gt * cs2) - 6.0 * ge / b * (ge * hk - gt * cs2 * w2);
+    row[4] = -ge * gi / (gt * cs2) - 6.0 * ge * gi / b * (hk - 6.0 * gr * cs2);
+    row[5] = -ge * gr / (gt * cs2) - 6.0 * ge * gr / b * (hk + 6.0 * gi * cs2);
+  } else if (mode == 1) {
+    row[3] = -gi * ge / (gt * cs2) - 6.0 * gi * ge / b * (hk + 6.0 * gr * cs2);
+    row[4] = -sq(gi) / (gt * cs2) - 6.0 * gi / b * (gi * hk - gt * cs2 * w2);
+    row[5] = -gi * gr / (gt * cs2) - 6.0 * gi * gr / b * (hk - 6.0 * ge * cs2);
+  } else {
+    row[3] = -gr * ge / (gt * cs2) - 6.0 * gr * ge / b * (hk - 6.0 * gi * cs2);
+    row[4] = -gr * gi / (gt * cs2) - 6.0 * gr * gi / b * (hk + 6.0 * ge * cs2);
+    row[5] = -sq(gr) / (gt * cs2) - 6.0 * gr / b * (gr * hk - gt * cs2 * w2);
+  }
+}
+
+// rho1/rho2: densities of points [3] and [2]; d3/d2 their (u, v, e_e, e_i,
+// e_r) in derived order.
+template <int DIR>
+__device__ __forceinline__ void build_eigen(const Phys& ph, double rho3, double rho2, double u3,
+                                            double v3, double ee3, double ei3, double er3,
+                                            double u2, double v2, double ee2, double ei2,
+                                            double er2, Eigen& E) {
+  double r1 = sqrt(rho3);
+  double r2 = sqrt(rho2);
+  double rs = r1 + r2;
+  double u = (u3 * r1 + u2 * r2) / rs;
+  double v = (v3 * r1 + v2 * r2) / rs;
+  double w2 = sq(u) + sq(v);
+  double ee = (ee3 * r1 + ee2 * r2) / rs;
+  double ei = (ei3 * r1 + ei2 * r2) / rs;
+  double er = (er3 * r1 + er2 * r2) / rs;
+  const double ge = ph.ge, gi = ph.gi, gr = ph.gr, gt = ph.gt;
+  double cs2 = ph.gamma_e * ge * ee + ph.gamma_i * gi * ei + ph.gamma_r * gr * er;
+  double cs = sqrt(cs2);
+  double he = 6.0 * gi * gr * (ph.gamma_i * ei - ph.gamma_r * er) + ph.gamma_e * gt * ee * w2;
+  double hi = 6.0 * ge * gr * (ph.gamma_r * er - ph.gamma_e * ee) + ph.gamma_i * gt * ei * w2;
+  double hr = 6.0 * ge * gi * (ph.gamma_e * ee - ph.gamma_i * ei) + ph.gamma_r * gt * er * w2;
+  double b = gt * (36.0 * ge * gi * gr + gt * sq(w2)) * cs2;
+  double ad = 12.0 * cs2;
+  const int ie = DIR == 0 ? 1 : 2, ii = DIR == 0 ? 2 : 3, ir = DIR == 0 ? 3 : 4;
+  const int it = DIR == 0 ? 4 : 1;
+  if (DIR == 0) {
+    E.L[0][0] = (gt * w2 + 6.0 * u * cs) / ad;
+    E.L[0][1] = (-2.0 * gt * u - 6.0 * cs) / ad;
+    E.L[0][2] = (-2.0 * gt * v) / ad;
+    E.L[5][0] = (gt * w2 - 6.0 * u * cs) / ad;
+    E.L[5][1] = (-2.0 * gt * u + 6.0 * cs) / ad;
+    E.L[5][2] = (-2.0 * gt * v) / ad;
+    E.L[it][0] = -v;
+    E.L[it][1] = 0.0;
+    E.L[it][2] = 1.0;
+  } else {
+    E.L[0][0] = (gt * w2 + 6.0 * v * cs) / ad;
+    E.L[0][1] = (-2.0 * gt * u) / ad;
+    E.L[0][2] = (-2.0 * gt * v - 6.0 * cs) / ad;
+    E.L[5][0] = (gt * w2 - 6.0 * v * cs) / ad;
+    E.L[5][1] = (-2.0 * gt * u) / ad;
+    E.L[5][2] = (-2.0 * gt * v + 6.0 * cs) / ad;
+    E.L[it][0] = -u;
+    E.L[it][1] = 1.0;
+    E.L[it][2] = 0.0;
+  }
+  E.L[it][3] = 0.0;
+  E.L[it][4] = 0.0;
+  E.L[it][5] = 0.0;
+  E.L[0][3] = 6.0 * ge / ad;
+  E.L[0][4] = 6.0 * gi / ad;
+  E.L[0][5] = 6.0 * gr / ad;
+  E.L[5][3] = E.L[0][3];
+  E.L[5][4] = E.L[0][4];
+  E.L[5][5] = E.L[0][5];
+  energy_row(0, he, u, v, w2, cs2, gt, b, ge, gi, gr, E.L[ie]);
+  energy_row(1, hi, u, v, w2, cs2, gt, b, ge, gi, gr, E.L[ii]);
+  energy_row(2, hr, u, v, w2, cs2, gt, b, ge, gi, gr, E.L[ir]);
+
+  double ge_ = ph.gamma_e * ee + w2 / 6.0;
+  double gi_ = ph.gamma_i * ei + w2 / 6.0;
+  double gr_ = ph.gamma_r * er + w2 / 6.0;
+  if (DIR == 0) {
+    double ucs = u * cs / 3.0;
+    const double r[6][6] = {{1.0, 1.0, 1.0, 1.0, 0.0, 1.0},
+                            {u - cs, u, u, u, 0.0, u + cs},
+                            {v, v, v, v, 1.0, v},
+                            {ge_ - ucs, gt * w2 / (6.0 * ge), -gr, gi, v / 3.0, ge_ + ucs},
+                            {gi_ - ucs, gr, gt * w2 / (6.0 * gi), -ge, v / 3.0, gi_ + ucs},
+                            {gr_ - ucs, -gi, ge, gt * w2 / (6.0 * gr), v / 3.0, gr_ + ucs}};
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) E.R[a][c] = r[a][c];
+    E.lam[0] = u - cs;
+    E.lam[1] = u;
+    E.lam[2] = u;
+    E.lam[3] = u;
+    E.lam[4] = u;
+    E.lam[5] = u + cs;
+  } else {
+    double vcs = v * cs / 3.0;
+    const double r[6][6] = {{1.0, 0.0, 1.0, 1.0, 1.0, 1.0},
+                            {u, 1.0, u, u, u, u},
+                            {v - cs, 0.0, v, v, v, v + cs},
+                            {ge_ - vcs, u / 3.0, gt * w2 / (6.0 * ge), -gr, gi, ge_ + vcs},
+                            {gi_ - vcs, u / 3.0, gr, gt * w2 / (6.0 * gi), -ge, gi_ + vcs},
+                            {gr_ - vcs, u / 3.0, -gi, ge, gt * w2 / (6.0 * gr), gr_ + vcs}};
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) E.R[a][c] = r[a][c];
+    E.lam[0] = v - cs;
+    E.lam[1] = v;
+    E.lam[2] = v;
+    E.lam[3] = v;
+    E.lam[4] = v;
+    E.lam[5] = v + cs;
+  }
+}
+
+// l_dot / r_dot row, src/weno.rs:823-843
+__device__ __forceinline__ double dot6(const double r[6], const double x[6]) {
+  return r[0] * x[0] + r[1] * x[1] + r[2] * x[2] + r[3] * x[3] + r[4] * x[4] + r[5] * x[5];
+}
+
+// q_pulse, src/noncon.rs:49-57
+__device__ __forceinline__ double q_pulse(double b0, double b1, double b2, double b3, double b4,
+                                          double b5, double b6) {
+  return -0.00714285714285723 * b0 + 0.0595238095238118 * b1 - 0.240476190476197 * b2 +
+         0.759523809523819 * b3 + 0.509523809523803 * b4 - 0.0904761904761891 * b5 +
+         0.00952380952380977 * b6;
+}
+
+// jump_component of n_jump_pre, src/noncon.rs:259-285, on a[k..k+8)
+__device__ __forceinline__ double jump(const double* a) {
+  return q_pulse(a[7], a[6], a[5], a[4], a[3], a[2], a[1]) -
+         q_pulse(a[0], a[1], a[2], a[3], a[4], a[5], a[6]);
+}
+
+// sixth_order_derivative, src/noncon.rs:26-44
+__device__ __forceinline__ double sixth(double qm3, double qm2, double qm1, double qp1, double qp2,
+                                        double qp3, double dx) {
+  return (qp3 - 9.0 * qp2 + 45.0 * qp1 - 45.0 * qm1 + 9.0 * qm2 - qm3) / (60.0 * dx);
+}
+
+// nonconservative_direction_pre, src/noncon.rs:288-332.  vel[k], pe/pi/pr[k]
+// at the 9 points d-4..d+4; writes the ee, ei, er components.
+__device__ __forceinline__ void noncon(const double vel[9], const double pe[9], const double pi[9],
+                                       const double pr[9], double ds, double out[3]) {
+  double v = vel[4];
+  double dpe = sixth(pe[1], pe[2], pe[3], pe[5], pe[6], pe[7], ds);
+  double dpi = sixth(pi[1], pi[2], pi[3], pi[5], pi[6], pi[7], ds);
+  double dpr = sixth(pr[1], pr[2], pr[3], pr[5], pr[6], pr[7], ds);
+  double der_ee = 2.0 * dpe - dpi - dpr;
+  double der_ei = 2.0 * dpi - dpe - dpr;
+  double der_er = 2.0 * dpr - dpe - dpi;
+  double nee[9], nei[9], ner[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    nee[k] = 2.0 * pe[k] - pi[k] - pr[k];
+    nei[k] = 2.0 * pi[k] - pe[k] - pr[k];
+    ner[k] = 2.0 * pr[k] - pe[k] - pi[k];
+  }
+  double cl = fmax(fmax(v, vel[3]), 0.0) / 3.0;
+  double cr = fmin(fmin(v, vel[5]), 0.0) / 3.0;
+  out[0] = v / 3.0 * der_ee + (cl * jump(nee) + cr * jump(nee + 1)) / ds;
+  out[1] = v / 3.0 * der_ei + (cl * jump(nei) + cr * jump(nei + 1)) / ds;
+  out[2] = v / 3.0 * der_er + (cl * jump(ner) + cr * jump(ner + 1)) / ds;
+}
+
+// build_diffusion_from_derived, src/diffusion.rs:27-50: t[0..6) are the
+// temperatures at points -2..+3 around the interface.  Returns the flux of
+// one energy equation (kappa * .../180); tr4 selects Tr^4.
+__device__ __forceinline__ double diffusion_flux(double kappa, const double t[6]) {
+  return kappa * (2.0 * t[5] - 25.0 * t[4] + 245.0 * t[3] - 245.0 * t[2] + 25.0 * t[1] -
+                  2.0 * t[0]) /
+         180.0;
+}
+
+// get_local_dt, src/dt.rs:5-34
+__device__ __forceinline__ double local_dt(const Phys& ph, const double s[6], double dx, double dy) {
+  const double A = ph.a_rad, OEI = ph.omega_ei, OER = ph.omega_er, CVE = ph.cve, CVI = ph.cvi;
+  double rho = s[0];
+  double u = s[1] / rho, v = s[2] / rho;
+  double ee = s[3] / rho - (sq(u) + sq(v)) / 6.0;
+  double ei = s[4] / rho - (sq(u) + sq(v)) / 6.0;
+  double er = s[5] / rho - (sq(u) + sq(v)) / 6.0;
+  double cs = sqrt(ph.gamma_e * ph.ge * ee + ph.gamma_i * ph.gi * ei + ph.gamma_r * ph.gr * er);
+  double cve4 = pow4(CVE), cve3 = cube(CVE);
+  double t = A * OEI * cve4 + A * OEI * cve3 * CVI + 4.0 * A * OER * CVI * cube(ee) +
+             OER * cve4 * CVI * rho;
+  double s1 = sqrt(sq(t) - 4.0 * A * OEI * OER * cve4 * CVI *
+                               (4.0 * A * cube(ee) + cve4 * rho + cve3 * CVI * rho));
+  double s2 = -(A * OEI * cve4 + A * OEI * cve3 * CVI + 4.0 * A * OER * CVI * cube(ee) +
+                OER * cve4 * CVI * rho);
+  double sd = 2.0 * A * cve4 * CVI * rho;
+  double alpha1 = fabs((-s1 + s2) / sd);
+  double alpha2 = fabs((s1 + s2) / sd);
+  double sj = fmax(alpha1, alpha2);
+  double dj = fmax(fmax(ph.kappa_e / (CVE * rho), ph.kappa_i / (CVI * rho)), ph.kappa_r / A);
+  double vj = (fabs(u) + cs) / dx + 2.0 * dj / (sq(dx) + sq(dy)) + sj + (fabs(v) + cs) / dy;
+  return ph.lambda / vj;
+}
+
+// assert_admissible, src/main.rs:22-33
+__device__ __forceinline__ bool admissible(const double s[6]) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+    if (!isfinite(s[c])) return false;
+  if (s[0] <= 0.0) return false;
+  double ux = s[1] / s[0], uy = s[2] / s[0];
+  double k = (ux * ux + uy * uy) / 6.0;
+  return !(s[3] / s[0] - k <= 0.0 || s[4] / s[0] - k <= 0.0 || s[5] / s[0] - k <= 0.0);
+}
+
+}  // namespace rh3t_dev

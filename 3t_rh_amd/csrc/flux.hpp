@@ -1,0 +1,306 @@
+// Characteristic WENO5 interface flux, Stencil6::reconstruction_fast of
+// colinhu123/3T_RH (src/weno.rs:685-802), for one interface per lane.
+//
+// Register plan: the 6-point stencils of U and of the physical flux F stay
+// in VGPRs (72 doubles); the Roe state is reduced to ~16 scalars and each row
+// of L / column of R (src/weno.rs:28-548) is generated when its
+// characteristic field is processed, so the 6x6 matrices never exist as a
+// whole.  Uniform subexpressions of the physics constants are evaluated once
+// on the host with the reference's association (they round identically).
+//
+// Two arithmetic variants share this code:
+//   Exact  (FAST = false): the reference's rounding sequence -- literal
+//          divisions, left-to-right dot products, no FMA contraction.
+//   Fast   (FAST = true): one reciprocal per shared denominator, FMA
+//          chains, and a single division per WENO5 evaluation.  Same
+//          algebra, different rounding.
+#pragma once
+
+#include "device_math.hpp"
+
+namespace rh3t_dev {
+
+// Host-precomputed uniform constants of the eigensystem (index k: 0 e, 1 i,
+// 2 r).  Each is evaluated exactly as the reference's expression prefix.
+struct EigConst {
+  double g[3];         // gamma_k - 1
+  double gam[3];       // gamma_k
+  double gamg[3];      // gamma_k * g_k                (cs2 terms)
+  double g_over_gt[3]; // g_k / gt
+  double g_gt[3];      // g_k * gt
+  double two_g_gt[3];  // 2.0 * g_k * gt
+  double six_g[3];     // 6.0 * g_k
+  double h6[3];        // 6.0*gi*gr, 6.0*ge*gr, 6.0*ge*gi (h_e, h_i, h_r)
+  double gam_gt[3];    // gamma_k * gt
+  double num[3][3];    // -g_k^2 (j == k) or -g_k * g_j
+  double coef[3][3];   // 6.0*g_k (j == k) or 6.0*g_k*g_j
+  double c36;          // 36.0*ge*gi*gr
+  double m2gt;         // -2.0*gt
+  double gt;
+  // Fast variant only
+  double inv_gt;       // 1/gt
+  double inv6g[3];     // 1/(6 g_k)
+};
+
+template <int DIR, bool FAST>
+struct Roe {
+  double u, v, w2, cs, cs2, b, h[3], gk_[3], pcs;  // gk_: gamma_k e_k + w2/6, pcs: (u|v)*cs/3
+  double cs2w2, gtcs2;
+  // Fast: reciprocals
+  double ics2, ib, iad, igtcs2;
+};
+
+template <bool FAST>
+__device__ __forceinline__ double mdiv(double a, double b, double ib) {
+  return FAST ? a * ib : a / b;
+}
+
+template <int DIR, bool FAST>
+__device__ __forceinline__ void roe_state(const EigConst& K, double rho3, double rho2, double u3, double v3,
+                                          const double e3[3], double u2, double v2, const double e2[3],
+                                          Roe<DIR, FAST>& R) {
+  const double r1 = sqrt(rho3), r2 = sqrt(rho2);
+  const double rs = r1 + r2;
+  double e[3];
+  if (FAST) {
+    const double irs = 1.0 / rs;
+    R.u = fma(u3, r1, u2 * r2) * irs;
+    R.v = fma(v3, r1, v2 * r2) * irs;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) e[k] = fma(e3[k], r1, e2[k] * r2) * irs;
+    R.w2 = fma(R.u, R.u, R.v * R.v);
+    R.cs2 = fma(K.gamg[2], e[2], fma(K.gamg[1], e[1], K.gamg[0] * e[0]));
+  } else {
+    R.u = (u3 * r1 + u2 * r2) / rs;
+    R.v = (v3 * r1 + v2 * r2) / rs;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) e[k] = (e3[k] * r1 + e2[k] * r2) / rs;
+    R.w2 = sq(R.u) + sq(R.v);
+    R.cs2 = K.gamg[0] * e[0] + K.gamg[1] * e[1] + K.gamg[2] * e[2];
+  }
+  R.cs = sqrt(R.cs2);
+  const double w2 = R.w2, cs2 = R.cs2;
+  // h_e, h_i, h_r (src/weno.rs:70-76)
+  const double ge_e = K.gam[0] * e[0], gi_e = K.gam[1] * e[1], gr_e = K.gam[2] * e[2];
+  if (FAST) {
+    R.h[0] = fma(K.h6[0], gi_e - gr_e, K.gam_gt[0] * e[0] * w2);
+    R.h[1] = fma(K.h6[1], gr_e - ge_e, K.gam_gt[1] * e[1] * w2);
+    R.h[2] = fma(K.h6[2], ge_e - gi_e, K.gam_gt[2] * e[2] * w2);
+    R.b = K.gt * fma(K.gt, w2 * w2, K.c36) * cs2;
+  } else {
+    R.h[0] = K.h6[0] * (gi_e - gr_e) + K.gam_gt[0] * e[0] * w2;
+    R.h[1] = K.h6[1] * (gr_e - ge_e) + K.gam_gt[1] * e[1] * w2;
+    R.h[2] = K.h6[2] * (ge_e - gi_e) + K.gam_gt[2] * e[2] * w2;
+    R.b = K.gt * (K.c36 + K.gt * sq(w2)) * cs2;
+  }
+  R.gtcs2 = K.gt * cs2;
+  R.cs2w2 = cs2 * w2;
+  const double w6 = FAST ? w2 * (1.0 / 6.0) : w2 / 6.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) R.gk_[k] = K.gam[k] * e[k] + w6;
+  const double wv = DIR == 0 ? R.u : R.v;
+  R.pcs = FAST ? wv * R.cs * (1.0 / 3.0) : wv * R.cs / 3.0;
+  if (FAST) {
+    R.ics2 = 1.0 / cs2;
+    R.ib = 1.0 / R.b;
+    R.iad = R.ics2 * (1.0 / 12.0);
+    R.igtcs2 = R.ics2 * K.inv_gt;
+  }
+}
+
+// Row c of L (src/weno.rs:28-459).  X: rows u-c, e, i, r, transverse, u+c.
+// Y: rows v-c, transverse, e, i, r, v+c.
+template <int DIR, bool FAST>
+__device__ __forceinline__ void l_row(const EigConst& K, const Roe<DIR, FAST>& R, int c, double row[6]) {
+  const double u = R.u, v = R.v, w2 = R.w2, cs = R.cs, cs2 = R.cs2;
+  const int tr = DIR == 0 ? 4 : 1;
+  if (c == 0 || c == 5) {
+    const double ad = 12.0 * cs2;
+    const double sgn = c == 0 ? 1.0 : -1.0;
+    const double wv = DIR == 0 ? u : v;
+    // (gt*w2 +- 6*w*cs) / ad, src/weno.rs:87-108 / 258-280
+    const double six_w_cs = 6.0 * wv * cs;
+    const double a0 = c == 0 ? K.gt * w2 + six_w_cs : K.gt * w2 - six_w_cs;
+    const double six_cs = 6.0 * cs;
+    double a1, a2;
+    if (DIR == 0) {
+      a1 = c == 0 ? K.m2gt * u - six_cs : K.m2gt * u + six_cs;
+      a2 = K.m2gt * v;
+    } else {
+      a1 = K.m2gt * u;
+      a2 = c == 0 ? K.m2gt * v - six_cs : K.m2gt * v + six_cs;
+    }
+    (void)sgn;
+    row[0] = mdiv<FAST>(a0, ad, R.iad);
+    row[1] = mdiv<FAST>(a1, ad, R.iad);
+    row[2] = mdiv<FAST>(a2, ad, R.iad);
+    row[3] = mdiv<FAST>(K.six_g[0], ad, R.iad);
+    row[4] = mdiv<FAST>(K.six_g[1], ad, R.iad);
+    row[5] = mdiv<FAST>(K.six_g[2], ad, R.iad);
+    return;
+  }
+  if (c == tr) {
+    row[0] = DIR == 0 ? -v : -u;
+    row[1] = DIR == 0 ? 0.0 : 1.0;
+    row[2] = DIR == 0 ? 1.0 : 0.0;
+    row[3] = row[4] = row[5] = 0.0;
+    return;
+  }
+  const int k = DIR == 0 ? c - 1 : c - 2;  // energy mode 0 e, 1 i, 2 r
+  const double gk = K.g[k], hk = R.h[k];
+  const double A = hk - R.cs2w2;
+  if (FAST) {
+    const double t = R.ib * A;
+    const double ics6 = R.ics2 * (1.0 / 6.0), ics3 = R.ics2 * (1.0 / 3.0);
+    row[0] = K.g_over_gt[k] - gk * w2 * ics6 - K.g_gt[k] * w2 * t;
+    row[1] = fma(gk * u, ics3, K.two_g_gt[k] * u * t);
+    row[2] = fma(gk * v, ics3, K.two_g_gt[k] * v * t);
+  } else {
+    row[0] = K.g_over_gt[k] - gk * w2 / (6.0 * cs2) - K.g_gt[k] * w2 / R.b * A;
+    row[1] = gk * u / (3.0 * cs2) + K.two_g_gt[k] * u / R.b * A;
+    row[2] = gk * v / (3.0 * cs2) + K.two_g_gt[k] * v / R.b * A;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double inner;
+    if (j == k) {
+      inner = gk * hk - R.gtcs2 * w2;
+    } else {
+      const int m = 3 - k - j;
+      const double t6 = K.six_g[m] * cs2;
+      inner = (j == (k + 1) % 3) ? hk - t6 : hk + t6;
+    }
+    if (FAST)
+      row[3 + j] = fma(K.num[k][j], R.igtcs2, -(K.coef[k][j] * R.ib) * inner);
+    else
+      row[3 + j] = K.num[k][j] / R.gtcs2 - K.coef[k][j] / R.b * inner;
+  }
+}
+
+// Entry (r, c) of R (src/weno.rs:484-548); structural zeros are reported
+// through *zero, ones through *one.
+template <int DIR, bool FAST>
+__device__ __forceinline__ double r_entry(const EigConst& K, const Roe<DIR, FAST>& R, int r, int c, bool* zero,
+                                          bool* one) {
+  *zero = false;
+  *one = false;
+  const double u = R.u, v = R.v, cs = R.cs;
+  const int tr = DIR == 0 ? 4 : 1;
+  if (c == tr) {  // transverse column: X (0,0,1,v/3,..), Y (0,1,0,u/3,..)
+    const double w = DIR == 0 ? v : u;
+    if (r < 3) {
+      const bool is_one = (DIR == 0 && r == 2) || (DIR == 1 && r == 1);
+      *zero = !is_one;
+      *one = is_one;
+      return is_one ? 1.0 : 0.0;
+    }
+    return FAST ? w * (1.0 / 3.0) : w / 3.0;
+  }
+  if (r == 0) {
+    *one = true;
+    return 1.0;
+  }
+  if (c == 0 || c == 5) {
+    const double s = c == 0 ? -1.0 : 1.0;
+    if (r == 1) return DIR == 0 ? (c == 0 ? u - cs : u + cs) : u;
+    if (r == 2) return DIR == 1 ? (c == 0 ? v - cs : v + cs) : v;
+    (void)s;
+    return c == 0 ? R.gk_[r - 3] - R.pcs : R.gk_[r - 3] + R.pcs;
+  }
+  if (r == 1) return u;
+  if (r == 2) return v;
+  const int k = DIR == 0 ? c - 1 : c - 2;  // energy column
+  const int j = r - 3;
+  if (j == k) return FAST ? K.gt * R.w2 * K.inv6g[k] : K.gt * R.w2 / K.six_g[k];
+  const int m = 3 - k - j;
+  return (j == (k + 1) % 3) ? K.g[m] : -K.g[m];
+}
+
+// weno5, src/weno.rs:846-886
+template <bool FAST>
+__device__ __forceinline__ double weno5v(double u0, double u1, double u2, double u3, double u4, double eps) {
+  if (!FAST) return weno5(u0, u1, u2, u3, u4, eps);
+  const double d2 = u2 - 2.0 * u3 + u4, e2 = fma(3.0, u2, fma(-4.0, u3, u4));
+  const double d1 = u1 - 2.0 * u2 + u3, e1 = u1 - u3;
+  const double d0 = u0 - 2.0 * u1 + u2, e0 = fma(-4.0, u1, u0) + 3.0 * u2;
+  const double c13 = 13.0 / 12.0;
+  const double beta2 = fma(c13 * d2, d2, 0.25 * e2 * e2);
+  const double beta1 = fma(c13 * d1, d1, 0.25 * e1 * e1);
+  const double beta0 = fma(c13 * d0, d0, 0.25 * e0 * e0);
+  double s0 = eps + beta0, s1 = eps + beta1, s2 = eps + beta2;
+  s0 *= s0;
+  s1 *= s1;
+  s2 *= s2;
+  // w_k = (d_k / s_k) / sum_m (d_m / s_m), with all three over one division
+  const double t0 = 0.1 * s1 * s2, t1 = 0.6 * s0 * s2, t2 = 0.3 * s0 * s1;
+  const double q0 = fma(2.0, u0, fma(-7.0, u1, 11.0 * u2));
+  const double q1 = fma(-1.0, u1, fma(5.0, u2, 2.0 * u3));
+  const double q2 = fma(2.0, u2, fma(5.0, u3, -u4));
+  const double num = fma(t0, q0, fma(t1, q1, t2 * q2));
+  return num / (6.0 * (t0 + t1 + t2));
+}
+
+template <bool FAST>
+__device__ __forceinline__ double dot6v(const double r[6], const double x[6]) {
+  if (!FAST) return dot6(r, x);
+  double acc = r[0] * x[0];
+#pragma unroll
+  for (int k = 1; k < 6; ++k) acc = fma(r[k], x[k], acc);
+  return acc;
+}
+
+// One interface: U[q], Fl[q] for stencil points q = 0..5 (interface between
+// points 2 and 3); e2/e3 the specific energies of points 2 and 3.
+template <int DIR, bool FAST>
+__device__ __forceinline__ void interface_flux(const EigConst& K, double eps, const double U[6][6],
+                                               const double Fl[6][6], double u3, double v3, const double e3[3],
+                                               double u2, double v2, const double e2[3], double out[6]) {
+  Roe<DIR, FAST> R;
+  roe_state<DIR, FAST>(K, U[3][0], U[2][0], u3, v3, e3, u2, v2, e2, R);
+  const int tr = DIR == 0 ? 4 : 1;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double row[6];
+    l_row<DIR, FAST>(K, R, c, row);
+    double lam = (c == 0) ? (DIR == 0 ? R.u - R.cs : R.v - R.cs)
+                          : (c == 5 ? (DIR == 0 ? R.u + R.cs : R.v + R.cs) : (DIR == 0 ? R.u : R.v));
+    const double a = fabs(lam);
+    double fp[6], fm[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      double cf, cu;
+      if (c == tr) {  // row (-w, e_t): two terms, exact in both variants
+        cf = row[0] * Fl[q][0] + (DIR == 0 ? Fl[q][2] : Fl[q][1]);
+        cu = row[0] * U[q][0] + (DIR == 0 ? U[q][2] : U[q][1]);
+      } else {
+        cf = dot6v<FAST>(row, Fl[q]);
+        cu = dot6v<FAST>(row, U[q]);
+      }
+      if (FAST) {
+        fp[q] = 0.5 * fma(a, cu, cf);
+        fm[q] = 0.5 * fma(-a, cu, cf);
+      } else {
+        fp[q] = 0.5 * (cf + a * cu);
+        fm[q] = 0.5 * (cf - a * cu);
+      }
+    }
+    const double fc = weno5v<FAST>(fp[0], fp[1], fp[2], fp[3], fp[4], eps) +
+                      weno5v<FAST>(fm[5], fm[4], fm[3], fm[2], fm[1], eps);
+    // r_dot accumulated over c left to right
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      bool zero, one;
+      const double rv = r_entry<DIR, FAST>(K, R, r, c, &zero, &one);
+      if (c == 0) {
+        out[r] = one ? fc : rv * fc;
+      } else if (!zero) {
+        if (one)
+          out[r] = out[r] + fc;
+        else
+          out[r] = FAST ? fma(rv, fc, out[r]) : out[r] + rv * fc;
+      }
+    }
+  }
+}
+
+}  // namespace rh3t_dev

@@ -1,0 +1,140 @@
+"""Benchmark of the l() / rk3_ssp() hot path on one MI355X.
+
+Workload: the smooth-mode periodic box [0,1)^2 at 2048 x 2048, 6 FP64
+fields, seeded sum of 8 Fourier modes per primitive field, SSP-RK3 with
+dt = 0.05*dt_cfl computed on the device every step (the reference's time
+loop body, src/main.rs:459-467).  A "step" is one RK3 step (3 l() + 3
+stage updates); throughput counts fluid cells x 3 stages x steps / wall.
+
+Prints ONE JSON line.  The timed region runs the device time loop with
+hipEvents recorded around every kernel on the solver's own stream; the
+per-kernel averages give the roofline entry for the dominant kernel (the
+x-direction WENO flux).  The CPU baseline is the oracle (C restatement of
+the reference algorithm) timed on a bounded sample of the same workload:
+one RK3 step of the same 2048^2 box on the host's cores.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 cell-stage updates/s, %HBM/FP64 peak; batched vs plain; adjoint/fwd time"
+PEAK_FP64 = 78.6e12    # MI355X dense FP64 (vector and matrix), flop/s
+PEAK_HBM = 8.0e12      # MI355X HBM3E, B/s
+# Algorithmic FP64 operation counts "as written in source" (+ - * / sqrt
+# each 1, powi(n) as n-1 multiplies): one characteristic WENO5 interface flux
+# of src/weno.rs:685-802 and one interior cell-stage (2 fluxes + noncon x/y +
+# derived + assembly + stage combination + admissibility).
+FLOP_PER_INTERFACE = 2293
+FLOP_PER_CELL_STAGE = 5460
+BYTES_PER_CELL_STAGE = 128  # stage 1: read U^k, write U (96 B); stages 2-3: + U^n (144 B)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(field, u, workloads, max_threads=16):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bridge as ob
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, max_threads))
+    prob = ob.OracleProblem(field)
+    n_fluid = int(field.fluid.sum())
+    dt = 0.05 * ob.global_dt(prob, u)
+    t0 = time.perf_counter()
+    ob.rk3_step(prob, u, dt, nthreads=cores)
+    wall = time.perf_counter() - t0
+    return {"value": n_fluid * 3 / wall, "unit": "cell-stage/s", "cores": cores, "kind": "port",
+            "sample": f"1 SSP-RK3 step (3 l() + stages) of the same {field.grid.nx}x{field.grid.ny} box, "
+                      f"C oracle with OpenMP, {wall:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
+        sys.exit(2)
+
+    rh3t = importlib.import_module("3t_rh_amd")
+    workloads = importlib.import_module("3t_rh_amd.workloads")
+    n = args.size
+    log(f"building {n}x{n} periodic box")
+    field = workloads.periodic_box(n)
+    u0 = workloads.smooth_modes_state(field, seed=3)
+    n_fluid = int(field.fluid.sum())
+    solver = field.solver(arith=rh3t.ARITH_FAST if args.arith == "fast" else rh3t.ARITH_EXACT)
+    solver.set_state(u0, 0.0)
+    log("warmup")
+    solver.advance(args.warmup, 0.05)
+
+    # timed region: K steps, per-kernel hipEvents on the solver's stream
+    solver.profile(True)
+    t0 = time.perf_counter()
+    solver.advance(args.steps, 0.05)  # returns after a stream sync
+    wall = time.perf_counter() - t0
+    prof = solver.profile_read()
+    solver.profile(False)
+    log(f"timed {args.steps} steps in {wall:.3f} s")
+
+    # graph replay (no per-kernel events) of the same loop, for reference
+    t0 = time.perf_counter()
+    solver.advance(args.steps, 0.05)
+    wall_graph = time.perf_counter() - t0
+    u_end, t_end = solver.get_state()
+    finite = bool(abs(u_end[field.fluid]).max() < 1e300)
+
+    rate = n_fluid * 3 * args.steps / wall
+    fx_ms, fx_n = prof["flux_x"]
+    interfaces = (n + 1) * n
+    achieved = FLOP_PER_INTERFACE * interfaces / (fx_ms * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": rate,
+        "unit": "cell-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"smooth-mode periodic box {n}x{n}, 6 FP64 fields, SSP-RK3, dt=0.05*dt_cfl",
+                   "grid": [n, n], "fluid_cells": n_fluid, "arith": args.arith},
+        "fp64_frac_of_peak": FLOP_PER_CELL_STAGE * rate / PEAK_FP64,
+        "hbm_frac_of_peak": BYTES_PER_CELL_STAGE * rate / PEAK_HBM,
+        "ms_per_step_graph": wall_graph / args.steps * 1e3,
+        "kernels_ms": {k: round(v[0], 4) for k, v in prof.items() if v[1]},
+        "state_finite": finite,
+        # compute-bound: priced against the dense FP64 peak (78.6 TFLOP/s, the
+        # same for MFMA and vector FP64 on MI355X); the kernel uses VALU FP64.
+        "roofline": {"bound": "mfma", "pipe": "fp64 valu", "kernel": "flux_x", "achieved": achieved / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "traffic": None, "flop_per_interface": FLOP_PER_INTERFACE, "interfaces": interfaces,
+                     "avg_launch_ms": fx_ms},
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle, 1 step)")
+        line["cpu_baseline"] = cpu_baseline(field, u0, workloads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

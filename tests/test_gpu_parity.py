@@ -1,0 +1,192 @@
+"""GPU parity: the HIP l() / rk3_ssp() path against the CPU oracle.
+
+Tolerance (north_star): 1e-12 relative, max-norm per field over fluid cells
+(``oracle_bridge.relerr``).  The EXACT arithmetic variant keeps the
+reference's rounding sequence and is held to 1e-13; ghost bookkeeping is
+integer-exact.  Every case prints its name before touching the GPU.
+"""
+import numpy as np
+import pytest
+
+import oracle_bridge as ob
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+TOL_EXACT = 1e-13
+
+
+def _say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def box64(rh3t, workloads):
+    f = workloads.periodic_box(64)
+    return f, workloads.smooth_modes_state(f, seed=3), ob.OracleProblem(f)
+
+
+@pytest.fixture(scope="module")
+def cyl(rh3t, workloads):
+    f = workloads.cylinder(121, 3.0)
+    return f, f.cylinder_state(), ob.OracleProblem(f)
+
+
+@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
+def test_l_periodic_box(rh3t, box64, arith):
+    f, u, p = box64
+    _say(f"l() periodic box 64 arith={arith}")
+    s = f.solver(arith=arith)
+    s.set_state(u)
+    rhs, gv = s.l(with_ghosts=True)
+    ref, gref = ob.l(p, u, with_ghosts=True)
+    assert np.array_equal(gv, gref), "periodic ghosts are copies: bit-exact"
+    err = ob.relerr(rhs, ref, f.fluid)
+    print("relerr", err)
+    assert err.max() <= (TOL_EXACT if arith == 0 else TOL)
+    s.close()
+
+
+def test_l_periodic_box_fast_vs_exact_2048_sample(rh3t, workloads):
+    """Full-size operator: fast vs exact variants on the 2048^2 box (a
+    size-independent check: the two must agree to roundoff everywhere)."""
+    _say("l() periodic box 2048 fast vs exact")
+    f = workloads.periodic_box(2048)
+    u = workloads.smooth_modes_state(f, seed=3)
+    s = f.solver(arith=1)
+    s.set_state(u)
+    a = s.l()
+    s.set_arith(0)
+    b = s.l()
+    err = ob.relerr(a, b)
+    print("fast vs exact relerr", err)
+    assert err.max() <= TOL
+    # conservation of the divergence form on a periodic box
+    tot = np.abs(b[..., :3].sum(axis=(0, 1))).max()
+    assert tot <= 1e-9 * np.abs(b).max() * 2048
+    s.close()
+
+
+def test_rk3_step_periodic_box(rh3t, box64):
+    f, u, p = box64
+    _say("rk3 step periodic box 64")
+    dt = 0.05 * ob.global_dt(p, u)
+    s = f.solver()
+    s.set_state(u, 0.0)
+    assert abs(s.calc_global_dt() - ob.global_dt(p, u)) <= 1e-15 * ob.global_dt(p, u)
+    s.rk3_ssp(dt)
+    got, t = s.get_state()
+    ref, tref = ob.rk3_step(p, u, dt)
+    assert t == tref
+    err = ob.relerr(got, ref, f.fluid)
+    print("relerr", err)
+    assert err.max() <= TOL
+    s.close()
+
+
+def test_advance_matches_oracle_steps(rh3t, box64):
+    """Device time loop (hipGraph, dt = 0.05*dt_cfl on the device) against the
+    oracle's step-by-step loop."""
+    f, u, p = box64
+    _say("advance 5 steps periodic box 64")
+    s = f.solver()
+    s.set_state(u, 0.0)
+    t = s.advance(5, 0.05)
+    got, _ = s.get_state()
+    ref, tref = u.copy(), 0.0
+    for _ in range(5):
+        dt = 0.05 * ob.global_dt(p, ref)
+        ref, tref = ob.rk3_step(p, ref, dt, tref)
+    assert abs(t - tref) <= 1e-14 * tref
+    err = ob.relerr(got, ref, f.fluid)
+    print("relerr", err)
+    assert err.max() <= TOL
+    s.close()
+
+
+@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
+def test_l_cylinder(rh3t, cyl, arith):
+    f, u, p = cyl
+    _say(f"l() cylinder 121x481 arith={arith}")
+    # develop some structure with the oracle first (2 steps)
+    for _ in range(2):
+        u, _t = ob.rk3_step(p, u, 0.05 * ob.global_dt(p, u))
+    s = f.solver(arith=arith)
+    s.set_state(u)
+    rhs, gv = s.l(with_ghosts=True)
+    ref, gref = ob.l(p, u, with_ghosts=True)
+    gerr = ob.relerr(gv, gref)
+    err = ob.relerr(rhs, ref, f.fluid)
+    print("ghost relerr", gerr, "rhs relerr", err)
+    assert gerr.max() <= 1e-12
+    assert err.max() <= (TOL_EXACT if arith == 0 else TOL)
+    s.close()
+
+
+def test_rk3_cylinder_steps(rh3t, cyl):
+    f, u, p = cyl
+    _say("rk3 cylinder 3 steps with per-step resync")
+    s = f.solver()
+    for step in range(3):
+        dt = 0.05 * ob.global_dt(p, u)
+        s.set_state(u, 0.0)
+        s.rk3_ssp(dt)
+        got, _ = s.get_state()
+        ref, _ = ob.rk3_step(p, u, dt)
+        err = ob.relerr(got, ref, f.fluid)
+        print("step", step, "relerr", err)
+        assert err.max() <= TOL
+        u = ref
+    s.close()
+
+
+def test_l_translating_shock(rh3t, workloads):
+    _say("l() translating shock 128")
+    f = workloads.translating_shock(128)
+    u = workloads.translating_shock_state(f, seed=2)
+    p = ob.OracleProblem(f)
+    s = f.solver()
+    s.set_state(u)
+    rhs, gv = s.l(with_ghosts=True)
+    ref, gref = ob.l(p, u, with_ghosts=True)
+    print("ghost relerr", ob.relerr(gv, gref), "rhs relerr", ob.relerr(rhs, ref, f.fluid))
+    assert ob.relerr(gv, gref).max() <= 1e-12
+    assert ob.relerr(rhs, ref, f.fluid).max() <= TOL
+    s.close()
+
+
+def test_l_diffusion_and_source(rh3t, workloads):
+    """kappa != 0 and omega != 0 switch on the diffusion and exchange terms
+    (src/constant.rs:22-25); both paths against the oracle."""
+    _say("l() periodic box 64 with diffusion + source")
+    f = workloads.periodic_box(64)
+    u = workloads.smooth_modes_state(f, seed=7)
+    kw = dict(kappa_e=0.01, kappa_i=0.02, kappa_r=0.005, omega_ei=1.0, omega_er=0.1)
+    s = f.solver(physics=rh3t.Physics(**kw))
+    s.set_state(u)
+    rhs = s.l()
+    ref = ob.l(ob.OracleProblem(f), u, phys=ob.physics(**kw))
+    err = ob.relerr(rhs, ref, f.fluid)
+    print("relerr", err)
+    assert err.max() <= TOL
+    s.close()
+
+
+def test_nonphysical_state_error(rh3t, box64):
+    """A negative density must surface as the first non-physical state of
+    stage 1 (main.rs:24-33), at the same cell as the oracle reports."""
+    f, u, p = box64
+    _say("error path: injected negative density")
+    bad = u.copy()
+    bad[10, 20, 0] = -1.0
+    s = f.solver()
+    s.set_state(bad)
+    with pytest.raises(rh3t.RH3TError) as ei:
+        s.rk3_ssp(1e-4)
+    with pytest.raises(ob.OracleError) as eo:
+        ob.rk3_step(p, bad, 1e-4)
+    g, o = ei.value.error, eo.value.info
+    print("gpu", g, "oracle", o)
+    assert ei.value.status == rh3t.ERR_NONPHYSICAL
+    assert (g["stage"], g["i"], g["j"]) == (o["stage"], o["i"], o["j"])
+    s.close()
