@@ -26,12 +26,13 @@ sys.path.insert(0, ROOT)
 METRIC = "FP64 cell-stage updates/s, %HBM/FP64 peak; batched vs plain; adjoint/fwd time"
 PEAK_FP64 = 78.6e12    # MI355X dense FP64 (vector and matrix), flop/s
 PEAK_HBM = 8.0e12      # MI355X HBM3E, B/s
-# Algorithmic FP64 operation counts "as written in source" (+ - * / sqrt
-# each 1, powi(n) as n-1 multiplies): one characteristic WENO5 interface flux
-# of src/weno.rs:685-802 and one interior cell-stage (2 fluxes + noncon x/y +
-# derived + assembly + stage combination + admissibility).
-FLOP_PER_INTERFACE = 2293
-FLOP_PER_CELL_STAGE = 5460
+# Algorithmic FP64 operation counts (+ - * / sqrt pow each 1, powi(n) as n-1
+# multiplies, constant subexpressions 0), regenerated by oracle/opcount.cpp
+# from the oracle: one characteristic WENO5 interface flux of
+# src/weno.rs:685-802 and one interior cell-stage (x and y fluxes, noncon
+# x/y, derived, assembly, mean stage combination, admissibility).
+FLOP_PER_INTERFACE = 2252
+FLOP_PER_CELL_STAGE = 5250
 BYTES_PER_CELL_STAGE = 128  # stage 1: read U^k, write U (96 B); stages 2-3: + U^n (144 B)
 
 

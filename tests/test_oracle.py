@@ -375,3 +375,18 @@ def test_farfield_ghosts_reproduce_uniform_freestream(rh3t):
     assert np.abs(gv[refl][:, [0, 3, 4, 5]] - uinf[[0, 3, 4, 5]]).max() == 0.0
     mn = np.hypot(gv[refl][:, 1], gv[refl][:, 2])
     assert np.abs(mn - abs(uinf[1])).max() < 1e-12
+
+
+def test_opcount_matches_bench_constants():
+    """bench.py's roofline flop counts are the OpCount instantiation's."""
+    import importlib
+    import json
+    import os
+    import subprocess
+    exe = os.path.join(ob.ORACLE_DIR, "_build", "opcount")
+    if not os.path.exists(exe):
+        ob.build()
+    counts = json.loads(subprocess.run([exe], check=True, capture_output=True, text=True).stdout)
+    bench = importlib.import_module("bench")
+    assert counts["flux"] == bench.FLOP_PER_INTERFACE
+    assert round(counts["cell_stage"]) == bench.FLOP_PER_CELL_STAGE
