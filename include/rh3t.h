@@ -225,6 +225,13 @@ int rh3t_advance(rh3t_ctx* ctx, int32_t nsteps, double cfl_factor,
 #define RH3T_ARITH_FAST 1
 int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode);
 
+/* Workgroup shape of the WENO flux / assembly kernels (bx lanes along y,
+ * by rows along x; bx a multiple of 16, bx*by <= 256) and heavy ghosts per
+ * workgroup (1..4, one wavefront each).  Chosen by the autotuner in
+ * 3t_rh_amd/autotune.py; no counterpart in the reference. */
+int rh3t_ctx_set_launch(rh3t_ctx* ctx, int32_t bx, int32_t by,
+                        int32_t ghosts_per_block);
+
 /* First failure of the last call (main.rs:24-33 / ghost.rs:229-263). */
 int rh3t_last_error(const rh3t_ctx* ctx, rh3t_error* err);
 
@@ -235,6 +242,38 @@ int rh3t_profile(rh3t_ctx* ctx, int32_t enable);
 int rh3t_profile_read(rh3t_ctx* ctx, int32_t max_kernels, char* names,
                       int32_t names_cap, double* avg_ms, int64_t* launches,
                       int32_t* n_kernels);
+
+/* ------------------------------------------------------------------------ */
+/* Batched solves: many independent problems (uneven grids, different      */
+/* inflow states) advanced together.  No counterpart in the reference,     */
+/* which runs one Field per process; each solve behaves exactly as an      */
+/* rh3t_ctx on its own problem (same kernels, per-solve dt).               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rh3t_batch rh3t_batch;
+
+int rh3t_batch_create(const rh3t_problem* const* problems, int32_t n,
+                      const rh3t_physics* phys, int32_t device,
+                      rh3t_batch** out);
+void rh3t_batch_destroy(rh3t_batch* b);
+int rh3t_batch_set_state(rh3t_batch* b, int32_t solve, const double* u_aos,
+                         double time);
+int rh3t_batch_get_state(rh3t_batch* b, int32_t solve, double* u_aos,
+                         double* time);
+/* nsteps x { per solve: dt_s = cfl_factor * calc_global_dt_s; rk3_ssp(dt_s) }
+ * for all solves at once: one hipGraph of batched kernels per step, every
+ * grid-shaped kernel covering the tiles of all solves.  times[n] (may be
+ * NULL) receives each solve's Field.time. */
+int rh3t_batch_advance(rh3t_batch* b, int32_t nsteps, double cfl_factor,
+                       double* times);
+int rh3t_batch_set_launch(rh3t_batch* b, int32_t bx, int32_t by,
+                          int32_t ghosts_per_block);
+int rh3t_batch_last_error(const rh3t_batch* b, int32_t solve,
+                          rh3t_error* err);
+int rh3t_batch_profile(rh3t_batch* b, int32_t enable);
+int rh3t_batch_profile_read(rh3t_batch* b, int32_t max_kernels, char* names,
+                            int32_t names_cap, double* avg_ms,
+                            int64_t* launches, int32_t* n_kernels);
 
 #ifdef __cplusplus
 }
