@@ -103,6 +103,8 @@ SIGNATURES = {
     "rh3t_problem_build_ghosts": (ctypes.c_int, [_P]),
     "rh3t_problem_init_cylinder": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.POINTER(_P)]),
     "rh3t_problem_cylinder_state": (ctypes.c_int, [_P, ctypes.c_double, _DP]),
+    "rh3t_problem_clone": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "rh3t_problem_set_bc_state": (ctypes.c_int, [_P, ctypes.c_int32, _DP]),
     "rh3t_problem_grid": (ctypes.c_int, [_P, ctypes.POINTER(c_grid)]),
     "rh3t_problem_counts": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_int64)]),
@@ -121,6 +123,18 @@ SIGNATURES = {
     "rh3t_calc_global_dt": (ctypes.c_int, [_P, _DP]),
     "rh3t_advance": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _DP]),
     "rh3t_ctx_set_arith": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "rh3t_ctx_set_launch": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "rh3t_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.POINTER(c_physics),
+                                         ctypes.c_int32, ctypes.POINTER(_P)]),
+    "rh3t_batch_destroy": (None, [_P]),
+    "rh3t_batch_set_state": (ctypes.c_int, [_P, ctypes.c_int32, _DP, ctypes.c_double]),
+    "rh3t_batch_get_state": (ctypes.c_int, [_P, ctypes.c_int32, _DP, _DP]),
+    "rh3t_batch_advance": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, _DP]),
+    "rh3t_batch_set_launch": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "rh3t_batch_last_error": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(c_error)]),
+    "rh3t_batch_profile": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "rh3t_batch_profile_read": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, _DP,
+                                               ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
     "rh3t_last_error": (ctypes.c_int, [_P, ctypes.POINTER(c_error)]),
     "rh3t_profile": (ctypes.c_int, [_P, ctypes.c_int32]),
     "rh3t_profile_read": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, _DP,
@@ -254,6 +268,18 @@ class Field:
         f.mach = mach
         return f
 
+    def clone(self):
+        """Copy sharing the (geometry-only) heavy-ghost tables."""
+        h = ctypes.c_void_p()
+        _check(lib().rh3t_problem_clone(self._h, ctypes.byref(h)), "clone")
+        f = Field(None, None, None, _handle=h)
+        f.mach = getattr(self, "mach", None)
+        return f
+
+    def set_bc_state(self, kind, state):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        _check(lib().rh3t_problem_set_bc_state(self._h, kind, _dptr(st)), "set_bc_state")
+
     def cylinder_state(self, mach=None):
         u = np.zeros((self.grid.nx, self.grid.ny, 6))
         _check(lib().rh3t_problem_cylinder_state(self._h, self.mach if mach is None else mach, _dptr(u)))
@@ -346,6 +372,9 @@ class Solver:
     def set_arith(self, mode):
         _check(lib().rh3t_ctx_set_arith(self._h, mode))
 
+    def set_launch(self, bx, by, ghosts_per_block=1):
+        _check(lib().rh3t_ctx_set_launch(self._h, bx, by, ghosts_per_block), "set_launch")
+
     def set_state(self, u, time=0.0):
         u = np.ascontiguousarray(u, dtype=np.float64)
         if u.shape != (self.nx, self.ny, 6):
@@ -395,3 +424,88 @@ class Solver:
         _check(lib().rh3t_profile_read(self._h, 32, names, 1024, avg, cnt, ctypes.byref(n)))
         labels = names.raw.split(b"\0")[: n.value]
         return {labels[k].decode(): (avg[k], cnt[k]) for k in range(n.value)}
+
+
+def _profile_read(fn, h):
+    names = ctypes.create_string_buffer(1024)
+    avg = (ctypes.c_double * 32)()
+    cnt = (ctypes.c_int64 * 32)()
+    n = ctypes.c_int32()
+    _check(fn(h, 32, names, 1024, avg, cnt, ctypes.byref(n)))
+    labels = names.raw.split(b"\0")[: n.value]
+    return {labels[k].decode(): (avg[k], cnt[k]) for k in range(n.value)}
+
+
+def cylinder_freestream(mach, gamma=1.4):
+    """euler_to_three_energy of init_cylinder's freestream (src/init.rs:7-24,
+    157-175): rho = 1, p = 1, u = mach * sqrt(gamma)."""
+    rho, p = 1.0, 1.0
+    ux = mach * np.sqrt(gamma * p / rho)
+    e_total = p / (gamma - 1.0) + 0.5 * rho * (ux * ux)
+    return np.array([rho, rho * ux, 0.0, e_total / 3.0, e_total / 3.0, e_total / 3.0])
+
+
+class Batch:
+    """Many independent solves advanced together (rh3t_batch).  Each member
+    behaves as a Solver on its own Field; all members share one set of
+    batched kernels and one hipGraph per step."""
+
+    def __init__(self, fields, physics=None, device=0, arith=ARITH_FAST):
+        self.fields = list(fields)
+        self.physics = physics if physics is not None else Physics()
+        arr = (ctypes.c_void_p * len(self.fields))(*[f._h for f in self.fields])
+        self._h = ctypes.c_void_p()
+        st = lib().rh3t_batch_create(arr, len(self.fields), ctypes.byref(self.physics.c), device,
+                                     ctypes.byref(self._h))
+        if st != OK:
+            raise RH3TError(st, "rh3t_batch_create")
+        if arith != ARITH_FAST:
+            raise RH3TError(ERR_ARG, "batches run the FAST flux arithmetic")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rh3t_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def __len__(self):
+        return len(self.fields)
+
+    def set_state(self, k, u, time=0.0):
+        g = self.fields[k].grid
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != (g.nx, g.ny, 6):
+            raise ValueError("bad state shape")
+        _check(lib().rh3t_batch_set_state(self._h, k, _dptr(u), float(time)), "batch set_state")
+
+    def get_state(self, k):
+        g = self.fields[k].grid
+        u = np.zeros((g.nx, g.ny, 6))
+        t = ctypes.c_double()
+        _check(lib().rh3t_batch_get_state(self._h, k, _dptr(u), ctypes.byref(t)), "batch get_state")
+        return u, t.value
+
+    def advance(self, nsteps, cfl_factor=0.05):
+        times = np.zeros(len(self.fields))
+        st = lib().rh3t_batch_advance(self._h, int(nsteps), float(cfl_factor), _dptr(times))
+        if st != OK:
+            errs = []
+            for k in range(len(self.fields)):
+                e = c_error()
+                lib().rh3t_batch_last_error(self._h, k, ctypes.byref(e))
+                if e.kind:
+                    errs.append((k, e.message.decode(errors="replace")))
+            raise RH3TError(st, f"batch advance: {errs[:3]}", errs)
+        return times
+
+    def set_launch(self, bx, by, ghosts_per_block=1):
+        _check(lib().rh3t_batch_set_launch(self._h, bx, by, ghosts_per_block), "batch set_launch")
+
+    def profile(self, enable=True):
+        _check(lib().rh3t_batch_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        return _profile_read(lib().rh3t_batch_profile_read, self._h)

@@ -51,10 +51,10 @@ def build(verbose=False, resource_usage=False):
     os.makedirs(BUILD, exist_ok=True)
     inc = [f"-I{INCLUDE}", f"-I{CSRC}"]
     host_o = os.path.join(BUILD, "problem.o")
-    dev_o = os.path.join(BUILD, "rh3t_device.o")
+    dev_o = os.path.join(BUILD, "engine.o")
     _run(["g++", *CXX_FLAGS, *inc, "-c", os.path.join(CSRC, "problem.cpp"), "-o", host_o], verbose)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
-    out = _run([_hipcc(), *HIPCC_FLAGS, *inc, *extra, "-c", os.path.join(CSRC, "rh3t_device.hip"),
+    out = _run([_hipcc(), *HIPCC_FLAGS, *inc, *extra, "-c", os.path.join(CSRC, "engine.hip"),
                 "-o", dev_o], verbose)
     tmp = LIB + ".tmp"
     _run([_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", host_o, dev_o, "-o", tmp,

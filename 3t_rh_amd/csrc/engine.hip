@@ -1,0 +1,1559 @@
+// HIP (gfx950) implementation of l() and rk3_ssp() of colinhu123/3T_RH over
+// a ragged batch of independent solves, and the device half of the C ABI in
+// include/rh3t.h.  A plain solve (rh3t_ctx) is a batch of one.
+//
+// Per l() evaluation of the stage state S of every solve
+// (src/main.rs:135-305):
+//   k_ghost_cheap   Periodic / Reflective / Zeroth / Constant ghosts
+//                   (src/bc1.rs:471-636, 930-1037): gather from a static
+//                   source cell into the ghost's padded slot.
+//   k_ghost_heavy   FarField / Outflow / Wall ghosts (src/bc1.rs:609-928,
+//                   193-255): one wavefront per ghost; the E_r gather, the
+//                   characteristic projection and the S_r quadratic forms are
+//                   spread over the 64 lanes, reductions keep the reference's
+//                   summation order.
+//   k_derived       Derived::from_state on every fluid cell and ghost
+//                   (src/state.rs:241-275, src/main.rs:149-153).
+//   k_flux<0|1>     characteristic WENO5 interface flux in x / y
+//                   (src/main.rs:165-204 -> src/weno.rs:685-802), one
+//                   interface per lane (flux.hpp).
+//   k_assemble      flux differences + noncon x/y + source + diffusion
+//                   (src/main.rs:251-304) fused with the RK stage combination
+//                   and the admissibility check (src/main.rs:307-349).
+//   k_dt_partial / k_dt_final   calc_global_dt as a two-level fmin
+//                   reduction per solve (src/main.rs:375-408, src/dt.rs:5-34)
+//                   and dt = cfl_factor * dt_cfl (src/main.rs:461), on the
+//                   device.
+// Every grid-shaped kernel enumerates the tiles of all solves in one launch:
+// a per-kernel prefix table maps a (remapped) workgroup index to its solve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "flux.hpp"
+#include "kernels.hpp"
+#include "problem.hpp"
+
+using namespace rh3t_dev;
+
+namespace {
+
+constexpr int kStOff[6] = {0, 1, 5, 14, 30, 55};
+constexpr int kSOffD[6] = {0, 1, 17, 98, 354, 979};
+constexpr int kWOffD[6] = {0, 1, 9, 36, 100, 225};
+
+struct SolveDesc {
+  Geo g;
+  double* U[3];
+  double* D;
+  double* FX;
+  double* FY;
+  double* RHS;          // compact SoA nx*ny (l() output)
+  const uint8_t* pt;    // 1 fluid, 2 ghost
+  Scalars* sc;
+  const long* gpad;     // ghost slots by sorted ghost id
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; give
+// each XCD a contiguous run of tiles so that halo rows shared by
+// neighbouring tiles stay in one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  int full = (nb / 8) * 8;
+  if (b >= full) return b;
+  return (b % 8) * (nb / 8) + b / 8;
+}
+
+// start[0] = 0 <= ... <= start[n]: the solve owning tile t
+__device__ __forceinline__ int find_solve(const int* __restrict__ start, int n, int t) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (start[mid] <= t)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void raise_error(Scalars* sc, unsigned phase, unsigned kind, unsigned idx) {
+  atomicMin(&sc->err, err_key(sc->step, phase, kind, idx));
+}
+
+// ------------------------------------------------------------ ghosts
+
+struct CheapGhosts {
+  int n;
+  const int* sid;       // solve
+  const long* pad;      // destination slot
+  const long* src;      // source slot (reflective / zeroth / periodic)
+  const int* kind;      // rh3t_bc_kind
+  const double* normal; // 2 per ghost
+  const double* cstate; // 6 per ghost (Constant)
+  const int* gid;       // sorted ghost id within its solve
+};
+
+__global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict__ descs, CheapGhosts cg, int buf,
+                                                     unsigned phase) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cg.n) return;
+  const SolveDesc& sd = descs[cg.sid[k]];
+  double* S = sd.U[buf];
+  const long plane = sd.g.plane;
+  int kind = cg.kind[k];
+  long dst = cg.pad[k];
+  double v[6];
+  if (kind == RH3T_BC_CONSTANT) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) v[f] = cg.cstate[6L * k + f];
+  } else {
+    long s = cg.src[k];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) v[f] = S[f * plane + s];
+    if (kind == RH3T_BC_REFLECTIVE_WALL) {  // src/bc1.rs:1015-1036
+      double nx = cg.normal[2L * k], ny = cg.normal[2L * k + 1];
+      double mom_n = v[1] * nx + v[2] * ny;
+      v[1] = v[1] - 2.0 * mom_n * nx;
+      v[2] = v[2] - 2.0 * mom_n * ny;
+    }
+  }
+  bool ok = true;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    S[f * plane + dst] = v[f];
+    ok = ok && isfinite(v[f]);
+  }
+  if (!ok) raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)cg.gid[k]);
+}
+
+struct HeavyGhosts {
+  int n;
+  const int* sid;        // solve
+  const long* pad;       // destination slot
+  const int* kind;       // Wall / Outflow / FarField
+  const int* gid;        // sorted ghost id within its solve
+  const long* nearest;   // slot of GhostBC::nearest
+  const double* normal;  // 2 per ghost
+  const double* dist;    // signed distance D
+  const double* bcstate; // 6 per ghost (FarField u_inf)
+  const double* outflow; // 3 per ghost (p_inf, sigma, l_domain)
+  const int* tab;        // GhostBC table row of the ghost (shared by clones)
+  const double* d5;      // 5 per table row
+  const long* stencil;   // 55 slots per table row
+  const double* S;       // 979 per table row
+  const double* W;       // 225 per table row
+};
+
+__device__ __forceinline__ void matvec6(const double M[6][6], const double x[6], double y[6]) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) y[i] = dot6(M[i], x);
+}
+
+// solve6 (LAPACK dgesv): LU with partial pivoting, first maximal pivot.
+__device__ bool solve6(const double A0[6][6], const double b0[6], double x[6]) {
+  double A[6][6], b[6];
+  for (int i = 0; i < 6; ++i) {
+    b[i] = b0[i];
+    for (int j = 0; j < 6; ++j) A[i][j] = A0[i][j];
+  }
+  for (int c = 0; c < 6; ++c) {
+    int p = c;
+    double best = fabs(A[c][c]);
+    for (int r = c + 1; r < 6; ++r)
+      if (fabs(A[r][c]) > best) {
+        best = fabs(A[r][c]);
+        p = r;
+      }
+    if (best == 0.0) return false;
+    if (p != c) {
+      for (int q = 0; q < 6; ++q) {
+        double t = A[c][q];
+        A[c][q] = A[p][q];
+        A[p][q] = t;
+      }
+      double t = b[c];
+      b[c] = b[p];
+      b[p] = t;
+    }
+    for (int r = c + 1; r < 6; ++r) {
+      double f = A[r][c] / A[c][c];
+      A[r][c] = f;
+      for (int q = c + 1; q < 6; ++q) A[r][q] -= f * A[c][q];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int r = 5; r >= 0; --r) {
+    double s = b[r];
+    for (int q = r + 1; q < 6; ++q) s -= A[r][q] * x[q];
+    x[r] = s / A[r][r];
+  }
+  return true;
+}
+
+// One 64-lane wavefront per heavy ghost (blockDim = 64 x G ghosts).
+// set_ghost_point_value's fast path (src/bc1.rs:663-928) evaluates
+// weno_extrapolation_pre five times (k = 0..4) on identical data; the
+// projected data and the smoothness indicators beta_r do not depend on k, so
+// they are formed once here (same values, a fifth of the work).
+constexpr int kMaxGhostsPerBlock = 4;
+
+__global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph,
+                                                     int buf, unsigned phase) {
+  const int w = threadIdx.y;
+  const int h = blockIdx.x * blockDim.y + w;
+  const int lane = threadIdx.x;
+  __shared__ double s_data[kMaxGhostsPerBlock][6][56];  // L . u6 at the 55 E_r points
+  __shared__ double s_part[kMaxGhostsPerBlock][330];    // data[c][a] * (S_r data[c])[a]
+  __shared__ double s_vk[kMaxGhostsPerBlock][5][6][5];  // [k][c][r]
+  __shared__ double s_alpha[kMaxGhostsPerBlock][6][5];  // [c][r]
+  __shared__ double s_v[kMaxGhostsPerBlock][5][6];      // V^(k) per characteristic
+  const bool active = h < hg.n;
+  const SolveDesc& sd = descs[active ? hg.sid[h] : hg.sid[0]];
+  double* S = sd.U[buf];
+  const long plane = sd.g.plane;
+  const double nx = active ? hg.normal[2L * h] : 1.0, ny = active ? hg.normal[2L * h + 1] : 0.0;
+  // build_local_plain, src/bc1.rs:294-312: rotate the nearest state to the
+  // (n, t) frame; X-direction eigenvectors of the Roe average of that state
+  // with itself.
+  Eigen E;
+  {
+    double val[6] = {1.0, 0.0, 0.0, 1.0, 1.0, 1.0};
+    if (active) {
+      long q = hg.nearest[h];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) val[f] = S[f * plane + q];
+    }
+    double s1[6] = {val[0], val[1] * nx + val[2] * ny, -val[1] * ny + val[2] * nx, val[3], val[4], val[5]};
+    double d1[D_N];
+    derived_from_state(ph, s1, d1, false);
+    build_eigen<0>(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V],
+                   d1[D_EE], d1[D_EI], d1[D_ER], E);
+  }
+  const long row = active ? hg.tab[h] : 0;
+  if (active && lane < 55) {
+    long q = hg.stencil[55L * row + lane];
+    double s[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) s[f] = S[f * plane + q];
+    double u6[6] = {s[0], s[1] * nx + s[2] * ny, -s[1] * ny + s[2] * nx, s[3], s[4], s[5]};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s_data[w][c][lane] = dot6(E.L[c], u6);
+  }
+  __syncthreads();
+  if (active) {
+    const double* Sh = hg.S + 979L * row;
+    const double* Wh = hg.W + 225L * row;
+    for (int q = lane; q < 330; q += 64) {
+      int r = 0, base = 0;
+      while (q >= base + 6 * (r + 1) * (r + 1)) {
+        base += 6 * (r + 1) * (r + 1);
+        ++r;
+      }
+      int m = (r + 1) * (r + 1);
+      int c = (q - base) / m, a = (q - base) % m;
+      const double* row = Sh + kSOffD[r] + a * m;
+      const double* dc = &s_data[w][c][kStOff[r]];
+      double acc = 0.0;
+      for (int b = 0; b < m; ++b) acc += row[b] * dc[b];
+      s_part[w][q] = dc[a] * acc;
+    }
+    for (int q = lane; q < 150; q += 64) {
+      int k = q / 30, rc = q % 30, r = rc / 6, c = rc % 6;
+      int m = (r + 1) * (r + 1);
+      double vk = 0.0;
+      if (k <= r) {
+        const double* wv = Wh + kWOffD[r] + k * m;
+        const double* dc = &s_data[w][c][kStOff[r]];
+        for (int a = 0; a < m; ++a) vk += wv[a] * dc[a];
+      }
+      s_vk[w][k][c][r] = vk;
+    }
+  }
+  __syncthreads();
+  if (active && lane < 30) {  // beta_r and alpha_r = d_r / (eps + beta_r)^q
+    int r = lane / 6, c = lane % 6;
+    int m = (r + 1) * (r + 1);
+    int base = 0;
+    for (int rr = 0; rr < r; ++rr) base += 6 * (rr + 1) * (rr + 1);
+    const double* p = &s_part[w][base + c * m];
+    double beta = 0.0;
+    for (int a = 0; a < m; ++a) beta += p[a];
+    s_alpha[w][c][r] = hg.d5[5L * row + r] / pow(ph.eps + beta, ph.weno_q);
+  }
+  __syncthreads();
+  if (active && lane < 30) {  // V^(k)_c = sum_r (alpha_r / sum alpha) v_k,r
+    int k = lane / 6, c = lane % 6;
+    double asum = 0.0;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) asum += s_alpha[w][c][r];
+    double res = 0.0;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) res += (s_alpha[w][c][r] / asum) * s_vk[w][k][c][r];
+    s_v[w][k][c] = res;
+  }
+  __syncthreads();
+  if (!active || lane != 0) return;
+
+  double uk[5][6];
+  const int kind = hg.kind[h];
+  bool ok = true, solved = true;
+  if (kind == RH3T_BC_FARFIELD) {  // src/bc1.rs:829-924
+    const double* ui = hg.bcstate + 6L * h;
+    double uloc[6] = {ui[0], ui[1] * nx + ui[2] * ny, -ui[1] * ny + ui[2] * nx, ui[3], ui[4], ui[5]};
+    double vinf[6];
+    matvec6(E.L, uloc, vinf);
+    double rhs[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) rhs[m] = E.lam[m] < 0.0 ? vinf[m] : s_v[w][0][m];
+    matvec6(E.R, rhs, uk[0]);
+    for (int k = 1; k <= 4; ++k) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m) rhs[m] = E.lam[m] < 0.0 ? 0.0 : s_v[w][k][m];
+      matvec6(E.R, rhs, uk[k]);
+    }
+  } else if (kind == RH3T_BC_OUTFLOW) {  // src/bc1.rs:760-828
+    const double* of = hg.outflow + 3L * h;
+    matvec6(E.R, s_v[w][0], uk[0]);
+    double rho0 = uk[0][0];
+    double un0 = uk[0][1] / rho0, ut0 = uk[0][2] / rho0;
+    double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
+    double ee1 = uk[0][3] / rho0 - w2 / 6.0;
+    double ei1 = uk[0][4] / rho0 - w2 / 6.0;
+    double er1 = uk[0][5] / rho0 - w2 / 6.0;
+    double p0 = rho0 * (ph.ge * ee1 + ph.gi * ei1 + ph.gr * er1);
+    double a2 = ph.gamma_e * ph.ge * ee1 + ph.gamma_i * ph.gi * ei1 + ph.gamma_r * ph.gr * er1;
+    double a = sqrt(a2);
+    double lambda1 = un0 - a;
+    double rhs1[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) rhs1[m] = s_v[w][1][m];
+    if (lambda1 < 0.0) {
+      double ml = fabs(un0 / a);
+      double l1 = of[1] * (1.0 - sq(ml)) * a / of[2] * (p0 - of[0]);
+      rhs1[0] = l1 / lambda1;
+    }
+    matvec6(E.R, rhs1, uk[1]);
+    for (int k = 2; k <= 4; ++k) matvec6(E.R, s_v[w][k], uk[k]);
+  } else {  // Wall-ILW, src/bc1.rs:691-756
+    double A[6][6], rhs[6];
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) A[i][j] = E.L[i][j];
+    for (int c = 0; c < 6; ++c) A[0][c] = c == 1 ? 1.0 : 0.0;
+    for (int m = 0; m < 6; ++m) rhs[m] = s_v[w][0][m];
+    rhs[0] = 0.0;
+    solved = solve6(A, rhs, uk[0]);
+    double u1 = uk[0][0], u3 = uk[0][2];
+    A[0][0] = -ph.gt * sq(u3) / (6.0 * sq(u1));  // build_ilw_row1, src/bc1.rs:448-465
+    A[0][1] = 0.0;
+    A[0][2] = ph.gt * u3 / (6.0 * u1);
+    A[0][3] = -ph.ge;
+    A[0][4] = -ph.gi;
+    A[0][5] = -ph.gr;
+    for (int m = 0; m < 6; ++m) rhs[m] = s_v[w][1][m];
+    rhs[0] = 0.0;
+    solved = solved && solve6(A, rhs, uk[1]);
+    for (int k = 2; k <= 4; ++k) matvec6(E.R, s_v[w][k], uk[k]);
+  }
+  // Taylor expansion to the ghost point, rotation back to (x, y)
+  double ug[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) ug[c] = uk[0][c];
+  double coef = 1.0;
+  const double dist = hg.dist[h];
+  for (int k = 1; k <= 4; ++k) {
+    coef *= dist / (double)k;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) ug[c] = ug[c] + coef * uk[k][c];
+  }
+  double out[6] = {ug[0], ug[1] * nx - ug[2] * ny, ug[1] * ny + ug[2] * nx, ug[3], ug[4], ug[5]};
+  long dst = hg.pad[h];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    S[f * plane + dst] = out[f];
+    ok = ok && isfinite(out[f]);
+  }
+  if (!solved)
+    raise_error(sd.sc, phase, RH3T_ERR_SINGULAR, (unsigned)hg.gid[h]);
+  else if (!ok)
+    raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
+}
+
+// ----------------------------------------------------------- derived
+
+__global__ __launch_bounds__(256) void k_derived(const SolveDesc* __restrict__ descs, const int* __restrict__ tiles,
+                                                 int nsolve, Phys ph, int buf, int nplanes) {
+  const int s = find_solve(tiles, nsolve, blockIdx.x);
+  const SolveDesc& sd = descs[s];
+  const long p = (long)(blockIdx.x - tiles[s]) * blockDim.x + threadIdx.x;
+  if (p >= sd.g.plane || sd.pt[p] == 0) return;
+  const double* S = sd.U[buf];
+  double st[6], d[D_N];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) st[f] = S[f * sd.g.plane + p];
+  derived_from_state(ph, st, d, nplanes > 8);
+#pragma unroll
+  for (int f = 0; f < D_N; ++f)
+    if (f < nplanes) sd.D[f * sd.g.plane + p] = d[f];
+}
+
+// ------------------------------------------------------- WENO flux
+
+// Interface flux of one direction for every solve.  DIR 0: interface i-1/2
+// of row j, i in [0, nx], stencil points i-3..i+2.  DIR 1: interface j-1/2
+// of column i, j in [0, ny].  Output planes are i-major: x at i*ny + j, y at
+// i*(ny+1) + j.  Workgroup shape (bx, by) is a launch parameter.
+#ifndef RH3T_FLUX_WAVES
+#define RH3T_FLUX_WAVES 2
+#endif
+template <int DIR, bool FAST>
+__global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* __restrict__ descs,
+                                                               const int* __restrict__ tiles, int nsolve,
+                                                               EigConst K, double eps, int buf) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int s = find_solve(tiles, nsolve, t);
+  const SolveDesc& sd = descs[s];
+  const Geo g = sd.g;
+  const int ni = DIR == 0 ? g.nx + 1 : g.nx;
+  const int nj = DIR == 0 ? g.ny : g.ny + 1;
+  const int tiles_j = (nj + blockDim.x - 1) / blockDim.x;
+  const int lt = t - tiles[s];
+  const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
+  const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
+  if (i >= ni || j >= nj) return;
+  const double* __restrict__ S = sd.U[buf];
+  const double* __restrict__ D = sd.D;
+  double* __restrict__ F = DIR == 0 ? sd.FX : sd.FY;
+  const long fpl = DIR == 0 ? (long)(g.nx + 1) * g.ny : (long)g.nx * (g.ny + 1);
+  const long fidx = DIR == 0 ? (long)i * g.ny + j : (long)i * (g.ny + 1) + j;
+  const long c0 = pad_index(g, i, j);
+  const long step = DIR == 0 ? g.PY : 1;
+  if (sd.pt[c0] != 1 && sd.pt[c0 - step] != 1) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) F[f * fpl + fidx] = 0.0;
+    return;
+  }
+  // 6-point stencils of U and of the physical flux (flux_from_derived,
+  // src/state.rs:56-85)
+  double U[6][6], Fl[6][6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const long p = c0 + (q - 3) * step;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) U[q][f] = S[f * g.plane + p];
+    const double u = D[D_U * g.plane + p], v = D[D_V * g.plane + p];
+    const double pe = D[D_PE * g.plane + p], pi = D[D_PI * g.plane + p], pr = D[D_PR * g.plane + p];
+    const double ptot = pe + pi + pr;
+    const double w = DIR == 0 ? u : v;
+    Fl[q][0] = U[q][0] * w;
+    Fl[q][1] = DIR == 0 ? U[q][0] * u * u + ptot : U[q][0] * u * v;
+    Fl[q][2] = DIR == 0 ? U[q][0] * u * v : U[q][0] * v * v + ptot;
+    Fl[q][3] = (U[q][3] + pe) * w;
+    Fl[q][4] = (U[q][4] + pi) * w;
+    Fl[q][5] = (U[q][5] + pr) * w;
+  }
+  const long p2 = c0 - step;
+  const double e3[3] = {D[D_EE * g.plane + c0], D[D_EI * g.plane + c0], D[D_ER * g.plane + c0]};
+  const double e2[3] = {D[D_EE * g.plane + p2], D[D_EI * g.plane + p2], D[D_ER * g.plane + p2]};
+  double out[6];
+  interface_flux<DIR, FAST>(K, eps, U, Fl, D[D_U * g.plane + c0], D[D_V * g.plane + c0], e3,
+                            D[D_U * g.plane + p2], D[D_V * g.plane + p2], e2, out);
+#pragma unroll
+  for (int f = 0; f < 6; ++f) F[f * fpl + fidx] = out[f];
+}
+
+// --------------------------------------------------------- assemble
+
+struct StageSpec {
+  int mode;        // 0: write rhs, 1..3: SSP-RK3 stage
+  int base, add, dst;  // buffer indices into SolveDesc::U
+  unsigned phase;
+};
+
+__global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ descs, const int* __restrict__ tiles,
+                                                  int nsolve, Phys ph, StageSpec sa) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int s = find_solve(tiles, nsolve, t);
+  const SolveDesc& sd = descs[s];
+  const Geo g = sd.g;
+  const int tiles_j = (g.ny + blockDim.x - 1) / blockDim.x;
+  const int lt = t - tiles[s];
+  const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
+  const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
+  if (i >= g.nx || j >= g.ny) return;
+  const long c0 = pad_index(g, i, j);
+  const long lin = (long)i * g.ny + j;
+  const long ncell = (long)g.nx * g.ny;
+  if (sd.pt[c0] != 1) {
+    if (sa.mode == 0)
+#pragma unroll
+      for (int f = 0; f < 6; ++f) sd.RHS[f * ncell + lin] = 0.0;
+    return;
+  }
+  const double* __restrict__ D = sd.D;
+  const double* __restrict__ FX = sd.FX;
+  const double* __restrict__ FY = sd.FY;
+  const long fxpl = (long)(g.nx + 1) * g.ny, fypl = (long)g.nx * (g.ny + 1);
+  const long ixl = (long)i * g.ny + j, ixr = ixl + g.ny;
+  const long iyb = (long)i * (g.ny + 1) + j, iyt = iyb + 1;
+  const double inv_dx = 1.0 / g.dx, inv_dy = 1.0 / g.dy;
+  double r[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const double fxt = -(FX[f * fxpl + ixr] - FX[f * fxpl + ixl]) * inv_dx;
+    const double fyt = -(FY[f * fypl + iyt] - FY[f * fypl + iyb]) * inv_dy;
+    r[f] = fxt + fyt;
+  }
+  // non-conservative terms, src/noncon.rs:288-332
+  double ncx[3], ncy[3];
+  {
+    double vel[9], pe[9], pi[9], pr[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const long p = c0 + (long)(k - 4) * g.PY;
+      vel[k] = D[D_U * g.plane + p];
+      pe[k] = D[D_PE * g.plane + p];
+      pi[k] = D[D_PI * g.plane + p];
+      pr[k] = D[D_PR * g.plane + p];
+    }
+    noncon(vel, pe, pi, pr, g.dx, ncx);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const long p = c0 + (k - 4);
+      vel[k] = D[D_V * g.plane + p];
+      pe[k] = D[D_PE * g.plane + p];
+      pi[k] = D[D_PI * g.plane + p];
+      pr[k] = D[D_PR * g.plane + p];
+    }
+    noncon(vel, pe, pi, pr, g.dy, ncy);
+  }
+#pragma unroll
+  for (int e = 0; e < 3; ++e) r[3 + e] = r[3 + e] + ncx[e] + ncy[e];
+  if (ph.source) {  // source, src/source.rs:7-21
+    const double te = D[D_TE * g.plane + c0], ti = D[D_TI * g.plane + c0], tr = D[D_TR * g.plane + c0];
+    r[3] = r[3] + (-ph.omega_ei * (te - ti) - ph.omega_er * (pow4(te) - pow4(tr)));
+    r[4] = r[4] + ph.omega_ei * (te - ti);
+    r[5] = r[5] + ph.omega_er * (pow4(te) - pow4(tr));
+  }
+  if (ph.diffusion) {  // src/main.rs:209-246, 288-295 with src/diffusion.rs:27-50
+    const double kap[3] = {ph.kappa_e, ph.kappa_i, ph.kappa_r};
+    const double cx = -1.0 / (g.dx * g.dx), cy = -1.0 / (g.dy * g.dy);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      double tx[7], ty[7];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        double a = D[(D_TE + e) * g.plane + c0 + (long)(k - 3) * g.PY];
+        double b = D[(D_TE + e) * g.plane + c0 + (k - 3)];
+        tx[k] = e == 2 ? pow4(a) : a;
+        ty[k] = e == 2 ? pow4(b) : b;
+      }
+      const double dl = diffusion_flux(kap[e], tx), dr = diffusion_flux(kap[e], tx + 1);
+      const double db = diffusion_flux(kap[e], ty), dt_ = diffusion_flux(kap[e], ty + 1);
+      r[3 + e] = r[3 + e] + ((-(dr - dl) * cx) + (-(dt_ - db) * cy));
+    }
+  }
+  if (sa.mode == 0) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) sd.RHS[f * ncell + lin] = r[f];
+    return;
+  }
+  // stage_update_rhs / stage_update_comb, src/main.rs:307-349
+  const double dt = sd.sc->dt;
+  const double* __restrict__ base = sd.U[sa.base];
+  const double* __restrict__ add = sd.U[sa.add];
+  double* dst = sd.U[sa.dst];
+  double v[6];
+  if (sa.mode == 1) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) v[f] = base[f * g.plane + c0] + r[f] * dt;
+  } else if (sa.mode == 2) {
+    const double coef = dt / 4.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) v[f] = base[f * g.plane + c0] * 0.75 + add[f * g.plane + c0] * 0.25 + r[f] * coef;
+  } else {
+    const double coef = 2.0 * dt / 3.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+      v[f] = base[f * g.plane + c0] * (1.0 / 3.0) + add[f * g.plane + c0] * (2.0 / 3.0) + r[f] * coef;
+  }
+#pragma unroll
+  for (int f = 0; f < 6; ++f) dst[f * g.plane + c0] = v[f];
+  if (!admissible(v)) raise_error(sd.sc, sa.phase, RH3T_ERR_NONPHYSICAL, (unsigned)lin);
+}
+
+// One thread per solve: copies the state of the first failure of `phase`
+// into that solve's scalars (runs right after the kernels that can fail).
+__global__ void k_capture(const SolveDesc* __restrict__ descs, int nsolve, unsigned phase, int buf) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsolve) return;
+  const SolveDesc& sd = descs[s];
+  Scalars* sc = sd.sc;
+  unsigned long long key = sc->err;
+  if (key == ~0ULL) return;
+  if (((key >> 44) & 0xFFFFF) != (sc->step & 0xFFFFF) || ((key >> 40) & 0xF) != phase) return;
+  unsigned idx = (unsigned)(key & 0xFFFFFFFFULL);
+  long p = (phase % 2 == 0) ? sd.gpad[idx] : pad_index(sd.g, idx / sd.g.ny, idx % sd.g.ny);
+  const double* b = sd.U[buf];
+  for (int f = 0; f < 6; ++f) sc->err_state[f] = b[f * sd.g.plane + p];
+}
+
+// ------------------------------------------------------------- dt
+
+__global__ __launch_bounds__(256) void k_dt_partial(const SolveDesc* __restrict__ descs,
+                                                    const int* __restrict__ tiles, int nsolve, Phys ph,
+                                                    double* __restrict__ partial) {
+  __shared__ double red[256];
+  const int s = find_solve(tiles, nsolve, blockIdx.x);
+  const SolveDesc& sd = descs[s];
+  const long lin = (long)(blockIdx.x - tiles[s]) * 256 + threadIdx.x;
+  double m = INFINITY;
+  if (lin < (long)sd.g.nx * sd.g.ny) {
+    long p = pad_index(sd.g, lin / sd.g.ny, lin % sd.g.ny);
+    if (sd.pt[p] == 1) {
+      const double* S = sd.U[0];
+      double st[6];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) st[f] = S[f * sd.g.plane + p];
+      m = local_dt(ph, st, sd.g.dx, sd.g.dy);
+    }
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// One workgroup per solve.  fmin is exact, so the two-level reduction equals
+// the serial f64::min of src/main.rs:402-403 whatever the order.
+__global__ __launch_bounds__(256) void k_dt_final(const SolveDesc* __restrict__ descs, const int* __restrict__ tiles,
+                                                  const double* __restrict__ partial, int advance) {
+  __shared__ double red[256];
+  const int s = blockIdx.x;
+  double m = INFINITY;
+  for (int k = tiles[s] + threadIdx.x; k < tiles[s + 1]; k += 256) m = fmin(m, partial[k]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    Scalars* sc = descs[s].sc;
+    double dt_cfl = red[0];
+    sc->dt_cfl = dt_cfl;
+    if (advance) {
+      sc->step = sc->step + 1;
+      double dt = sc->dt_fixed > 0.0 ? sc->dt_fixed : sc->cfl_factor * dt_cfl;
+      sc->dt = dt;
+      if (!(dt > 0.0)) atomicMin(&sc->err, err_key(sc->step, 1, RH3T_ERR_DT, 0));
+      sc->time = sc->time + dt;
+    }
+  }
+}
+
+__global__ void k_err_clean(const SolveDesc* __restrict__ descs, int nsolve) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsolve) descs[s].sc->err = ~0ULL;
+}
+
+// ----------------------------------------------------------- layout I/O
+
+// host AoS (i*ny+j, 6 doubles) <-> padded SoA on fluid cells of one solve
+__global__ __launch_bounds__(256) void k_aos_to_pad(const double* __restrict__ aos, SolveDesc sd) {
+  long lin = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lin >= (long)sd.g.nx * sd.g.ny) return;
+  long p = pad_index(sd.g, lin / sd.g.ny, lin % sd.g.ny);
+  if (sd.pt[p] != 1) return;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) sd.U[0][f * sd.g.plane + p] = aos[6 * lin + f];
+}
+
+__global__ __launch_bounds__(256) void k_pad_to_aos(SolveDesc sd, double* __restrict__ aos) {
+  long lin = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lin >= (long)sd.g.nx * sd.g.ny) return;
+  long p = pad_index(sd.g, lin / sd.g.ny, lin % sd.g.ny);
+  if (sd.pt[p] != 1) return;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) aos[6 * lin + f] = sd.U[0][f * sd.g.plane + p];
+}
+
+__global__ __launch_bounds__(256) void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos,
+                                                    long n) {
+  long lin = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lin >= n) return;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) aos[6 * lin + f] = soa[f * n + lin];
+}
+
+__global__ __launch_bounds__(256) void k_gather_ghosts(SolveDesc sd, int n, double* __restrict__ out) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) out[6L * k + f] = sd.U[0][f * sd.g.plane + sd.gpad[k]];
+}
+
+}  // namespace
+
+// ======================================================================
+// Engine: device state of a batch of solves
+// ======================================================================
+
+enum KernelId { K_GHOST_CHEAP, K_GHOST_HEAVY, K_DERIVED, K_FLUX_X, K_FLUX_Y, K_ASSEMBLE, K_DT_PARTIAL,
+                K_DT_FINAL, K_CAPTURE, K_COUNT };
+static const char* kKernelNames[K_COUNT] = {"ghost_cheap", "ghost_heavy", "derived", "flux_x", "flux_y",
+                                            "assemble", "dt_partial", "dt_final", "capture"};
+
+struct ProfRec {
+  int kernel;
+  hipEvent_t a, b;
+};
+
+struct SolveHost {
+  Geo g;
+  long n_ghost = 0;
+  std::vector<uint8_t> fluid;
+  std::vector<double> nonfluid;  // AoS values of non-fluid cells (kept as given)
+};
+
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Phys phys{};
+  EigConst eig{};
+  bool fast = true;
+  int nplanes = 8;
+  int nsolve = 0;
+  std::vector<SolveHost> solves;
+  std::vector<SolveDesc> h_desc;
+  SolveDesc* d_desc = nullptr;
+  Scalars* d_sc = nullptr;
+  CheapGhosts cheap{};
+  HeavyGhosts heavy{};
+
+  // launch shapes (autotuned): flux / assemble workgroup (bx, by), heavy
+  // ghosts per workgroup
+  int bx = 64, by = 4, ghosts_per_block = 1;
+  std::vector<int> t_fx, t_fy, t_as, t_der, t_dt;  // host prefix tables
+  int* d_tabs = nullptr;  // device copy: fx | fy | as | der | dt, each nsolve+1
+  double* d_partial = nullptr;
+  double* d_aos = nullptr;   // staging (largest solve)
+  double* d_gvals = nullptr;
+  std::vector<void*> allocs;
+
+  hipGraphExec_t graph = nullptr;
+  bool profiling = false;
+  std::vector<ProfRec> prof;
+  double prof_ms[K_COUNT] = {0};
+  long long prof_n[K_COUNT] = {0};
+
+  std::vector<rh3t_error> last;  // per solve
+  char message[256] = {0};
+
+  const int* tab(int k) const { return d_tabs + k * (nsolve + 1); }
+};
+
+struct rh3t_ctx {
+  Engine e;
+};
+
+struct rh3t_batch {
+  Engine e;
+};
+
+namespace {
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      std::snprintf(E.message, sizeof(E.message), "%s: %s", #expr, hipGetErrorString(e_)); \
+      return RH3T_ERR_HIP;                                                                \
+    }                                                                                     \
+  } while (0)
+
+template <class T>
+int dev_alloc(Engine& E, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    std::snprintf(E.message, sizeof(E.message), "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+    return RH3T_ERR_HIP;
+  }
+  E.allocs.push_back(q);
+  *p = (T*)q;
+  return RH3T_OK;
+}
+
+template <class T>
+int upload(Engine& E, T** p, const std::vector<T>& v) {
+  int st = dev_alloc(E, p, v.size());
+  if (st) return st;
+  if (!v.empty()) HIP_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return RH3T_OK;
+}
+
+struct Launcher {
+  Engine& E;
+  int kid;
+  hipEvent_t a = nullptr, b = nullptr;
+  Launcher(Engine& e, int k) : E(e), kid(k) {
+    if (E.profiling) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, E.stream);
+    }
+  }
+  ~Launcher() {
+    if (E.profiling) {
+      (void)hipEventRecord(b, E.stream);
+      E.prof.push_back(ProfRec{kid, a, b});
+    }
+  }
+};
+
+int total(const std::vector<int>& t) { return t.back(); }
+
+// prefix tables for the current launch shape
+int build_tables(Engine& E) {
+  const int n = E.nsolve;
+  E.t_fx.assign(n + 1, 0);
+  E.t_fy.assign(n + 1, 0);
+  E.t_as.assign(n + 1, 0);
+  E.t_der.assign(n + 1, 0);
+  E.t_dt.assign(n + 1, 0);
+  for (int s = 0; s < n; ++s) {
+    const Geo& g = E.solves[s].g;
+    auto tiles = [&](int ni, int nj) { return ((nj + E.bx - 1) / E.bx) * ((ni + E.by - 1) / E.by); };
+    E.t_fx[s + 1] = E.t_fx[s] + tiles(g.nx + 1, g.ny);
+    E.t_fy[s + 1] = E.t_fy[s] + tiles(g.nx, g.ny + 1);
+    E.t_as[s + 1] = E.t_as[s] + tiles(g.nx, g.ny);
+    E.t_der[s + 1] = E.t_der[s] + (int)((g.plane + 255) / 256);
+    E.t_dt[s + 1] = E.t_dt[s] + (int)(((long)g.nx * g.ny + 255) / 256);
+  }
+  std::vector<int> all;
+  for (auto* t : {&E.t_fx, &E.t_fy, &E.t_as, &E.t_der, &E.t_dt}) all.insert(all.end(), t->begin(), t->end());
+  if (!E.d_tabs) {
+    int st = dev_alloc(E, &E.d_tabs, all.size());
+    if (st) return st;
+  }
+  HIP_TRY(hipMemcpy(E.d_tabs, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (E.graph) {
+    (void)hipGraphExecDestroy(E.graph);
+    E.graph = nullptr;
+  }
+  return RH3T_OK;
+}
+
+void launch_ghosts(Engine& E, int buf, unsigned phase) {
+  if (E.cheap.n > 0) {
+    Launcher L(E, K_GHOST_CHEAP);
+    k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.cheap, buf, phase);
+  }
+  if (E.heavy.n > 0) {
+    Launcher L(E, K_GHOST_HEAVY);
+    int g = E.ghosts_per_block;
+    k_ghost_heavy<<<(E.heavy.n + g - 1) / g, dim3(64, g), 0, E.stream>>>(E.d_desc, E.heavy, E.phys, buf, phase);
+  }
+  Launcher L(E, K_CAPTURE);
+  k_capture<<<(E.nsolve + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.nsolve, phase, buf);
+}
+
+// ghosts -> derived -> x/y fluxes -> assemble (one l() + stage) on buffer src
+void launch_stage(Engine& E, int src, const StageSpec& sa) {
+  launch_ghosts(E, src, sa.mode == 0 ? 2u : (unsigned)(2 * sa.mode));
+  {
+    Launcher L(E, K_DERIVED);
+    k_derived<<<total(E.t_der), 256, 0, E.stream>>>(E.d_desc, E.tab(3), E.nsolve, E.phys, src, E.nplanes);
+  }
+  dim3 blk(E.bx, E.by);
+  {
+    Launcher L(E, K_FLUX_X);
+    if (E.fast)
+      k_flux<0, true><<<total(E.t_fx), blk, 0, E.stream>>>(E.d_desc, E.tab(0), E.nsolve, E.eig, E.phys.eps, src);
+    else
+      k_flux<0, false><<<total(E.t_fx), blk, 0, E.stream>>>(E.d_desc, E.tab(0), E.nsolve, E.eig, E.phys.eps, src);
+  }
+  {
+    Launcher L(E, K_FLUX_Y);
+    if (E.fast)
+      k_flux<1, true><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.eig, E.phys.eps, src);
+    else
+      k_flux<1, false><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.eig, E.phys.eps, src);
+  }
+  {
+    Launcher L(E, K_ASSEMBLE);
+    k_assemble<<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa);
+  }
+  if (sa.mode > 0) {
+    Launcher L(E, K_CAPTURE);
+    k_capture<<<(E.nsolve + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.nsolve, sa.phase, sa.dst);
+  }
+}
+
+void launch_dt(Engine& E, int advance) {
+  {
+    Launcher L(E, K_DT_PARTIAL);
+    k_dt_partial<<<total(E.t_dt), 256, 0, E.stream>>>(E.d_desc, E.tab(4), E.nsolve, E.phys, E.d_partial);
+  }
+  Launcher L(E, K_DT_FINAL);
+  k_dt_final<<<E.nsolve, 256, 0, E.stream>>>(E.d_desc, E.tab(4), E.d_partial, advance);
+}
+
+// rk3_ssp, src/main.rs:351-373, with the state kept in U0:
+//   stage 1: U1 = U0 + dt L(U0)
+//   stage 2: U2 = 3/4 U0 + 1/4 U1 + dt/4 L(U1)
+//   stage 3: U0 = 1/3 U0 + 2/3 U2 + 2dt/3 L(U2)   (in place: each cell
+//            reads its own U0 value before writing it)
+void launch_rk3(Engine& E) {
+  launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3});
+  launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5});
+  launch_stage(E, 2, StageSpec{3, 0, 2, 0, 7});
+}
+
+// Reads every solve's first-failure key; returns the first failing status.
+int check_errors(Engine& E) {
+  std::vector<Scalars> sc(E.nsolve);
+  HIP_TRY(hipMemcpyAsync(sc.data(), E.d_sc, sizeof(Scalars) * E.nsolve, hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  int first = RH3T_OK;
+  E.last.assign(E.nsolve, rh3t_error{});
+  for (int s = 0; s < E.nsolve; ++s) {
+    if (sc[s].err == ~0ULL) continue;
+    unsigned long long k = sc[s].err;
+    unsigned phase = (unsigned)((k >> 40) & 0xF), kind = (unsigned)((k >> 32) & 0xFF);
+    unsigned idx = (unsigned)(k & 0xFFFFFFFFULL);
+    rh3t_error& e = E.last[s];
+    e.kind = (int)kind;
+    e.stage = phase >= 2 ? (int)((phase - 2) / 2 + 1) : 0;
+    e.ghost_id = -1;
+    e.i = e.j = -1;
+    std::memcpy(e.state, sc[s].err_state, sizeof(e.state));
+    if (kind == RH3T_ERR_DT) {
+      std::snprintf(e.message, sizeof(e.message), "solve %d: non-positive dt at step %u", s, (unsigned)(k >> 44));
+    } else if (phase % 2 == 0) {
+      e.ghost_id = (int)idx;
+      std::snprintf(e.message, sizeof(e.message), "solve %d: NON-FINITE GHOST STATE ghost id %u", s, idx);
+    } else {
+      e.i = (int)(idx / E.solves[s].g.ny);
+      e.j = (int)(idx % E.solves[s].g.ny);
+      std::snprintf(e.message, sizeof(e.message), "solve %d: non-physical state at (%d, %d) in RK%d state", s, e.i,
+                    e.j, e.stage);
+    }
+    if (first == RH3T_OK) first = (int)kind;
+  }
+  return first;
+}
+
+int reset_errors(Engine& E) {
+  k_err_clean<<<(E.nsolve + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.nsolve);
+  HIP_TRY(hipGetLastError());
+  return RH3T_OK;
+}
+
+Phys make_phys(const rh3t_physics& p) {
+  Phys ph;
+  ph.gamma_e = p.gamma_e;
+  ph.gamma_i = p.gamma_i;
+  ph.gamma_r = p.gamma_r;
+  ph.gi = p.gamma_i - 1.0;
+  ph.ge = p.gamma_e - 1.0;
+  ph.gr = p.gamma_r - 1.0;
+  ph.gt = ph.gi + ph.ge + ph.gr;
+  ph.kappa_e = p.kappa_e;
+  ph.kappa_i = p.kappa_i;
+  ph.kappa_r = p.kappa_r;
+  ph.cve = p.cve;
+  ph.cvi = p.cvi;
+  ph.a_rad = p.a_rad;
+  ph.omega_ei = p.omega_ei;
+  ph.omega_er = p.omega_er;
+  ph.lambda = p.lambda;
+  ph.weno_q = p.weno_q;
+  ph.eps = p.eps;
+  ph.diffusion = (p.kappa_e != 0.0 || p.kappa_i != 0.0 || p.kappa_r != 0.0) ? 1 : 0;
+  ph.source = (p.omega_ei != 0.0 || p.omega_er != 0.0) ? 1 : 0;
+  return ph;
+}
+
+// Uniform prefixes of the eigensystem expressions, evaluated with the
+// reference's association (src/weno.rs:56-84): identical roundings to the
+// per-interface evaluation.
+EigConst make_eig(const Phys& ph) {
+  EigConst K;
+  const double g[3] = {ph.ge, ph.gi, ph.gr};
+  const double gam[3] = {ph.gamma_e, ph.gamma_i, ph.gamma_r};
+  const double gt = ph.gt;
+  for (int k = 0; k < 3; ++k) {
+    K.g[k] = g[k];
+    K.gam[k] = gam[k];
+    K.gamg[k] = gam[k] * g[k];
+    K.g_over_gt[k] = g[k] / gt;
+    K.g_gt[k] = g[k] * gt;
+    K.two_g_gt[k] = 2.0 * g[k] * gt;
+    K.six_g[k] = 6.0 * g[k];
+    K.gam_gt[k] = gam[k] * gt;
+    K.inv6g[k] = 1.0 / (6.0 * g[k]);
+    for (int j = 0; j < 3; ++j) {
+      K.num[k][j] = j == k ? -(g[k] * g[k]) : -g[k] * g[j];
+      K.coef[k][j] = j == k ? 6.0 * g[k] : 6.0 * g[k] * g[j];
+    }
+  }
+  K.h6[0] = 6.0 * ph.gi * ph.gr;
+  K.h6[1] = 6.0 * ph.ge * ph.gr;
+  K.h6[2] = 6.0 * ph.ge * ph.gi;
+  K.c36 = 36.0 * ph.ge * ph.gi * ph.gr;
+  K.m2gt = -2.0 * gt;
+  K.gt = gt;
+  K.inv_gt = 1.0 / gt;
+  return K;
+}
+
+void free_engine(Engine& E) {
+  (void)hipSetDevice(E.device);
+  if (E.graph) (void)hipGraphExecDestroy(E.graph);
+  for (auto& r : E.prof) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (void* p : E.allocs) (void)hipFree(p);
+  if (E.stream) (void)hipStreamDestroy(E.stream);
+  E.allocs.clear();
+}
+
+int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_physics& phys, int device) {
+  E.device = device;
+  E.phys = make_phys(phys);
+  E.eig = make_eig(E.phys);
+  E.nsolve = n;
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+  E.nplanes = (E.phys.diffusion || E.phys.source) ? D_N : 8;
+  E.solves.resize(n);
+  E.h_desc.resize(n);
+  E.last.assign(n, rh3t_error{});
+  int st;
+  if ((st = dev_alloc(E, &E.d_sc, (size_t)n))) return st;
+  {
+    std::vector<Scalars> sc(n);
+    for (auto& s : sc) {
+      std::memset(&s, 0, sizeof(s));
+      s.err = ~0ULL;
+      s.cfl_factor = 0.05;
+    }
+    HIP_TRY(hipMemcpy(E.d_sc, sc.data(), sizeof(Scalars) * n, hipMemcpyHostToDevice));
+  }
+  std::vector<int> c_sid, c_kind, c_gid, h_sid, h_kind, h_gid, h_tab;
+  std::vector<std::pair<const void*, int>> table_base;  // shared heavy table -> first row
+  std::vector<long> c_pad, c_src, h_pad, h_near, h_st;
+  std::vector<double> c_n, c_state, h_n, h_dist, h_state, h_of, h_d5, h_S, h_W;
+  long max_cells = 0, max_ghost = 1;
+  for (int s = 0; s < n; ++s) {
+    const rh3t::Problem& pb = *P[s];
+    SolveHost& sh = E.solves[s];
+    Geo& g = sh.g;
+    int row0 = -1;
+    for (auto& tb : table_base)
+      if (tb.first == (const void*)pb.heavy.get()) row0 = tb.second;
+    const bool new_tables = row0 < 0;
+    if (new_tables) {
+      row0 = (int)(h_d5.size() / 5);
+      table_base.push_back({(const void*)pb.heavy.get(), row0});
+    }
+    g.nx = pb.grid.nx;
+    g.ny = pb.grid.ny;
+    g.PY = ((g.ny + 8 + 15) / 16) * 16;
+    g.plane = (long)(g.nx + 8) * g.PY;
+    g.dx = pb.grid.dx;
+    g.dy = pb.grid.dy;
+    const long ncell = (long)g.nx * g.ny;
+    max_cells = std::max(max_cells, ncell);
+    sh.fluid = pb.fluid;
+    sh.nonfluid.assign(6 * (size_t)ncell, 0.0);
+    sh.n_ghost = (long)pb.ghosts.size();
+    max_ghost = std::max(max_ghost, sh.n_ghost);
+    std::vector<uint8_t> pt(g.plane, 0);
+    for (long i = 0; i < g.nx; ++i)
+      for (long j = 0; j < g.ny; ++j)
+        if (pb.fluid[i * g.ny + j]) pt[pad_index(g, i, j)] = 1;
+    std::vector<long> gpad(pb.ghosts.size());
+    for (size_t k = 0; k < pb.ghosts.size(); ++k) {
+      const auto& gh = pb.ghosts[k];
+      if (gh.i < -4 || gh.i >= g.nx + 4 || gh.j < -4 || gh.j >= g.ny + 4) {
+        std::snprintf(E.message, sizeof(E.message), "ghost outside the halo");
+        return RH3T_ERR_GEOMETRY;
+      }
+      if (gh.bc_kind == RH3T_BC_TIME_DEPENDENT) {
+        std::snprintf(E.message, sizeof(E.message), "TimeDependent boundary conditions are not part of this path");
+        return RH3T_ERR_UNSUPPORTED;
+      }
+      gpad[k] = pad_index(g, gh.i, gh.j);
+      pt[gpad[k]] = 2;
+      const auto& bc = pb.bc_list(gh.boundary)[gh.side_id];
+      if (gh.heavy_id < 0) {
+        c_sid.push_back(s);
+        c_pad.push_back(gpad[k]);
+        c_src.push_back(gh.src_i >= 0 ? pad_index(g, gh.src_i, gh.src_j) : 0);
+        c_kind.push_back(gh.bc_kind);
+        c_gid.push_back((int)k);
+        c_n.push_back(gh.project.normal.x);
+        c_n.push_back(gh.project.normal.y);
+        for (int f = 0; f < 6; ++f) c_state.push_back(bc.state[f]);
+      } else {
+        const auto& hb = (*pb.heavy)[gh.heavy_id];
+        h_sid.push_back(s);
+        h_tab.push_back(row0 + gh.heavy_id);
+        h_pad.push_back(gpad[k]);
+        h_kind.push_back(gh.bc_kind);
+        h_gid.push_back((int)k);
+        h_near.push_back(pad_index(g, hb.nearest_i, hb.nearest_j));
+        h_n.push_back(gh.project.normal.x);
+        h_n.push_back(gh.project.normal.y);
+        h_dist.push_back(gh.project.distance);
+        for (int f = 0; f < 6; ++f) h_state.push_back(bc.state[f]);
+        h_of.push_back(bc.p_inf);
+        h_of.push_back(bc.sigma);
+        h_of.push_back(bc.l_domain);
+      }
+    }
+    if (new_tables) {
+      for (const auto& hb : *pb.heavy) {
+        for (int q = 0; q < 5; ++q) h_d5.push_back(hb.d[q]);
+        for (int q = 0; q < 55; ++q) {
+          uint32_t l = hb.stencil[q];
+          h_st.push_back(pad_index(g, l / g.ny, l % g.ny));
+        }
+        h_S.insert(h_S.end(), hb.s, hb.s + 979);
+        h_W.insert(h_W.end(), hb.w, hb.w + 225);
+      }
+    }
+    SolveDesc& d = E.h_desc[s];
+    d.g = g;
+    uint8_t* dpt;
+    if ((st = upload(E, &dpt, pt))) return st;
+    d.pt = dpt;
+    long* dg;
+    if ((st = upload(E, &dg, gpad))) return st;
+    d.gpad = dg;
+    for (int b = 0; b < 3; ++b) {
+      if ((st = dev_alloc(E, &d.U[b], 6 * (size_t)g.plane))) return st;
+      HIP_TRY(hipMemset(d.U[b], 0, 6 * (size_t)g.plane * sizeof(double)));
+    }
+    if ((st = dev_alloc(E, &d.D, (size_t)E.nplanes * g.plane))) return st;
+    HIP_TRY(hipMemset(d.D, 0, (size_t)E.nplanes * g.plane * sizeof(double)));
+    if ((st = dev_alloc(E, &d.FX, 6 * (size_t)(g.nx + 1) * g.ny))) return st;
+    if ((st = dev_alloc(E, &d.FY, 6 * (size_t)g.nx * (g.ny + 1)))) return st;
+    if ((st = dev_alloc(E, &d.RHS, 6 * (size_t)ncell))) return st;
+    d.sc = E.d_sc + s;
+  }
+  if ((st = upload(E, &E.d_desc, E.h_desc))) return st;
+  if ((st = dev_alloc(E, &E.d_aos, 6 * (size_t)max_cells))) return st;
+  if ((st = dev_alloc(E, &E.d_gvals, 6 * (size_t)max_ghost))) return st;
+
+  E.cheap.n = (int)c_pad.size();
+  E.heavy.n = (int)h_pad.size();
+  long* lp;
+  int* ip;
+  double* dp;
+#define UP(vec, dst)                          \
+  if ((st = upload(E, &dst##_tmp, vec))) return st; \
+  dst = dst##_tmp;
+  int* ip_tmp;
+  long* lp_tmp;
+  double* dp_tmp;
+  (void)lp;
+  (void)ip;
+  (void)dp;
+  UP(c_sid, ip) E.cheap.sid = ip;
+  UP(c_pad, lp) E.cheap.pad = lp;
+  UP(c_src, lp) E.cheap.src = lp;
+  UP(c_kind, ip) E.cheap.kind = ip;
+  UP(c_gid, ip) E.cheap.gid = ip;
+  UP(c_n, dp) E.cheap.normal = dp;
+  UP(c_state, dp) E.cheap.cstate = dp;
+  UP(h_sid, ip) E.heavy.sid = ip;
+  UP(h_pad, lp) E.heavy.pad = lp;
+  UP(h_kind, ip) E.heavy.kind = ip;
+  UP(h_gid, ip) E.heavy.gid = ip;
+  UP(h_near, lp) E.heavy.nearest = lp;
+  UP(h_n, dp) E.heavy.normal = dp;
+  UP(h_dist, dp) E.heavy.dist = dp;
+  UP(h_state, dp) E.heavy.bcstate = dp;
+  UP(h_of, dp) E.heavy.outflow = dp;
+  UP(h_tab, ip) E.heavy.tab = ip;
+  UP(h_d5, dp) E.heavy.d5 = dp;
+  UP(h_st, lp) E.heavy.stencil = lp;
+  UP(h_S, dp) E.heavy.S = dp;
+  UP(h_W, dp) E.heavy.W = dp;
+#undef UP
+  if ((st = build_tables(E))) return st;
+  if ((st = dev_alloc(E, &E.d_partial, (size_t)std::max(1, total(E.t_dt))))) return st;
+  return RH3T_OK;
+}
+
+int set_state(Engine& E, int s, const double* u, double time) {
+  if (s < 0 || s >= E.nsolve || !u) return RH3T_ERR_ARG;
+  HIP_TRY(hipSetDevice(E.device));
+  SolveHost& sh = E.solves[s];
+  const long ncell = (long)sh.g.nx * sh.g.ny;
+  for (long l = 0; l < ncell; ++l)
+    if (!sh.fluid[l])
+      for (int f = 0; f < 6; ++f) sh.nonfluid[6 * l + f] = u[6 * l + f];
+  HIP_TRY(hipMemcpyAsync(E.d_aos, u, 6 * ncell * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  k_aos_to_pad<<<(unsigned)((ncell + 255) / 256), 256, 0, E.stream>>>(E.d_aos, E.h_desc[s]);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(&E.d_sc[s].time, &time, sizeof(double), hipMemcpyHostToDevice, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  return RH3T_OK;
+}
+
+int get_state(Engine& E, int s, double* u, double* time) {
+  if (s < 0 || s >= E.nsolve || !u) return RH3T_ERR_ARG;
+  HIP_TRY(hipSetDevice(E.device));
+  SolveHost& sh = E.solves[s];
+  const long ncell = (long)sh.g.nx * sh.g.ny;
+  k_pad_to_aos<<<(unsigned)((ncell + 255) / 256), 256, 0, E.stream>>>(E.h_desc[s], E.d_aos);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(u, E.d_aos, 6 * ncell * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  double t = 0.0;
+  HIP_TRY(hipMemcpyAsync(&t, &E.d_sc[s].time, sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  for (long l = 0; l < ncell; ++l)
+    if (!sh.fluid[l])
+      for (int f = 0; f < 6; ++f) u[6 * l + f] = sh.nonfluid[6 * l + f];
+  if (time) *time = t;
+  return RH3T_OK;
+}
+
+int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
+  if (nsteps < 0) return RH3T_ERR_ARG;
+  HIP_TRY(hipSetDevice(E.device));
+  // read-modify-write of the scalars on the solver's stream: CFL rule, fixed
+  // dt, and a clean first-failure key
+  std::vector<Scalars> sc(E.nsolve);
+  HIP_TRY(hipMemcpyAsync(sc.data(), E.d_sc, sizeof(Scalars) * E.nsolve, hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  for (auto& s : sc) {
+    s.cfl_factor = cfl_factor;
+    s.dt_fixed = dt_fixed;
+    s.err = ~0ULL;
+  }
+  HIP_TRY(hipMemcpyAsync(E.d_sc, sc.data(), sizeof(Scalars) * E.nsolve, hipMemcpyHostToDevice, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  if (E.profiling) {
+    for (int s = 0; s < nsteps; ++s) {
+      launch_dt(E, 1);
+      launch_rk3(E);
+    }
+    HIP_TRY(hipGetLastError());
+  } else {
+    if (!E.graph) {
+      hipGraph_t gr;
+      HIP_TRY(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+      launch_dt(E, 1);
+      launch_rk3(E);
+      HIP_TRY(hipStreamEndCapture(E.stream, &gr));
+      hipError_t e = hipGraphInstantiate(&E.graph, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (e != hipSuccess) {
+        E.graph = nullptr;
+        std::snprintf(E.message, sizeof(E.message), "graph instantiate: %s", hipGetErrorString(e));
+        return RH3T_ERR_HIP;
+      }
+    }
+    for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(E.graph, E.stream));
+  }
+  return check_errors(E);
+}
+
+int profile_enable(Engine& E, int enable) {
+  HIP_TRY(hipSetDevice(E.device));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  for (auto& r : E.prof) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  E.prof.clear();
+  for (int k = 0; k < K_COUNT; ++k) {
+    E.prof_ms[k] = 0;
+    E.prof_n[k] = 0;
+  }
+  E.profiling = enable != 0;
+  return RH3T_OK;
+}
+
+int profile_read(Engine& E, int max_kernels, char* names, int names_cap, double* avg_ms, int64_t* launches,
+                 int32_t* n_kernels) {
+  if (!n_kernels) return RH3T_ERR_ARG;
+  HIP_TRY(hipSetDevice(E.device));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  for (auto& r : E.prof) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+    E.prof_ms[r.kernel] += ms;
+    E.prof_n[r.kernel] += 1;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  E.prof.clear();
+  int n = std::min<int>(K_COUNT, max_kernels);
+  std::string s;
+  for (int k = 0; k < n; ++k) {
+    s += kKernelNames[k];
+    s.push_back('\0');
+    if (avg_ms) avg_ms[k] = E.prof_n[k] ? E.prof_ms[k] / (double)E.prof_n[k] : 0.0;
+    if (launches) launches[k] = E.prof_n[k];
+  }
+  if (names && names_cap > 0) std::memcpy(names, s.data(), std::min<int>((int)s.size(), names_cap));
+  *n_kernels = n;
+  return RH3T_OK;
+}
+
+int set_launch(Engine& E, int bx, int by, int gpb) {
+  if (bx <= 0 || by <= 0 || bx * by > 256 || bx % 16 != 0 || gpb < 1 || gpb > kMaxGhostsPerBlock) return RH3T_ERR_ARG;
+  E.bx = bx;
+  E.by = by;
+  E.ghosts_per_block = gpb;
+  return build_tables(E);
+}
+
+rh3t_physics phys_or_default(const rh3t_physics* p) {
+  rh3t_physics q;
+  if (p)
+    q = *p;
+  else
+    rh3t_physics_default(&q);
+  return q;
+}
+
+}  // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+
+extern "C" {
+
+int rh3t_device_count(int32_t* n) {
+  if (!n) return RH3T_ERR_ARG;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return RH3T_OK;
+}
+
+static int check_device(int32_t device) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RH3T_ERR_NO_DEVICE;
+  if (device < 0 || device >= ndev) return RH3T_ERR_ARG;
+  return RH3T_OK;
+}
+
+int rh3t_ctx_create(const rh3t_problem* pb, const rh3t_physics* phys, int32_t device, rh3t_ctx** out) {
+  if (!pb || !out || !pb->p.ghosts_built) return RH3T_ERR_ARG;
+  *out = nullptr;
+  int st = check_device(device);
+  if (st) return st;
+  auto* ctx = new rh3t_ctx();
+  const rh3t::Problem* P = &pb->p;
+  st = build_engine(ctx->e, &P, 1, phys_or_default(phys), device);
+  if (st != RH3T_OK) {
+    free_engine(ctx->e);
+    delete ctx;
+    return st;
+  }
+  *out = ctx;
+  return RH3T_OK;
+}
+
+void rh3t_ctx_destroy(rh3t_ctx* ctx) {
+  if (!ctx) return;
+  free_engine(ctx->e);
+  delete ctx;
+}
+
+int rh3t_set_state(rh3t_ctx* ctx, const double* u, double time) {
+  if (!ctx) return RH3T_ERR_ARG;
+  return set_state(ctx->e, 0, u, time);
+}
+
+int rh3t_get_state(rh3t_ctx* ctx, double* u, double* time) {
+  if (!ctx) return RH3T_ERR_ARG;
+  return get_state(ctx->e, 0, u, time);
+}
+
+int rh3t_l(rh3t_ctx* ctx, double* rhs_aos, double* ghost_vals) {
+  if (!ctx || !rhs_aos) return RH3T_ERR_ARG;
+  Engine& E = ctx->e;
+  HIP_TRY(hipSetDevice(E.device));
+  int st = reset_errors(E);
+  if (st) return st;
+  launch_stage(E, 0, StageSpec{0, 0, 0, 0, 2});
+  HIP_TRY(hipGetLastError());
+  const SolveDesc& d = E.h_desc[0];
+  const long ncell = (long)d.g.nx * d.g.ny;
+  k_soa_to_aos<<<(unsigned)((ncell + 255) / 256), 256, 0, E.stream>>>(d.RHS, E.d_aos, ncell);
+  HIP_TRY(hipMemcpyAsync(rhs_aos, E.d_aos, 6 * ncell * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  const long ng = E.solves[0].n_ghost;
+  if (ghost_vals && ng > 0) {
+    k_gather_ghosts<<<(unsigned)((ng + 255) / 256), 256, 0, E.stream>>>(d, (int)ng, E.d_gvals);
+    HIP_TRY(hipMemcpyAsync(ghost_vals, E.d_gvals, 6 * ng * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  }
+  return check_errors(E);
+}
+
+int rh3t_rk3_step(rh3t_ctx* ctx, double dt) {
+  if (!ctx) return RH3T_ERR_ARG;
+  if (!(dt > 0.0)) return RH3T_ERR_DT;
+  Engine& E = ctx->e;
+  HIP_TRY(hipSetDevice(E.device));
+  int st = reset_errors(E);
+  if (st) return st;
+  HIP_TRY(hipMemcpyAsync(&E.d_sc[0].dt, &dt, sizeof(double), hipMemcpyHostToDevice, E.stream));
+  launch_rk3(E);
+  HIP_TRY(hipGetLastError());
+  // Field.time advances by dt (u3.time = u.time + dt, src/main.rs:370)
+  double t = 0.0;
+  HIP_TRY(hipMemcpyAsync(&t, &E.d_sc[0].time, sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  t = t + dt;
+  HIP_TRY(hipMemcpyAsync(&E.d_sc[0].time, &t, sizeof(double), hipMemcpyHostToDevice, E.stream));
+  return check_errors(E);
+}
+
+int rh3t_calc_global_dt(rh3t_ctx* ctx, double* dt_cfl) {
+  if (!ctx || !dt_cfl) return RH3T_ERR_ARG;
+  Engine& E = ctx->e;
+  HIP_TRY(hipSetDevice(E.device));
+  launch_dt(E, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(dt_cfl, &E.d_sc[0].dt_cfl, sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  return RH3T_OK;
+}
+
+int rh3t_advance(rh3t_ctx* ctx, int32_t nsteps, double cfl_factor, double dt_fixed, double* time) {
+  if (!ctx) return RH3T_ERR_ARG;
+  Engine& E = ctx->e;
+  int st = advance(E, nsteps, cfl_factor, dt_fixed);
+  if (time) {
+    HIP_TRY(hipMemcpy(time, &E.d_sc[0].time, sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return st;
+}
+
+int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode) {
+  if (!ctx || (mode != RH3T_ARITH_EXACT && mode != RH3T_ARITH_FAST)) return RH3T_ERR_ARG;
+  bool f = mode == RH3T_ARITH_FAST;
+  if (f != ctx->e.fast && ctx->e.graph) {
+    (void)hipGraphExecDestroy(ctx->e.graph);
+    ctx->e.graph = nullptr;
+  }
+  ctx->e.fast = f;
+  return RH3T_OK;
+}
+
+int rh3t_ctx_set_launch(rh3t_ctx* ctx, int32_t bx, int32_t by, int32_t ghosts_per_block) {
+  if (!ctx) return RH3T_ERR_ARG;
+  return set_launch(ctx->e, bx, by, ghosts_per_block);
+}
+
+int rh3t_last_error(const rh3t_ctx* ctx, rh3t_error* err) {
+  if (!ctx || !err) return RH3T_ERR_ARG;
+  *err = ctx->e.last.empty() ? rh3t_error{} : ctx->e.last[0];
+  if (err->kind == 0 && ctx->e.message[0]) std::memcpy(err->message, ctx->e.message, sizeof(err->message));
+  return RH3T_OK;
+}
+
+int rh3t_profile(rh3t_ctx* ctx, int32_t enable) {
+  if (!ctx) return RH3T_ERR_ARG;
+  return profile_enable(ctx->e, enable);
+}
+
+int rh3t_profile_read(rh3t_ctx* ctx, int32_t max_kernels, char* names, int32_t names_cap, double* avg_ms,
+                      int64_t* launches, int32_t* n_kernels) {
+  if (!ctx) return RH3T_ERR_ARG;
+  return profile_read(ctx->e, max_kernels, names, names_cap, avg_ms, launches, n_kernels);
+}
+
+// ------------------------------------------------------------- batch
+
+int rh3t_batch_create(const rh3t_problem* const* problems, int32_t n, const rh3t_physics* phys, int32_t device,
+                      rh3t_batch** out) {
+  if (!problems || n <= 0 || !out) return RH3T_ERR_ARG;
+  for (int s = 0; s < n; ++s)
+    if (!problems[s] || !problems[s]->p.ghosts_built) return RH3T_ERR_ARG;
+  *out = nullptr;
+  int st = check_device(device);
+  if (st) return st;
+  std::vector<const rh3t::Problem*> P(n);
+  for (int s = 0; s < n; ++s) P[s] = &problems[s]->p;
+  auto* b = new rh3t_batch();
+  st = build_engine(b->e, P.data(), n, phys_or_default(phys), device);
+  if (st != RH3T_OK) {
+    free_engine(b->e);
+    delete b;
+    return st;
+  }
+  *out = b;
+  return RH3T_OK;
+}
+
+void rh3t_batch_destroy(rh3t_batch* b) {
+  if (!b) return;
+  free_engine(b->e);
+  delete b;
+}
+
+int rh3t_batch_set_state(rh3t_batch* b, int32_t solve, const double* u, double time) {
+  if (!b) return RH3T_ERR_ARG;
+  return set_state(b->e, solve, u, time);
+}
+
+int rh3t_batch_get_state(rh3t_batch* b, int32_t solve, double* u, double* time) {
+  if (!b) return RH3T_ERR_ARG;
+  return get_state(b->e, solve, u, time);
+}
+
+int rh3t_batch_advance(rh3t_batch* b, int32_t nsteps, double cfl_factor, double* times) {
+  if (!b) return RH3T_ERR_ARG;
+  Engine& E = b->e;
+  int st = advance(E, nsteps, cfl_factor, 0.0);
+  if (times) {
+    std::vector<Scalars> sc(E.nsolve);
+    HIP_TRY(hipMemcpy(sc.data(), E.d_sc, sizeof(Scalars) * E.nsolve, hipMemcpyDeviceToHost));
+    for (int s = 0; s < E.nsolve; ++s) times[s] = sc[s].time;
+  }
+  return st;
+}
+
+int rh3t_batch_set_launch(rh3t_batch* b, int32_t bx, int32_t by, int32_t ghosts_per_block) {
+  if (!b) return RH3T_ERR_ARG;
+  return set_launch(b->e, bx, by, ghosts_per_block);
+}
+
+int rh3t_batch_last_error(const rh3t_batch* b, int32_t solve, rh3t_error* err) {
+  if (!b || !err || solve < 0 || solve >= b->e.nsolve) return RH3T_ERR_ARG;
+  *err = b->e.last.empty() ? rh3t_error{} : b->e.last[solve];
+  return RH3T_OK;
+}
+
+int rh3t_batch_profile(rh3t_batch* b, int32_t enable) {
+  if (!b) return RH3T_ERR_ARG;
+  return profile_enable(b->e, enable);
+}
+
+int rh3t_batch_profile_read(rh3t_batch* b, int32_t max_kernels, char* names, int32_t names_cap, double* avg_ms,
+                            int64_t* launches, int32_t* n_kernels) {
+  if (!b) return RH3T_ERR_ARG;
+  return profile_read(b->e, max_kernels, names, names_cap, avg_ms, launches, n_kernels);
+}
+
+}  // extern "C"

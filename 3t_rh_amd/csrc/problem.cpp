@@ -716,7 +716,7 @@ void Problem::build_ghosts() {  // src/ghost.rs:50-118, 378-478
   int64_t np_y = (int64_t)std::round((by1 - by0) / grid.dy);
 
   ghosts.clear();
-  heavy.clear();
+  heavy = std::make_shared<std::vector<GhostBC>>();
   ghosts.reserve(idx.size());
   for (const auto& ij : idx) {
     GhostInfo g;
@@ -749,8 +749,8 @@ void Problem::build_ghosts() {  // src/ghost.rs:50-118, 378-478
       case RH3T_BC_WALL:
       case RH3T_BC_OUTFLOW:
       case RH3T_BC_FARFIELD:
-        g.heavy_id = (int32_t)heavy.size();
-        heavy.push_back(precompute_ghost_bc(*this, g.project, beta_forms));
+        g.heavy_id = (int32_t)heavy->size();
+        heavy->push_back(precompute_ghost_bc(*this, g.project, beta_forms));
         break;
       case RH3T_BC_REFLECTIVE_WALL:
       case RH3T_BC_ZEROTH_ORDER:
@@ -961,6 +961,29 @@ int rh3t_problem_init_cylinder(int32_t nx, double mach, rh3t_problem** out) {
   });
 }
 
+int rh3t_problem_clone(const rh3t_problem* src, rh3t_problem** out) {
+  if (!src || !out) return RH3T_ERR_ARG;
+  *out = nullptr;
+  return guarded([&]() {
+    auto* pb = new rh3t_problem();
+    pb->p = src->p;  // ghost tables shared through the heavy shared_ptr
+    *out = pb;
+    return (int)RH3T_OK;
+  });
+}
+
+int rh3t_problem_set_bc_state(rh3t_problem* pb, int32_t kind, const double state[6]) {
+  if (!pb || !state || (kind != RH3T_BC_FARFIELD && kind != RH3T_BC_CONSTANT)) return RH3T_ERR_ARG;
+  int n = 0;
+  for (auto* l : {&pb->p.bc_outer, &pb->p.bc_inner})
+    for (auto& b : *l)
+      if (b.kind == kind) {
+        std::memcpy(b.state, state, sizeof(double) * 6);
+        ++n;
+      }
+  return n > 0 ? RH3T_OK : RH3T_ERR_ARG;
+}
+
 int rh3t_problem_cylinder_state(const rh3t_problem* pb, double mach, double* u) {
   if (!pb || !u) return RH3T_ERR_ARG;
   double gamma = 1.4, rho = 1.0, p = 1.0;
@@ -987,7 +1010,7 @@ int rh3t_problem_counts(const rh3t_problem* pb, int64_t* n_fluid, int64_t* n_gho
   for (uint8_t f : pb->p.fluid) nf += f;
   if (n_fluid) *n_fluid = nf;
   if (n_ghost) *n_ghost = (int64_t)pb->p.ghosts.size();
-  if (n_heavy) *n_heavy = (int64_t)pb->p.heavy.size();
+  if (n_heavy) *n_heavy = (int64_t)pb->p.heavy->size();
   return pb->p.ghosts_built ? RH3T_OK : RH3T_ERR_ARG;
 }
 
@@ -1022,8 +1045,8 @@ int rh3t_problem_ghosts(const rh3t_problem* pb, rh3t_ghost* out) {
 }
 
 int rh3t_problem_ghost_bc(const rh3t_problem* pb, int32_t heavy_id, rh3t_ghost_bc* out) {
-  if (!pb || !out || heavy_id < 0 || (size_t)heavy_id >= pb->p.heavy.size()) return RH3T_ERR_ARG;
-  const auto& h = pb->p.heavy[heavy_id];
+  if (!pb || !out || heavy_id < 0 || (size_t)heavy_id >= pb->p.heavy->size()) return RH3T_ERR_ARG;
+  const auto& h = (*pb->p.heavy)[heavy_id];
   out->nearest_i = h.nearest_i;
   out->nearest_j = h.nearest_j;
   out->normal[0] = h.normal.x;

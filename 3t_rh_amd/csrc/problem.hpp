@@ -8,6 +8,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -100,7 +101,10 @@ struct Problem {
 
   bool ghosts_built = false;
   std::vector<GhostInfo> ghosts;  // sorted by (i, j)
-  std::vector<GhostBC> heavy;
+  // Heavy-ghost tables depend only on geometry and grid, never on BC
+  // states: clones (e.g. the same cylinder at another Mach number) share
+  // them, on the host and on the device.
+  std::shared_ptr<std::vector<GhostBC>> heavy = std::make_shared<std::vector<GhostBC>>();
 
   // GridInfo helpers (src/field1.rs:43-73)
   bool grid_in_domain(int64_t i, int64_t j) const {

@@ -59,6 +59,71 @@ def cpu_baseline(field, u, workloads, max_threads=16):
                       f"C oracle with OpenMP, {wall:.2f} s"}
 
 
+def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
+    """Config of the batched form: n_solves independent half-cylinder solves
+    with grids drawn from {121x481, 161x641, 201x801, 241x961} and inflow Mach
+    ~ U[2, 4] (seeded LCG), `steps` SSP-RK3 steps each with per-solve dt =
+    0.05 dt_cfl; one batched launch sequence per step (hipGraph) against the
+    same solves run back to back with the plain solver."""
+    workloads = importlib.import_module("3t_rh_amd.workloads")
+    autotune = importlib.import_module("3t_rh_amd.autotune")
+    rng = workloads.LCG(seed)
+    classes = [121, 161, 201, 241]
+    draws = []
+    for _ in range(n_solves):
+        draws.append((classes[min(3, int(rng() * 4))], 2.0 + 2.0 * rng()))
+    base = {}
+    for nx in sorted(set(d[0] for d in draws)):
+        log(f"batched: building cylinder {nx}")
+        base[nx] = rh3t.Field.init_cylinder(nx, 3.0)
+    fields, states = [], []
+    for nx, mach in draws:
+        f = base[nx].clone()
+        f.set_bc_state(rh3t.BC_FARFIELD, rh3t.cylinder_freestream(mach))
+        fields.append(f)
+        states.append(f.cylinder_state(mach))
+    cells = sum(int(f.fluid.sum()) for f in fields)
+    batch = rh3t.Batch(fields)
+
+    def reset():
+        for k, u in enumerate(states):
+            batch.set_state(k, u, 0.0)
+
+    tuned = None
+    if tune:
+        reset()
+        key = autotune.signature([(f.grid.nx, f.grid.ny) for f in fields], "fast")
+        tuned = autotune.tune(batch, steps=3, log=log)
+        tuned["key"] = key
+    reset()
+    batch.advance(1)
+    reset()
+    t0 = time.perf_counter()
+    batch.advance(steps)
+    t_batch = time.perf_counter() - t0
+    # the same solves back to back with the plain solver (graph per solve)
+    solvers = [f.solver() for f in fields]
+    for s, u in zip(solvers, states):
+        s.set_state(u, 0.0)
+        s.advance(1)
+        s.set_state(u, 0.0)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.advance(steps)
+    t_plain = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    batch.close()
+    return {"solves": n_solves, "steps": steps, "fluid_cells": cells,
+            "grids": {str(nx): sum(1 for d in draws if d[0] == nx) for nx in classes},
+            "mach_range": [min(d[1] for d in draws), max(d[1] for d in draws)],
+            "batched_s": t_batch, "plain_back_to_back_s": t_plain, "batched_vs_plain": t_plain / t_batch,
+            "batched_cell_stage_per_s": cells * 3 * steps / t_batch,
+            "plain_cell_stage_per_s": cells * 3 * steps / t_plain,
+            "autotune": None if tuned is None else {k: tuned[k] for k in ("bx", "by", "ghosts_per_block",
+                                                                          "ms_per_step")}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +132,9 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--arith", choices=["fast", "exact"], default="fast")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--batched-solves", type=int, default=64)
+    ap.add_argument("--batched-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
@@ -131,6 +199,9 @@ def main():
                      "traffic": None, "flop_per_interface": FLOP_PER_INTERFACE, "interfaces": interfaces,
                      "avg_launch_ms": fx_ms},
     }
+    if not args.no_batched:
+        log("batched leg")
+        line["batched"] = batched_leg(rh3t, args.batched_solves, args.batched_steps)
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 step)")
         line["cpu_baseline"] = cpu_baseline(field, u0, workloads)

@@ -163,6 +163,17 @@ int rh3t_problem_build_ghosts(rh3t_problem* pb);
  * reference exactly).  The returned problem has its ghosts built. */
 int rh3t_problem_init_cylinder(int32_t nx, double mach, rh3t_problem** out);
 
+/* Copy of a problem (fluid mask, BCs, ghost list).  The heavy-ghost tables
+ * depend only on geometry and grid and are shared with the source, on the
+ * host and, inside one rh3t_batch, on the device. */
+int rh3t_problem_clone(const rh3t_problem* src, rh3t_problem** out);
+
+/* Replace the state of every FarField (kind = RH3T_BC_FARFIELD) or
+ * Constant (RH3T_BC_CONSTANT) side, e.g. the inflow Mach number of the
+ * cylinder.  Ghost tables are unaffected. */
+int rh3t_problem_set_bc_state(rh3t_problem* pb, int32_t kind,
+                              const double state[6]);
+
 /* Initial state of init_cylinder (uniform freestream on fluid cells, zero
  * elsewhere), AoS nx*ny*6. */
 int rh3t_problem_cylinder_state(const rh3t_problem* pb, double mach,
