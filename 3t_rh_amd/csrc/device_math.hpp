@@ -290,6 +290,28 @@ __device__ __forceinline__ double local_dt(const Phys& ph, const double s[6], do
   return ph.lambda / vj;
 }
 
+// (eps + beta)^q of the boundary WENO weights (powf(WENO_Q), src/bc1.rs:241).
+// For q = 10: double-double product chain rounded once (correctly rounded,
+// the value glibc's pow returns except within ~2^-60 of a rounding
+// midpoint); the oracle evaluates it identically.
+__device__ __forceinline__ void dd_mul(double ah, double al, double bh, double bl, double* rh, double* rl) {
+  double p = ah * bh;
+  double e = fma(ah, bh, -p);
+  e += ah * bl + al * bh;
+  double s = p + e;
+  *rl = e - (s - p);
+  *rh = s;
+}
+__device__ __forceinline__ double weno_pow(double x, double q) {
+  if (q != 10.0) return pow(x, q);
+  double x2h = x * x, x2l = fma(x, x, -x2h);
+  double x4h, x4l, x8h, x8l, rh, rl;
+  dd_mul(x2h, x2l, x2h, x2l, &x4h, &x4l);
+  dd_mul(x4h, x4l, x4h, x4l, &x8h, &x8l);
+  dd_mul(x8h, x8l, x2h, x2l, &rh, &rl);
+  return rh + rl;
+}
+
 // assert_admissible, src/main.rs:22-33
 __device__ __forceinline__ bool admissible(const double s[6]) {
 #pragma unroll

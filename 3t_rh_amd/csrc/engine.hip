@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     const double* p = &s_part[w][base + c * m];
     double beta = 0.0;
     for (int a = 0; a < m; ++a) beta += p[a];
-    s_alpha[w][c][r] = hg.d5[5L * row + r] / pow(ph.eps + beta, ph.weno_q);
+    s_alpha[w][c][r] = hg.d5[5L * row + r] / weno_pow(ph.eps + beta, ph.weno_q);
   }
   __syncthreads();
   if (active && lane < 30) {  // V^(k)_c = sum_r (alpha_r / sum alpha) v_k,r

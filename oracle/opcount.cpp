@@ -68,6 +68,10 @@ static inline OpCount pow(const OpCount& a, const OpCount& b) {
   return mk(std::pow(a.v, b.v), a.c && b.c);
 }
 static inline OpCount fabs(const OpCount& a) { return mk(std::fabs(a.v), a.c); }
+static inline OpCount fma(const OpCount& a, const OpCount& b, const OpCount& c) {
+  if (!(a.c && b.c && c.c)) g_ops += 2;
+  return mk(std::fma(a.v, b.v, c.v), a.c && b.c && c.c);
+}
 static inline OpCount fmin(const OpCount& a, const OpCount& b) { return mk(std::fmin(a.v, b.v), a.c && b.c); }
 static inline OpCount fmax(const OpCount& a, const OpCount& b) { return mk(std::fmax(a.v, b.v), a.c && b.c); }
 static inline bool isfinite(const OpCount& a) { return std::isfinite(a.v); }

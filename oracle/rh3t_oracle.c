@@ -32,6 +32,29 @@ static inline double powi(double x, int n) {
   return mul;
 }
 
+/* (eps + beta)^q of the boundary WENO weights (powf(WENO_Q), src/bc1.rs:241).
+ * For q = 10 the power is evaluated in double-double arithmetic and rounded
+ * once, i.e. correctly rounded -- the value glibc's pow returns except for
+ * inputs within ~2^-60 of a rounding midpoint.  The GPU kernel evaluates it
+ * the same way, so the two agree bit for bit. */
+static inline void dd_mul(double ah, double al, double bh, double bl, double* rh, double* rl) {
+  double p = ah * bh;
+  double e = fma(ah, bh, -p);
+  e += ah * bl + al * bh;
+  double s = p + e;
+  *rl = e - (s - p);
+  *rh = s;
+}
+static inline double weno_pow(double x, double q) {
+  if (q != 10.0) return pow(x, q);
+  double x2h = x * x, x2l = fma(x, x, -x2h);
+  double x4h, x4l, x8h, x8l, rh, rl;
+  dd_mul(x2h, x2l, x2h, x2l, &x4h, &x4l);
+  dd_mul(x4h, x4l, x4h, x4l, &x8h, &x8l);
+  dd_mul(x8h, x8l, x2h, x2l, &rh, &rl);
+  return rh + rl;
+}
+
 /* ------------------------------------------------------------ state.rs */
 
 /* Derived::from_state, src/state.rs:241-275 */
@@ -432,7 +455,7 @@ static void weno_extrapolation_pre(const ro_physics* ph, const double L[36], con
         int woff = kWOff[r] + k * m;
         for (int a = 0; a < m; ++a) vk += W[woff + a] * data[c][a];
       }
-      alpha[c][r] = d5[r] / pow(ph->eps + beta, ph->weno_q);
+      alpha[c][r] = d5[r] / weno_pow(ph->eps + beta, ph->weno_q);
       vks[c][r] = vk;
       alpha_sum[c] += alpha[c][r];
     }

@@ -123,10 +123,18 @@ def test_l_cylinder(rh3t, cyl, arith):
     s.close()
 
 
-def test_rk3_cylinder_steps(rh3t, cyl):
+@pytest.mark.parametrize("arith,tol", [(0, TOL), (1, 1e-11)], ids=["exact", "fast"])
+def test_rk3_cylinder_steps(rh3t, cyl, arith, tol):
+    """One full RK3 step from the same input, three times with resync.  The
+    boundary WENO weights d_r/(eps+beta_r)^10 (src/bc1.rs:241) amplify
+    last-bit differences of a stage state by up to ~1e8 in near-uniform flow
+    (pinned on the oracle alone by test_oracle.py::
+    test_cylinder_step_is_ill_conditioned), so per-step agreement at 1e-12
+    needs the reference's rounding: EXACT reproduces it bit for bit; FAST,
+    whose interior fluxes round differently, is held to 1e-11."""
     f, u, p = cyl
-    _say("rk3 cylinder 3 steps with per-step resync")
-    s = f.solver()
+    _say(f"rk3 cylinder 3 steps with per-step resync arith={arith}")
+    s = f.solver(arith=arith)
     for step in range(3):
         dt = 0.05 * ob.global_dt(p, u)
         s.set_state(u, 0.0)
@@ -135,7 +143,7 @@ def test_rk3_cylinder_steps(rh3t, cyl):
         ref, _ = ob.rk3_step(p, u, dt)
         err = ob.relerr(got, ref, f.fluid)
         print("step", step, "relerr", err)
-        assert err.max() <= TOL
+        assert err.max() <= tol
         u = ref
     s.close()
 

@@ -390,3 +390,30 @@ def test_opcount_matches_bench_constants():
     bench = importlib.import_module("bench")
     assert counts["flux"] == bench.FLOP_PER_INTERFACE
     assert round(counts["cell_stage"]) == bench.FLOP_PER_CELL_STAGE
+
+
+def test_cylinder_step_is_ill_conditioned(rh3t):
+    """The reference operator itself amplifies last-bit perturbations near
+    the cylinder's rear junction: after two RK stages of the impulsively
+    started Mach-3 flow, a 2e-16 relative perturbation of the stage state
+    moves the FarField outflow ghosts by ~1e-9 and l() by ~1e-8 (the boundary
+    weights d_r/(eps+beta_r)^10 with eps = 1e-12 and beta_r ~ eps).  Two
+    correct implementations that round differently therefore cannot agree to
+    1e-12 there; this pins that fact on the oracle alone."""
+    f = rh3t.Field.init_cylinder(121, 3.0)
+    u = f.cylinder_state()
+    p = ob.OracleProblem(f)
+    fl = f.fluid
+    dt = 0.05 * ob.global_dt(p, u)
+    u1 = u + dt * ob.l(p, u)
+    u1[~fl] = 0.0
+    u2 = 0.75 * u + 0.25 * u1 + (dt / 4.0) * ob.l(p, u1)
+    u2[~fl] = 0.0
+    ra, ga = ob.l(p, u2, with_ghosts=True)
+    rng = np.random.default_rng(0)
+    up = u2 * (1.0 + 2e-16 * rng.standard_normal(u2.shape))
+    up[~fl] = 0.0
+    rb, gb = ob.l(p, up, with_ghosts=True)
+    assert np.abs(up - u2).max() < 1e-14
+    assert np.abs(gb - ga).max() > 1e-10
+    assert np.abs(rb - ra).max() > 1e-9
