@@ -135,6 +135,13 @@ SIGNATURES = {
     "rh3t_batch_profile": (ctypes.c_int, [_P, ctypes.c_int32]),
     "rh3t_batch_profile_read": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, _DP,
                                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)]),
+    "rh3t_ctx_set_physics": (ctypes.c_int, [_P, ctypes.POINTER(c_physics)]),
+    "rh3t_l_vjp": (ctypes.c_int, [_P, _DP, _DP, _DP]),
+    "rh3t_rk3_vjp": (ctypes.c_int, [_P, ctypes.c_double, _DP, _DP, _DP]),
+    "rh3t_adjoint_create": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(_P)]),
+    "rh3t_adjoint_destroy": (None, [_P]),
+    "rh3t_adjoint_forward": (ctypes.c_int, [_P, ctypes.c_int32, _DP]),
+    "rh3t_adjoint_backward": (ctypes.c_int, [_P, _DP, _DP, _DP]),
     "rh3t_last_error": (ctypes.c_int, [_P, ctypes.POINTER(c_error)]),
     "rh3t_profile": (ctypes.c_int, [_P, ctypes.c_int32]),
     "rh3t_profile_read": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, _DP,
@@ -192,6 +199,23 @@ class Physics:
 
     def as_tuple(self):
         return tuple(getattr(self.c, k) for k in self.FIELDS)
+
+    def replace(self, **kw):
+        """A copy with some constants changed."""
+        q = {k: getattr(self.c, k) for k in self.FIELDS}
+        q.update(kw)
+        return Physics(**q)
+
+    def params(self):
+        """The differentiable parameters, in PARAMS order."""
+        return np.array([getattr(self.c, k) for k in PARAMS])
+
+    def with_params(self, values):
+        return self.replace(**dict(zip(PARAMS, (float(v) for v in values))))
+
+
+# the parameters the reverse mode differentiates (src/constant.rs:17-25)
+PARAMS = ("omega_ei", "omega_er", "kappa_e", "kappa_i", "kappa_r")
 
 
 class BCType:
@@ -413,6 +437,39 @@ class Solver:
             self._raise(st, "advance")
         return t.value
 
+    def set_physics(self, physics):
+        """Replaces the physics constants (e.g. other exchange coefficients)."""
+        self.physics = physics
+        _check(lib().rh3t_ctx_set_physics(self._h, ctypes.byref(physics.c)), "set_physics")
+
+    def _cot(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (self.nx, self.ny, 6):
+            raise ValueError(f"cotangent must have shape {(self.nx, self.ny, 6)}")
+        return a
+
+    def l_vjp(self, rbar):
+        """((dl/du)^T rbar, d<rbar, l>/d PARAMS) at the current state."""
+        rbar = self._cot(rbar)
+        ubar, pg = np.zeros_like(rbar), np.zeros(len(PARAMS))
+        st = lib().rh3t_l_vjp(self._h, _dptr(rbar), _dptr(ubar), _dptr(pg))
+        if st != OK:
+            self._raise(st, "l_vjp")
+        return ubar, pg
+
+    def rk3_vjp(self, dt, lam):
+        """VJP of one rk3_ssp step of size dt from the current state (which
+        is not advanced)."""
+        lam = self._cot(lam)
+        ubar, pg = np.zeros_like(lam), np.zeros(len(PARAMS))
+        st = lib().rh3t_rk3_vjp(self._h, float(dt), _dptr(lam), _dptr(ubar), _dptr(pg))
+        if st != OK:
+            self._raise(st, "rk3_vjp")
+        return ubar, pg
+
+    def adjoint(self, checkpoint_every=5):
+        return Adjoint(self, checkpoint_every)
+
     def profile(self, enable=True):
         _check(lib().rh3t_profile(self._h, 1 if enable else 0))
 
@@ -424,6 +481,43 @@ class Solver:
         _check(lib().rh3t_profile_read(self._h, 32, names, 1024, avg, cnt, ctypes.byref(n)))
         labels = names.raw.split(b"\0")[: n.value]
         return {labels[k].decode(): (avg[k], cnt[k]) for k in range(n.value)}
+
+
+class Adjoint:
+    """Checkpoint-and-recompute adjoint of a sequence of rk3_ssp steps with a
+    frozen dt schedule (state kept every ``checkpoint_every`` steps)."""
+
+    def __init__(self, solver, checkpoint_every=5):
+        self.solver = solver
+        self._h = ctypes.c_void_p()
+        st = lib().rh3t_adjoint_create(solver._h, int(checkpoint_every), ctypes.byref(self._h))
+        if st != OK:
+            solver._raise(st, "rh3t_adjoint_create")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rh3t_adjoint_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def forward(self, dts):
+        """len(dts) RK3 steps from the solver's state."""
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        st = lib().rh3t_adjoint_forward(self._h, len(dts), _dptr(dts))
+        if st != OK:
+            self.solver._raise(st, "adjoint forward")
+
+    def backward(self, lam):
+        """((du^N/du^0)^T lam, parameter cotangents) of the last forward."""
+        lam = self.solver._cot(lam)
+        out, pg = np.zeros_like(lam), np.zeros(len(PARAMS))
+        st = lib().rh3t_adjoint_backward(self._h, _dptr(lam), _dptr(out), _dptr(pg))
+        if st != OK:
+            self.solver._raise(st, "adjoint backward")
+        return out, pg
 
 
 def _profile_read(fn, h):

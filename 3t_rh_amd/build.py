@@ -51,13 +51,17 @@ def build(verbose=False, resource_usage=False):
     os.makedirs(BUILD, exist_ok=True)
     inc = [f"-I{INCLUDE}", f"-I{CSRC}"]
     host_o = os.path.join(BUILD, "problem.o")
-    dev_o = os.path.join(BUILD, "engine.o")
     _run(["g++", *CXX_FLAGS, *inc, "-c", os.path.join(CSRC, "problem.cpp"), "-o", host_o], verbose)
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
-    out = _run([_hipcc(), *HIPCC_FLAGS, *inc, *extra, "-c", os.path.join(CSRC, "engine.hip"),
-                "-o", dev_o], verbose)
+    out = ""
+    dev_objs = []
+    for name in ("engine", "adjoint"):
+        obj = os.path.join(BUILD, f"{name}.o")
+        out += _run([_hipcc(), *HIPCC_FLAGS, *inc, *extra, "-c", os.path.join(CSRC, f"{name}.hip"),
+                     "-o", obj], verbose)
+        dev_objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", host_o, dev_o, "-o", tmp,
+    _run([_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", host_o, *dev_objs, "-o", tmp,
           "-Wl,-rpath,/opt/rocm/lib"], verbose)
     os.replace(tmp, LIB)
     return out if resource_usage else LIB

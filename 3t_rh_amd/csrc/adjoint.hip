@@ -1,0 +1,937 @@
+// Reverse mode (VJP) of l() and of the SSP-RK3 step of colinhu123/3T_RH on
+// gfx950, with a checkpoint-and-recompute schedule over time steps.
+//
+// For a stage state S with cotangent LAM of the stage output and a scale c
+// (c = dt, dt/4, 2dt/3 for the three stages), one VJP evaluation forms
+//   OUT = a LAM + b1 X1 + b2 X2 + c (dl/dS)^T LAM
+// in gather form (every output written by exactly one lane, every sum in a
+// fixed order: results are bitwise reproducible):
+//   k_adj_iface<0|1> per interface: the pullback of the characteristic WENO5
+//                    flux (src/weno.rs:685-802) through the eigensystem,
+//                    the physical fluxes and the Derived values of its 6
+//                    stencil points (36 state cotangents per interface,
+//                    stored)
+//   k_adj_gather     per fluid cell / ghost: sums its 12 interface
+//                    contributions, the pullbacks of the noncon terms of the
+//                    9-point lines through it (src/noncon.rs:288-332), of the
+//                    diffusion fluxes (src/diffusion.rs:27-50) and of the
+//                    exchange source (src/source.rs:7-21), then
+//                    Derived::from_state's pullback (src/state.rs:241-275);
+//                    per-block partial sums of the parameter cotangents
+//   k_adj_ghost_T    transposes the cheap ghost maps (Periodic / Zeroth
+//                    copies, ReflectiveWall mirror, src/bc1.rs:471-607,
+//                    1015-1036) into their source cells
+//   k_adj_param      sums the parameter partials of a call, in block order
+// The parameters are (omega_ei, omega_er, kappa_e, kappa_i, kappa_r)
+// (src/constant.rs:17-25); their cotangents are formed whether or not the
+// forward skips the zero-valued terms.  The time step is not differentiated
+// (dt is frozen).  Heavy (FarField / Outflow / Wall) ghosts are not covered.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "adjoint.hpp"
+#include "engine.hpp"
+
+using namespace rh3t_dev;
+using namespace rh3t_eng;
+
+namespace {
+
+constexpr int kNParam = 5;
+
+struct AdjDesc {
+  Geo g;
+  const uint8_t* pt;
+  const double* S;     // stage state with its ghosts
+  const double* D;     // its Derived planes (D_N)
+  const double* LAM;   // cotangent of the stage output (fluid cells, 0 elsewhere)
+  double* XB;          // 36 planes over x interfaces
+  double* YB;          // 36 planes over y interfaces
+  double* SB;          // 6 planes: state cotangents of ghost points
+  double* OUT;         // 6 planes: result on fluid cells
+  const double* X1;
+  const double* X2;
+  double a, b1, b2, rscale;
+  double* PG;          // kNParam partial sums per gather block
+};
+
+// sqrt(rho)-weighted Roe average of stencil points 3 (slot c0) and 2
+struct RoeAvg {
+  double r1, r2, irs, q3[5], q2[5], qr[5];
+};
+
+__device__ __forceinline__ void roe_average(const double* S, const double* D, long P, long c0, long p2,
+                                            RoeAvg& A) {
+  A.r1 = sqrt(S[c0]);
+  A.r2 = sqrt(S[p2]);
+  A.irs = 1.0 / (A.r1 + A.r2);
+  const int planes[5] = {D_U, D_V, D_EE, D_EI, D_ER};
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    A.q3[k] = D[planes[k] * P + c0];
+    A.q2[k] = D[planes[k] * P + p2];
+    A.qr[k] = (A.q3[k] * A.r1 + A.q2[k] * A.r2) * A.irs;
+  }
+}
+
+// Interface flux pullback, one lane per interface, in two phases.
+// Characteristic phase: the WENO5 / local Lax-Friedrichs / eigenvector
+// pullback of every characteristic (src/weno.rs:685-802) with the 6-point
+// stencil in registers; the split-flux cotangents (10 per characteristic)
+// go to LDS, the eigensystem cotangents to the Roe state.  Point phase:
+// those pulled back through L (F and U projections), the physical fluxes
+// (flux_from_derived, src/state.rs:56-85), the Roe average and
+// Derived::from_state to the 6 stencil states (36 values per interface,
+// stored for the gather).  The register footprint allows one wavefront per
+// SIMD, so a workgroup is one wavefront with 60 x 64 doubles of LDS.
+constexpr int kIfBx = 64;
+constexpr int kCb = 60;
+
+template <int DIR>
+__global__ __launch_bounds__(kIfBx) void k_adj_iface(AdjDesc A, EigConst K, Phys ph) {
+  __shared__ double s_cb[kCb][kIfBx];
+  const Geo g = A.g;
+  const int ni = DIR == 0 ? g.nx + 1 : g.nx;
+  const int nj = DIR == 0 ? g.ny : g.ny + 1;
+  const int tiles_j = (nj + kIfBx - 1) / kIfBx;
+  const int j = (int)(blockIdx.x % tiles_j) * kIfBx + threadIdx.x;
+  const int i = (int)(blockIdx.x / tiles_j);
+  if (i >= ni || j >= nj) return;
+  const long fpl = DIR == 0 ? (long)(g.nx + 1) * g.ny : (long)g.nx * (g.ny + 1);
+  const long fidx = DIR == 0 ? (long)i * g.ny + j : (long)i * (g.ny + 1) + j;
+  double* __restrict__ out = DIR == 0 ? A.XB : A.YB;
+  const long c0 = pad_index(g, i, j);
+  const long step = DIR == 0 ? g.PY : 1;
+  if (A.pt[c0] != 1 && A.pt[c0 - step] != 1) {
+#pragma unroll
+    for (int k = 0; k < 36; ++k) out[k * fpl + fidx] = 0.0;
+    return;
+  }
+  const double* S = A.S;
+  const double* D = A.D;
+  const long P = g.plane;
+  double* cbs = &s_cb[0][threadIdx.x];  // this lane's column, stride kIfBx
+  RoeAvg ra;
+  roe_average(S, D, P, c0, c0 - step, ra);
+  RoeV R;
+  roe_values<DIR>(K, ra.qr[0], ra.qr[1], ra.qr + 2, R);
+  double roeb[5];
+  {
+    // flux F[i] enters r(i) with +1/ds and r(i-1) with -1/ds (src/main.rs:266-270)
+    const double inv = 1.0 / (DIR == 0 ? g.dx : g.dy);
+    double ob[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) ob[f] = A.rscale * inv * (A.LAM[f * P + c0] - A.LAM[f * P + c0 - step]);
+    double U[6][6], F[6][6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) load_point<DIR>(S, D, P, (unsigned)((c0 + (q - 3) * step) * 8), U[q], F[q]);
+    RoeBar B;
+    roe_bar_zero(B);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double cb[10];
+      characteristic_step<DIR>(c, K, ph.eps, R, U, F, ob, cb, B);
+#pragma unroll
+      for (int k = 0; k < 10; ++k) cbs[(c * 10 + k) * kIfBx] = cb[k];
+    }
+    roe_bar_final<DIR>(K, R, B, roeb);
+  }
+  // ---- point phase
+  double L[6][6], a[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double col[6], lam;
+    row_col<DIR>(K, R, c, L[c], col, &lam);
+    a[c] = fabs(lam);
+  }
+  double b3[5], b2[5], r1b = 0.0, r2b = 0.0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    b3[k] = roeb[k] * ra.r1 * ra.irs;
+    b2[k] = roeb[k] * ra.r2 * ra.irs;
+    r1b += roeb[k] * (ra.q3[k] - ra.qr[k]) * ra.irs;
+    r2b += roeb[k] * (ra.q2[k] - ra.qr[k]) * ra.irs;
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const long p = c0 + (q - 3) * step;
+    // cotangents of this point's physical flux (fb) and state (sb) through L
+    double fb[6] = {0, 0, 0, 0, 0, 0}, sb[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double fpb = q < 5 ? cbs[(c * 10 + q) * kIfBx] : 0.0;
+      const double fmb = q > 0 ? cbs[(c * 10 + 4 + q) * kIfBx] : 0.0;
+      const double cfb = 0.5 * (fpb + fmb), cub = a[c] * 0.5 * (fpb - fmb);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        fb[f] += cfb * L[c][f];
+        sb[f] += cub * L[c][f];
+      }
+    }
+    double st[6], db[D_N];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) st[f] = S[f * P + p];
+#pragma unroll
+    for (int k = 0; k < D_N; ++k) db[k] = 0.0;
+    const double u = D[D_U * P + p], v = D[D_V * P + p];
+    const double pk[3] = {D[D_PE * P + p], D[D_PI * P + p], D[D_PR * P + p]};
+    const double rho = st[0], w = DIR == 0 ? u : v;
+    double ub = 0.0, vb = 0.0, wb = fb[0] * rho;
+    sb[0] += fb[0] * w;
+    if (DIR == 0) {
+      sb[0] += fb[1] * u * u + fb[2] * u * v;
+      ub += fb[1] * 2.0 * rho * u + fb[2] * rho * v;
+      vb += fb[2] * rho * u;
+    } else {
+      sb[0] += fb[1] * u * v + fb[2] * v * v;
+      ub += fb[1] * rho * v;
+      vb += fb[1] * rho * u + fb[2] * 2.0 * rho * v;
+    }
+    const double Pb = DIR == 0 ? fb[1] : fb[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      sb[3 + k] += fb[3 + k] * w;
+      wb += fb[3 + k] * (st[3 + k] + pk[k]);
+      db[D_PE + k] = Pb + fb[3 + k] * w;
+    }
+    if (DIR == 0)
+      ub += wb;
+    else
+      vb += wb;
+    db[D_U] = ub;
+    db[D_V] = vb;
+    if (q == 3 || q == 2) {
+      const double* bb = q == 3 ? b3 : b2;
+      db[D_U] += bb[0];
+      db[D_V] += bb[1];
+      db[D_EE] = bb[2];
+      db[D_EI] = bb[3];
+      db[D_ER] = bb[4];
+      sb[0] += (q == 3 ? r1b * 0.5 / ra.r1 : r2b * 0.5 / ra.r2);
+    }
+    derived_vjp(ph, st, db, sb);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) out[(q * 6 + f) * fpl + fidx] = sb[f];
+  }
+}
+
+// Weights of N_t in jump(N[0..8)) and jump(N[1..9)) (n_jump_pre,
+// src/noncon.rs:259-285, with the literal q_pulse weights of
+// src/noncon.rs:49-57), of sixth_order_derivative's numerator
+// (src/noncon.rs:26-44), and (2, -27, 270, -490, 270, -27, 2): the
+// difference of the two diffusion fluxes of src/diffusion.rs:27-50 around a
+// cell.  Used with compile-time indices only.
+constexpr double kQc[7] = {-0.00714285714285723, 0.0595238095238118, -0.240476190476197, 0.759523809523819,
+                           0.509523809523803,    -0.0904761904761891, 0.00952380952380977};
+constexpr double jm_w(int t) { return (t >= 1 && t <= 7 ? kQc[7 - t] : 0.0) - (t <= 6 ? kQc[t] : 0.0); }
+constexpr double jp_w(int t) { return t >= 1 ? jm_w(t - 1) : 0.0; }
+constexpr double kS6[9] = {0.0, -1.0, 9.0, -45.0, 0.0, 45.0, -9.0, 1.0, 0.0};
+constexpr double kLap[7] = {2.0, -27.0, 270.0, -490.0, 270.0, -27.0, 2.0};
+
+// One line (DIR 0: x, DIR 1: y) of cell-term pullbacks into point p: the
+// noncon terms of the 9 cells whose stencils hold p (pressure and velocity
+// cotangents) and the diffusion fluxes of the 7 cells around it
+// (temperature cotangents, T_r^4 for the radiation field).  The pressures
+// and velocities of the 11-point line are read once.
+template <int DIR>
+__device__ __forceinline__ void line_gather(const AdjDesc& A, const Phys& ph, long p, int i, int j, double db[D_N],
+                                            double tb[3]) {
+  const Geo& g = A.g;
+  const long P = g.plane;
+  const long st = DIR == 0 ? g.PY : 1;
+  const double ds = DIR == 0 ? g.dx : g.dy;
+  const double ids = 1.0 / ds;
+  const int VP = DIR == 0 ? D_U : D_V;
+  const int along = DIR == 0 ? i : j, across = DIR == 0 ? j : i;
+  const int n_along = DIR == 0 ? g.nx : g.ny, n_across = DIR == 0 ? g.ny : g.nx;
+  if (across < 0 || across >= n_across) return;
+  const double* D = A.D;
+  // p_e, p_i, p_r and the velocity at offsets -5..5 (0 outside the padded
+  // planes; such points are never in the stencil of a fluid cell)
+  double Pv[3][11], V[11];
+#pragma unroll
+  for (int o = -5; o <= 5; ++o) {
+    const int a = along + o;
+    const bool in = a >= -4 && a < n_along + 4;
+    const long q = p + o * st;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) Pv[m][o + 5] = in ? D[(D_PE + m) * P + q] : 0.0;
+    V[o + 5] = in ? D[VP * P + q] : 0.0;
+  }
+  // rho_m(c) = sum_e rb_e(c) M[e][m], M = [[2,-1,-1],[-1,2,-1],[-1,-1,2]],
+  // rb = rscale * LAM of the energy equations of cell c
+  double Pb[3] = {0.0, 0.0, 0.0};
+  double rc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // rho_m of cells -1, 0, +1
+  bool fc[3] = {false, false, false};
+  const double kap[3] = {ph.kappa_e, ph.kappa_i, ph.kappa_r};
+  const double dsc = 1.0 / (180.0 * ds * ds);
+#pragma unroll
+  for (int o = -4; o <= 4; ++o) {
+    const int a = along + o;
+    const long c = p + o * st;
+    const bool fl = a >= 0 && a < n_along && A.pt[c] == 1;
+    if (!fl) continue;
+    const double r0 = A.rscale * A.LAM[3 * P + c], r1 = A.rscale * A.LAM[4 * P + c];
+    const double r2 = A.rscale * A.LAM[5 * P + c];
+    const double sr = r0 + r1 + r2;
+    const double rh[3] = {3.0 * r0 - sr, 3.0 * r1 - sr, 3.0 * r2 - sr};
+    const int t = 4 - o;  // position of p in the stencil of c
+    const double vc = V[o + 5], vm = V[o + 4], vp = V[o + 6];
+    const double cl = fmax(fmax(vc, vm), 0.0) * (1.0 / 3.0);
+    const double cr = fmin(fmin(vc, vp), 0.0) * (1.0 / 3.0);
+    const double coef = (vc * (1.0 / 3.0) * kS6[t] * (1.0 / 60.0) + cl * jm_w(t) + cr * jp_w(t)) * ids;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) Pb[m] += rh[m] * coef;
+    if (o >= -1 && o <= 1) {
+      fc[o + 1] = true;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) rc[o + 1][m] = rh[m];
+    }
+    if (ph.diffusion && o >= -3 && o <= 3) {
+      const double w = kLap[3 - o] * dsc;
+      tb[0] += kap[0] * w * r0;
+      tb[1] += kap[1] * w * r1;
+      tb[2] += kap[2] * w * r2;
+    }
+  }
+  db[D_PE] += Pb[0];
+  db[D_PI] += Pb[1];
+  db[D_PR] += Pb[2];
+  // velocity: the v/3 factor of cell p and the max/min selections of cl
+  // (cells p, p+1) and cr (cells p-1, p).  sh: offset of a cell's stencil
+  // origin in the line.
+  auto jump_sum = [&](const double r[3], int sh, bool plus) {
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double a = 0.0;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) a += (plus ? jp_w(t) : jm_w(t)) * Pv[m][t + sh];
+      acc += r[m] * a;
+    }
+    return acc * ids;
+  };
+  const double vs = V[5];
+  double velb = 0.0;
+  if (fc[1]) {
+    double der = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double d = 0.0;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) d += kS6[t] * Pv[m][t + 1];
+      der += rc[1][m] * d;
+    }
+    velb += der * (1.0 / 180.0) * ids;
+    if (fmax(vs, V[4]) > 0.0 && vs >= V[4]) velb += jump_sum(rc[1], 1, false) * (1.0 / 3.0);
+    if (fmin(vs, V[6]) < 0.0 && vs <= V[6]) velb += jump_sum(rc[1], 1, true) * (1.0 / 3.0);
+  }
+  if (fc[2] && fmax(V[6], vs) > 0.0 && !(V[6] >= vs)) velb += jump_sum(rc[2], 2, false) * (1.0 / 3.0);
+  if (fc[0] && fmin(V[4], vs) < 0.0 && !(V[4] <= vs)) velb += jump_sum(rc[0], 0, true) * (1.0 / 3.0);
+  db[VP] += velb;
+}
+
+// sum_t lap_t T(c + (t-3) st) / (180 ds^2) for the three temperatures
+template <int DIR>
+__device__ __forceinline__ void laplacian(const AdjDesc& A, long c, double lap[3]) {
+  const Geo& g = A.g;
+  const long P = g.plane;
+  const long st = DIR == 0 ? g.PY : 1;
+  const double ds = DIR == 0 ? g.dx : g.dy;
+  const double sc = 1.0 / (180.0 * ds * ds);
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < 7; ++t) {
+      double T = A.D[(D_TE + e) * P + c + (t - 3) * st];
+      if (e == 2) T = pow4(T);
+      acc += kLap[t] * T;
+    }
+    lap[e] += acc * sc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adj_gather(AdjDesc A, Phys ph) {
+  __shared__ double red[kNParam][256];
+  const Geo g = A.g;
+  const long P = g.plane;
+  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  double pg[kNParam] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const int kind = p < P ? A.pt[p] : 0;
+  if (kind != 0) {
+    const int i = (int)(p / g.PY) - 4, j = (int)(p % g.PY) - 4;
+    double sb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // the 6 x and 6 y interfaces whose stencils hold p
+    const long fxpl = (long)(g.nx + 1) * g.ny, fypl = (long)g.nx * (g.ny + 1);
+    if (j >= 0 && j < g.ny) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int ix = i + 3 - q;
+        if (ix < 0 || ix > g.nx) continue;
+        const long fi = (long)ix * g.ny + j;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) sb[f] += A.XB[(q * 6 + f) * fxpl + fi];
+      }
+    }
+    if (i >= 0 && i < g.nx) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const int jy = j + 3 - q;
+        if (jy < 0 || jy > g.ny) continue;
+        const long fi = (long)i * (g.ny + 1) + jy;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) sb[f] += A.YB[(q * 6 + f) * fypl + fi];
+      }
+    }
+    double db[D_N];
+#pragma unroll
+    for (int k = 0; k < D_N; ++k) db[k] = 0.0;
+    double tb[3] = {0.0, 0.0, 0.0};
+    line_gather<0>(A, ph, p, i, j, db, tb);
+    line_gather<1>(A, ph, p, i, j, db, tb);
+    if (ph.diffusion) {
+      db[D_TE] += tb[0];
+      db[D_TI] += tb[1];
+      db[D_TR] += tb[2] * 4.0 * cube(A.D[D_TR * P + p]);
+    }
+    if (kind == 1) {
+      const double rb3 = A.rscale * A.LAM[3 * P + p], rb4 = A.rscale * A.LAM[4 * P + p];
+      const double rb5 = A.rscale * A.LAM[5 * P + p];
+      const double te = A.D[D_TE * P + p], ti = A.D[D_TI * P + p], tr = A.D[D_TR * P + p];
+      if (ph.source) {  // src/source.rs:7-21
+        const double te3 = 4.0 * cube(te), tr3 = 4.0 * cube(tr);
+        db[D_TE] += rb3 * (-ph.omega_ei - ph.omega_er * te3) + rb4 * ph.omega_ei + rb5 * ph.omega_er * te3;
+        db[D_TI] += (rb3 - rb4) * ph.omega_ei;
+        db[D_TR] += (rb3 - rb5) * ph.omega_er * tr3;
+      }
+      pg[0] = (te - ti) * (rb4 - rb3);
+      pg[1] = (pow4(te) - pow4(tr)) * (rb5 - rb3);
+      double lap[3] = {0.0, 0.0, 0.0};
+      laplacian<0>(A, p, lap);
+      laplacian<1>(A, p, lap);
+      pg[2] = rb3 * lap[0];
+      pg[3] = rb4 * lap[1];
+      pg[4] = rb5 * lap[2];
+    }
+    double s[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) s[f] = A.S[f * P + p];
+    derived_vjp(ph, s, db, sb);
+    if (kind == 1) {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        double v = A.a * A.LAM[f * P + p] + sb[f];
+        if (A.X1) v += A.b1 * A.X1[f * P + p];
+        if (A.X2) v += A.b2 * A.X2[f * P + p];
+        A.OUT[f * P + p] = v;
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) A.SB[f * P + p] = sb[f];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kNParam; ++k) red[k][threadIdx.x] = pg[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < kNParam; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  // each block owns its slots: partial sums accumulate over the VJPs of a
+  // call in launch order
+  if (threadIdx.x < kNParam) A.PG[blockIdx.x * kNParam + threadIdx.x] += red[threadIdx.x][0];
+}
+
+// One lane per source cell: adds the cotangents of the cheap ghosts copied
+// from it (ghost-id order).
+__global__ __launch_bounds__(256) void k_adj_ghost_T(int nrows, const long* __restrict__ row_cell,
+                                                     const int* __restrict__ row_ptr, const long* __restrict__ ent_pad,
+                                                     const int* __restrict__ ent_kind,
+                                                     const double* __restrict__ ent_n, const double* __restrict__ SB,
+                                                     double* __restrict__ OUT, long P) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const long c = row_cell[r];
+  double acc[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) acc[f] = OUT[f * P + c];
+  for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+    const long gp = ent_pad[e];
+    double sb[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) sb[f] = SB[f * P + gp];
+    if (ent_kind[e] == RH3T_BC_REFLECTIVE_WALL) {  // (I - 2 n n^T) is symmetric
+      const double nx = ent_n[2 * e], ny = ent_n[2 * e + 1];
+      const double mb = sb[1] * nx + sb[2] * ny;
+      sb[1] -= 2.0 * mb * nx;
+      sb[2] -= 2.0 * mb * ny;
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) acc[f] += sb[f];
+  }
+#pragma unroll
+  for (int f = 0; f < 6; ++f) OUT[f * P + c] = acc[f];
+}
+
+// acc[k] += sum over blocks of PG (fixed order: per-lane strided sums, then a
+// tree); one workgroup.
+__global__ __launch_bounds__(256) void k_adj_param(const double* __restrict__ PG, int nblocks, double* acc) {
+  __shared__ double red[kNParam][256];
+  double s[kNParam] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblocks; b += 256)
+#pragma unroll
+    for (int k = 0; k < kNParam; ++k) s[k] += PG[b * kNParam + k];
+#pragma unroll
+  for (int k = 0; k < kNParam; ++k) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+#pragma unroll
+      for (int k = 0; k < kNParam; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < kNParam) acc[threadIdx.x] += red[threadIdx.x][0];
+}
+
+__global__ void k_fill(double* p, long n, double v) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+// host AoS (fluid cells) <-> padded SoA cotangent buffer
+__global__ __launch_bounds__(256) void k_aos_to_buf(const double* __restrict__ aos, Geo g,
+                                                    const uint8_t* __restrict__ pt, double* __restrict__ dst) {
+  const long lin = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lin >= (long)g.nx * g.ny) return;
+  const long p = pad_index(g, lin / g.ny, lin % g.ny);
+  const bool fl = pt[p] == 1;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) dst[f * g.plane + p] = fl ? aos[6 * lin + f] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_buf_to_aos(const double* __restrict__ src, Geo g,
+                                                    const uint8_t* __restrict__ pt, double* __restrict__ aos) {
+  const long lin = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lin >= (long)g.nx * g.ny) return;
+  const long p = pad_index(g, lin / g.ny, lin % g.ny);
+  const bool fl = pt[p] == 1;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) aos[6 * lin + f] = fl ? src[f * g.plane + p] : 0.0;
+}
+
+// dt of step n from the frozen schedule (and Field.time += dt when asked)
+__global__ void k_set_dt(Scalars* sc, const double* __restrict__ dts, int n, int advance_time) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->dt = dts[n];
+    if (advance_time) sc->time = sc->time + dts[n];
+  }
+}
+
+}  // namespace
+
+// ======================================================================
+// adjoint workspace of an Engine (single solve)
+// ======================================================================
+
+namespace rh3t_eng {
+
+struct AdjWork {
+  double* lam[4] = {nullptr, nullptr, nullptr, nullptr};  // cotangent buffers, 6 planes each
+  double* XB = nullptr;
+  double* YB = nullptr;
+  double* SB = nullptr;
+  double* PG = nullptr;
+  int pg_blocks = 0;
+  double* pacc = nullptr;  // kNParam
+  double* aos = nullptr;   // staging
+  int nrows = 0;
+  long* row_cell = nullptr;
+  int* row_ptr = nullptr;
+  long* ent_pad = nullptr;
+  int* ent_kind = nullptr;
+  double* ent_n = nullptr;
+  double* dts = nullptr;   // frozen dt schedule (device)
+  int dts_cap = 0;
+};
+
+void free_adjoint(Engine& E) {
+  delete E.adj;  // device memory is owned by E.allocs
+  E.adj = nullptr;
+}
+
+}  // namespace rh3t_eng
+
+namespace {
+
+int init_adjoint(Engine& E) {
+  if (E.adj) return RH3T_OK;
+  if (E.nsolve != 1) {
+    std::snprintf(E.message, sizeof(E.message), "the adjoint covers single solves");
+    return RH3T_ERR_UNSUPPORTED;
+  }
+  if (E.heavy.n > 0) {
+    std::snprintf(E.message, sizeof(E.message),
+                  "the adjoint covers Periodic / ReflectiveWall / ZerothOrder / Constant ghosts only");
+    return RH3T_ERR_UNSUPPORTED;
+  }
+  HIP_TRY(hipSetDevice(E.device));
+  const SolveDesc& d = E.h_desc[0];
+  const Geo& g = d.g;
+  const SolveHost& sh = E.solves[0];
+  // ghost transpose lists grouped by source cell (sources must be fluid)
+  std::vector<std::pair<long, int>> ents;  // (source slot, ghost list index)
+  for (int k = 0; k < E.cheap.n; ++k) {
+    const int kind = E.h_cheap_kind[k];
+    if (kind == RH3T_BC_CONSTANT) continue;
+    const long src = E.h_cheap_src[k];
+    const long i = src / g.PY - 4, j = src % g.PY - 4;
+    if (i < 0 || i >= g.nx || j < 0 || j >= g.ny || !sh.fluid[i * g.ny + j]) {
+      std::snprintf(E.message, sizeof(E.message), "ghost source is not a fluid cell");
+      return RH3T_ERR_UNSUPPORTED;
+    }
+    ents.push_back({src, k});
+  }
+  std::stable_sort(ents.begin(), ents.end(),
+                   [](const std::pair<long, int>& a, const std::pair<long, int>& b) { return a.first < b.first; });
+  std::vector<long> row_cell, ent_pad;
+  std::vector<int> row_ptr, ent_kind;
+  std::vector<double> ent_n;
+  for (size_t e = 0; e < ents.size(); ++e) {
+    if (e == 0 || ents[e].first != ents[e - 1].first) {
+      row_cell.push_back(ents[e].first);
+      row_ptr.push_back((int)e);
+    }
+    const int k = ents[e].second;
+    ent_pad.push_back(E.h_cheap_pad[k]);
+    ent_kind.push_back(E.h_cheap_kind[k]);
+    ent_n.push_back(E.h_cheap_normal[2 * k]);
+    ent_n.push_back(E.h_cheap_normal[2 * k + 1]);
+  }
+  row_ptr.push_back((int)ents.size());
+  // published in E.adj only once complete (device memory is owned by E.allocs)
+  auto* W = new AdjWork();
+  struct Guard {
+    AdjWork* w;
+    ~Guard() { delete w; }
+  } guard{W};
+  int st;
+  const size_t plane6 = 6 * (size_t)g.plane;
+  for (int b = 0; b < 4; ++b) {
+    if ((st = dev_alloc(E, &W->lam[b], plane6))) return st;
+    HIP_TRY(hipMemset(W->lam[b], 0, plane6 * sizeof(double)));
+  }
+  if ((st = dev_alloc(E, &W->SB, plane6))) return st;
+  HIP_TRY(hipMemset(W->SB, 0, plane6 * sizeof(double)));
+  if ((st = dev_alloc(E, &W->XB, 36 * (size_t)(g.nx + 1) * g.ny))) return st;
+  if ((st = dev_alloc(E, &W->YB, 36 * (size_t)g.nx * (g.ny + 1)))) return st;
+  W->pg_blocks = (int)((g.plane + 255) / 256);
+  if ((st = dev_alloc(E, &W->PG, (size_t)kNParam * W->pg_blocks))) return st;
+  if ((st = dev_alloc(E, &W->pacc, (size_t)kNParam))) return st;
+  if ((st = dev_alloc(E, &W->aos, 6 * (size_t)g.nx * g.ny))) return st;
+  W->nrows = (int)row_cell.size();
+  if ((st = upload(E, &W->row_cell, row_cell))) return st;
+  if ((st = upload(E, &W->row_ptr, row_ptr))) return st;
+  if ((st = upload(E, &W->ent_pad, ent_pad))) return st;
+  if ((st = upload(E, &W->ent_kind, ent_kind))) return st;
+  if ((st = upload(E, &W->ent_n, ent_n))) return st;
+  guard.w = nullptr;
+  E.adj = W;
+  // the gather needs the temperatures of every stage state
+  if (E.nplanes != D_N) {
+    E.nplanes = D_N;
+    if (E.graph) {
+      (void)hipGraphExecDestroy(E.graph);
+      E.graph = nullptr;
+    }
+  }
+  return RH3T_OK;
+}
+
+// OUT = a LAM + b1 X1 + b2 X2 + rscale (dl/dS)^T LAM for the state in
+// buffer buf; parameter cotangents accumulate into W.pacc.
+void launch_vjp(Engine& E, int buf, const double* LAM, double rscale, double a, const double* X1, double b1,
+                const double* X2, double b2, double* OUT) {
+  AdjWork& W = *E.adj;
+  const SolveDesc& d = E.h_desc[0];
+  const Geo& g = d.g;
+  launch_ghosts(E, buf, 2u, false);
+  launch_derived(E, buf);
+  AdjDesc A;
+  A.g = g;
+  A.pt = d.pt;
+  A.S = d.U[buf];
+  A.D = d.D;
+  A.LAM = LAM;
+  A.XB = W.XB;
+  A.YB = W.YB;
+  A.SB = W.SB;
+  A.OUT = OUT;
+  A.X1 = X1;
+  A.X2 = X2;
+  A.a = a;
+  A.b1 = b1;
+  A.b2 = b2;
+  A.rscale = rscale;
+  A.PG = W.PG;
+  {
+    Launcher L(E, K_ADJ_IFACE_X);
+    const int tiles = ((g.ny + kIfBx - 1) / kIfBx) * (g.nx + 1);
+    k_adj_iface<0><<<tiles, kIfBx, 0, E.stream>>>(A, E.eig, E.phys);
+  }
+  {
+    Launcher L(E, K_ADJ_IFACE_Y);
+    const int tiles = ((g.ny + 1 + kIfBx - 1) / kIfBx) * g.nx;
+    k_adj_iface<1><<<tiles, kIfBx, 0, E.stream>>>(A, E.eig, E.phys);
+  }
+  {
+    Launcher L(E, K_ADJ_GATHER);
+    k_adj_gather<<<W.pg_blocks, 256, 0, E.stream>>>(A, E.phys);
+  }
+  if (W.nrows > 0) {
+    Launcher L(E, K_ADJ_GHOST);
+    k_adj_ghost_T<<<(W.nrows + 255) / 256, 256, 0, E.stream>>>(W.nrows, W.row_cell, W.row_ptr, W.ent_pad,
+                                                               W.ent_kind, W.ent_n, W.SB, OUT, g.plane);
+  }
+}
+
+int begin_adjoint_call(Engine& E) {
+  int st = init_adjoint(E);
+  if (st) return st;
+  HIP_TRY(hipSetDevice(E.device));
+  if ((st = reset_errors(E))) return st;
+  AdjWork& W = *E.adj;
+  const long npg = (long)kNParam * W.pg_blocks;
+  k_fill<<<(unsigned)((npg + 255) / 256), 256, 0, E.stream>>>(W.PG, npg, 0.0);
+  k_fill<<<1, 64, 0, E.stream>>>(W.pacc, kNParam, 0.0);
+  HIP_TRY(hipGetLastError());
+  return RH3T_OK;
+}
+
+int upload_cot(Engine& E, const double* aos, double* dst) {
+  const SolveDesc& d = E.h_desc[0];
+  const long ncell = (long)d.g.nx * d.g.ny;
+  HIP_TRY(hipMemcpyAsync(E.adj->aos, aos, 6 * ncell * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  k_aos_to_buf<<<(unsigned)((ncell + 255) / 256), 256, 0, E.stream>>>(E.adj->aos, d.g, d.pt, dst);
+  HIP_TRY(hipGetLastError());
+  return RH3T_OK;
+}
+
+int download_cot(Engine& E, const double* src, double* aos, double* pgrad) {
+  const SolveDesc& d = E.h_desc[0];
+  const long ncell = (long)d.g.nx * d.g.ny;
+  k_buf_to_aos<<<(unsigned)((ncell + 255) / 256), 256, 0, E.stream>>>(src, d.g, d.pt, E.adj->aos);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(aos, E.adj->aos, 6 * ncell * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  if (pgrad) {
+    {
+      Launcher L(E, K_ADJ_PARAM);
+      k_adj_param<<<1, 256, 0, E.stream>>>(E.adj->PG, E.adj->pg_blocks, E.adj->pacc);
+    }
+    HIP_TRY(hipMemcpyAsync(pgrad, E.adj->pacc, kNParam * sizeof(double), hipMemcpyDeviceToHost, E.stream));
+  }
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  return RH3T_OK;
+}
+
+int ensure_dts(Engine& E, int n) {
+  AdjWork& W = *E.adj;
+  if (W.dts_cap >= n) return RH3T_OK;
+  int st = dev_alloc(E, &W.dts, (size_t)n);
+  if (st) return st;
+  W.dts_cap = n;
+  return RH3T_OK;
+}
+
+// Reverse sweep of one RK3 step whose stage states sit in U[0] (U^n),
+// U[1] (S1) and U[2] (S2):
+//   S2b  = 2/3 lam + (2dt/3) J(S2)^T lam
+//   S1b  = 1/4 S2b + (dt/4) J(S1)^T S2b
+//   out  = 1/3 lam + 3/4 S2b + S1b + dt J(U^n)^T S1b
+void launch_rk3_reverse(Engine& E, double dt, const double* lam, double* s2b, double* s1b, double* out) {
+  launch_vjp(E, 2, lam, 2.0 * dt / 3.0, 2.0 / 3.0, nullptr, 0.0, nullptr, 0.0, s2b);
+  launch_vjp(E, 1, s2b, dt / 4.0, 0.25, nullptr, 0.0, nullptr, 0.0, s1b);
+  launch_vjp(E, 0, s1b, dt, 1.0, lam, 1.0 / 3.0, s2b, 0.75, out);
+}
+
+}  // namespace
+
+struct rh3t_adjoint {
+  rh3t_ctx* ctx = nullptr;
+  int every = 5;
+  int nsteps = 0;
+  std::vector<double> dts;
+  std::vector<double*> ckpt;  // U^{kK}, k = 0, 1, ...
+  std::vector<double*> tape;  // 3 stage states per step of one segment
+  double* final_state = nullptr;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int adj_alloc(rh3t_adjoint* A, double** p, size_t n) {
+  Engine& E = A->ctx->e;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, n * sizeof(double));
+  if (e != hipSuccess) {
+    std::snprintf(E.message, sizeof(E.message), "hipMalloc(%zu): %s", n * sizeof(double), hipGetErrorString(e));
+    return RH3T_ERR_HIP;
+  }
+  A->allocs.push_back(q);
+  *p = (double*)q;
+  return RH3T_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys);
+
+int rh3t_l_vjp(rh3t_ctx* ctx, const double* rbar, double* ubar, double* pgrad) {
+  if (!ctx || !rbar || !ubar) return RH3T_ERR_ARG;
+  Engine& E = ctx->e;
+  int st = begin_adjoint_call(E);
+  if (st) return st;
+  AdjWork& W = *E.adj;
+  if ((st = upload_cot(E, rbar, W.lam[0]))) return st;
+  launch_vjp(E, 0, W.lam[0], 1.0, 0.0, nullptr, 0.0, nullptr, 0.0, W.lam[1]);
+  HIP_TRY(hipGetLastError());
+  if ((st = download_cot(E, W.lam[1], ubar, pgrad))) return st;
+  return check_errors(E);
+}
+
+int rh3t_rk3_vjp(rh3t_ctx* ctx, double dt, const double* lam, double* ubar, double* pgrad) {
+  if (!ctx || !lam || !ubar) return RH3T_ERR_ARG;
+  if (!(dt > 0.0)) return RH3T_ERR_DT;
+  Engine& E = ctx->e;
+  int st = begin_adjoint_call(E);
+  if (st) return st;
+  AdjWork& W = *E.adj;
+  if ((st = ensure_dts(E, 1))) return st;
+  HIP_TRY(hipMemcpyAsync(W.dts, &dt, sizeof(double), hipMemcpyHostToDevice, E.stream));
+  k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, W.dts, 0, 0);
+  // forward stages 1 and 2 (U[0] is left untouched)
+  launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3});
+  launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5});
+  if ((st = upload_cot(E, lam, W.lam[0]))) return st;
+  launch_rk3_reverse(E, dt, W.lam[0], W.lam[1], W.lam[2], W.lam[3]);
+  HIP_TRY(hipGetLastError());
+  if ((st = download_cot(E, W.lam[3], ubar, pgrad))) return st;
+  return check_errors(E);
+}
+
+int rh3t_adjoint_create(rh3t_ctx* ctx, int32_t checkpoint_every, rh3t_adjoint** out) {
+  if (!ctx || !out || checkpoint_every < 1) return RH3T_ERR_ARG;
+  *out = nullptr;
+  Engine& E = ctx->e;
+  int st = init_adjoint(E);
+  if (st) return st;
+  auto* A = new rh3t_adjoint();
+  A->ctx = ctx;
+  A->every = checkpoint_every;
+  const size_t plane6 = 6 * (size_t)E.h_desc[0].g.plane;
+  A->tape.assign(3 * (size_t)checkpoint_every, nullptr);
+  for (auto& t : A->tape)
+    if ((st = adj_alloc(A, &t, plane6))) {
+      rh3t_adjoint_destroy(A);
+      return st;
+    }
+  if ((st = adj_alloc(A, &A->final_state, plane6))) {
+    rh3t_adjoint_destroy(A);
+    return st;
+  }
+  *out = A;
+  return RH3T_OK;
+}
+
+void rh3t_adjoint_destroy(rh3t_adjoint* A) {
+  if (!A) return;
+  (void)hipSetDevice(A->ctx->e.device);
+  (void)hipStreamSynchronize(A->ctx->e.stream);
+  for (void* p : A->allocs) (void)hipFree(p);
+  delete A;
+}
+
+int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
+  if (!A || nsteps < 1 || !dt) return RH3T_ERR_ARG;
+  for (int n = 0; n < nsteps; ++n)
+    if (!(dt[n] > 0.0)) return RH3T_ERR_DT;
+  Engine& E = A->ctx->e;
+  int st = begin_adjoint_call(E);
+  if (st) return st;
+  const SolveDesc& d = E.h_desc[0];
+  const size_t bytes = 6 * (size_t)d.g.plane * sizeof(double);
+  const int nck = (nsteps + A->every - 1) / A->every;
+  while ((int)A->ckpt.size() < nck) {
+    double* p;
+    if ((st = adj_alloc(A, &p, bytes / sizeof(double)))) return st;
+    A->ckpt.push_back(p);
+  }
+  if ((st = ensure_dts(E, nsteps))) return st;
+  A->dts.assign(dt, dt + nsteps);
+  A->nsteps = nsteps;
+  HIP_TRY(hipMemcpyAsync(E.adj->dts, dt, nsteps * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  for (int n = 0; n < nsteps; ++n) {
+    if (n % A->every == 0)
+      HIP_TRY(hipMemcpyAsync(A->ckpt[n / A->every], d.U[0], bytes, hipMemcpyDeviceToDevice, E.stream));
+    k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, E.adj->dts, n, 1);
+    launch_rk3(E);
+  }
+  HIP_TRY(hipMemcpyAsync(A->final_state, d.U[0], bytes, hipMemcpyDeviceToDevice, E.stream));
+  HIP_TRY(hipGetLastError());
+  return check_errors(E);
+}
+
+int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_initial, double* pgrad) {
+  if (!A || !lam_final || !lam_initial) return RH3T_ERR_ARG;
+  if (A->nsteps < 1) return RH3T_ERR_ARG;
+  Engine& E = A->ctx->e;
+  int st = begin_adjoint_call(E);
+  if (st) return st;
+  AdjWork& W = *E.adj;
+  const SolveDesc& d = E.h_desc[0];
+  const size_t bytes = 6 * (size_t)d.g.plane * sizeof(double);
+  double* lam = W.lam[0];
+  double* nxt = W.lam[3];
+  if ((st = upload_cot(E, lam_final, lam))) return st;
+  HIP_TRY(hipMemcpyAsync(E.adj->dts, A->dts.data(), A->nsteps * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  const int nseg = (A->nsteps + A->every - 1) / A->every;
+  for (int k = nseg - 1; k >= 0; --k) {
+    const int n0 = k * A->every, n1 = std::min(A->nsteps, n0 + A->every);
+    // recompute the segment from its checkpoint, taping the stage states
+    HIP_TRY(hipMemcpyAsync(d.U[0], A->ckpt[k], bytes, hipMemcpyDeviceToDevice, E.stream));
+    for (int n = n0; n < n1; ++n) {
+      const int m = n - n0;
+      k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, W.dts, n, 0);
+      HIP_TRY(hipMemcpyAsync(A->tape[3 * m], d.U[0], bytes, hipMemcpyDeviceToDevice, E.stream));
+      launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3});
+      HIP_TRY(hipMemcpyAsync(A->tape[3 * m + 1], d.U[1], bytes, hipMemcpyDeviceToDevice, E.stream));
+      launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5});
+      HIP_TRY(hipMemcpyAsync(A->tape[3 * m + 2], d.U[2], bytes, hipMemcpyDeviceToDevice, E.stream));
+      if (n + 1 < n1) launch_stage(E, 2, StageSpec{3, 0, 2, 0, 7});
+    }
+    // reverse sweep over the segment
+    for (int n = n1 - 1; n >= n0; --n) {
+      const int m = n - n0;
+      if (n + 1 < n1) {
+        for (int b = 0; b < 3; ++b)
+          HIP_TRY(hipMemcpyAsync(d.U[b], A->tape[3 * m + b], bytes, hipMemcpyDeviceToDevice, E.stream));
+      }
+      launch_rk3_reverse(E, A->dts[n], lam, W.lam[1], W.lam[2], nxt);
+      std::swap(lam, nxt);
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  if ((st = download_cot(E, lam, lam_initial, pgrad))) return st;
+  // leave the context at the forward's final state
+  HIP_TRY(hipMemcpyAsync(d.U[0], A->final_state, bytes, hipMemcpyDeviceToDevice, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  return check_errors(E);
+}
+
+}  // extern "C"

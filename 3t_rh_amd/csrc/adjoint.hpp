@@ -1,223 +1,30 @@
-// Reverse-mode (VJP) building blocks of l(): the pullbacks of the pointwise
-// pieces of colinhu123/3T_RH's operator, written by hand.
+// Reverse-mode (VJP) building blocks of l(): pullbacks of the pointwise
+// pieces of colinhu123/3T_RH's operator, derived by hand.
 //
 //   derived_vjp      Derived::from_state            src/state.rs:241-275
 //   weno5_vjp        weno5                          src/weno.rs:846-886
 //   interface_vjp    Stencil6::reconstruction_fast  src/weno.rs:685-802
-//                    (L rows / R columns / lambda as functions of the Roe
-//                    state are differentiated with 5-direction forward-mode
-//                    duals, everything else in reverse mode)
-//   noncon_vjp       nonconservative_direction_pre  src/noncon.rs:288-332
+//                    with build_l_plain / build_r_plain (src/weno.rs:28-548)
+//                    and flux_from_derived (src/state.rs:56-85)
 //
-// The derivatives are those of the mathematical function the reference
-// evaluates (kinks of |lambda|, max/min and the WENO weights' branches are
-// taken on the side the forward value selects).
+// They differentiate the mathematical function the forward kernels evaluate
+// (both arithmetic variants compute the same function); at the kinks of
+// |lambda| and of max/min the derivative of the branch the forward value
+// takes is used.  The adjoint is not held to the forward's rounding
+// sequence, so products and sums may contract to FMA here.
 #pragma once
 
 #include "device_math.hpp"
 #include "flux.hpp"
 
+#pragma clang fp contract(fast)
+
 namespace rh3t_dev {
-
-// ----------------------------------------------------------- Dual5
-
-struct D5 {
-  double v, d[5];
-};
-__device__ __forceinline__ D5 dconst(double x) {
-  D5 r;
-  r.v = x;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) r.d[k] = 0.0;
-  return r;
-}
-__device__ __forceinline__ D5 dvar(double x, int k) {
-  D5 r = dconst(x);
-  r.d[k] = 1.0;
-  return r;
-}
-__device__ __forceinline__ D5 operator+(const D5& a, const D5& b) {
-  D5 r;
-  r.v = a.v + b.v;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] + b.d[k];
-  return r;
-}
-__device__ __forceinline__ D5 operator-(const D5& a, const D5& b) {
-  D5 r;
-  r.v = a.v - b.v;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] - b.d[k];
-  return r;
-}
-__device__ __forceinline__ D5 operator-(const D5& a) {
-  D5 r;
-  r.v = -a.v;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) r.d[k] = -a.d[k];
-  return r;
-}
-__device__ __forceinline__ D5 operator*(const D5& a, const D5& b) {
-  D5 r;
-  r.v = a.v * b.v;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
-  return r;
-}
-__device__ __forceinline__ D5 operator*(double a, const D5& b) {
-  D5 r;
-  r.v = a * b.v;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) r.d[k] = a * b.d[k];
-  return r;
-}
-__device__ __forceinline__ D5 operator/(const D5& a, const D5& b) {
-  D5 r;
-  const double ib = 1.0 / b.v;
-  r.v = a.v * ib;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
-  return r;
-}
-__device__ __forceinline__ D5 operator/(double a, const D5& b) { return dconst(a) / b; }
-__device__ __forceinline__ D5 dsqrt(const D5& a) {
-  D5 r;
-  r.v = sqrt(a.v);
-  const double h = 0.5 / r.v;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * h;
-  return r;
-}
-
-// Roe-state functions of (u, v, e_e, e_i, e_r) with the closed forms of
-// src/weno.rs:28-548 (flux.hpp keeps the production rounding; these carry
-// derivatives).
-template <int DIR>
-struct RoeD {
-  D5 u, v, w2, cs, cs2, b, h[3], gk_[3], pcs, cs2w2, gtcs2;
-};
-
-template <int DIR>
-__device__ __forceinline__ void roe_dual(const EigConst& K, double u, double v, const double e[3], RoeD<DIR>& R) {
-  R.u = dvar(u, 0);
-  R.v = dvar(v, 1);
-  D5 E[3] = {dvar(e[0], 2), dvar(e[1], 3), dvar(e[2], 4)};
-  R.w2 = R.u * R.u + R.v * R.v;
-  R.cs2 = K.gamg[0] * E[0] + K.gamg[1] * E[1] + K.gamg[2] * E[2];
-  R.cs = dsqrt(R.cs2);
-  D5 ge_e = K.gam[0] * E[0], gi_e = K.gam[1] * E[1], gr_e = K.gam[2] * E[2];
-  R.h[0] = K.h6[0] * (gi_e - gr_e) + K.gam_gt[0] * E[0] * R.w2;
-  R.h[1] = K.h6[1] * (gr_e - ge_e) + K.gam_gt[1] * E[1] * R.w2;
-  R.h[2] = K.h6[2] * (ge_e - gi_e) + K.gam_gt[2] * E[2] * R.w2;
-  R.b = K.gt * (dconst(K.c36) + K.gt * (R.w2 * R.w2)) * R.cs2;
-  R.gtcs2 = K.gt * R.cs2;
-  R.cs2w2 = R.cs2 * R.w2;
-  D5 w6 = (1.0 / 6.0) * R.w2;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) R.gk_[k] = K.gam[k] * E[k] + w6;
-  R.pcs = (1.0 / 3.0) * ((DIR == 0 ? R.u : R.v) * R.cs);
-}
-
-template <int DIR>
-__device__ __forceinline__ void l_row_dual(const EigConst& K, const RoeD<DIR>& R, int c, D5 row[6]) {
-  const int tr = DIR == 0 ? 4 : 1;
-  if (c == 0 || c == 5) {
-    D5 iad = 1.0 / (12.0 * R.cs2);
-    D5 wv = DIR == 0 ? R.u : R.v;
-    D5 six_w_cs = 6.0 * (wv * R.cs);
-    D5 a0 = c == 0 ? K.gt * R.w2 + six_w_cs : K.gt * R.w2 - six_w_cs;
-    D5 six_cs = 6.0 * R.cs;
-    D5 a1, a2;
-    if (DIR == 0) {
-      a1 = c == 0 ? K.m2gt * R.u - six_cs : K.m2gt * R.u + six_cs;
-      a2 = K.m2gt * R.v;
-    } else {
-      a1 = K.m2gt * R.u;
-      a2 = c == 0 ? K.m2gt * R.v - six_cs : K.m2gt * R.v + six_cs;
-    }
-    row[0] = a0 * iad;
-    row[1] = a1 * iad;
-    row[2] = a2 * iad;
-    row[3] = K.six_g[0] * iad;
-    row[4] = K.six_g[1] * iad;
-    row[5] = K.six_g[2] * iad;
-    return;
-  }
-  if (c == tr) {
-    row[0] = DIR == 0 ? -R.v : -R.u;
-    row[1] = dconst(DIR == 0 ? 0.0 : 1.0);
-    row[2] = dconst(DIR == 0 ? 1.0 : 0.0);
-    row[3] = row[4] = row[5] = dconst(0.0);
-    return;
-  }
-  const int k = DIR == 0 ? c - 1 : c - 2;
-  const double gk = K.g[k];
-  D5 hk = R.h[k];
-  D5 A = hk - R.cs2w2;
-  D5 ib = 1.0 / R.b;
-  D5 ics2 = 1.0 / R.cs2;
-  D5 t = ib * A;
-  row[0] = dconst(K.g_over_gt[k]) - (gk / 6.0) * (R.w2 * ics2) - K.g_gt[k] * (R.w2 * t);
-  row[1] = (gk / 3.0) * (R.u * ics2) + K.two_g_gt[k] * (R.u * t);
-  row[2] = (gk / 3.0) * (R.v * ics2) + K.two_g_gt[k] * (R.v * t);
-  D5 igtcs2 = 1.0 / R.gtcs2;
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    D5 inner;
-    if (j == k) {
-      inner = gk * hk - R.gtcs2 * R.w2;
-    } else {
-      const int m = 3 - k - j;
-      D5 t6 = K.six_g[m] * R.cs2;
-      inner = (j == (k + 1) % 3) ? hk - t6 : hk + t6;
-    }
-    row[3 + j] = K.num[k][j] * igtcs2 - K.coef[k][j] * (ib * inner);
-  }
-}
-
-// column c of R
-template <int DIR>
-__device__ __forceinline__ void r_col_dual(const EigConst& K, const RoeD<DIR>& R, int c, D5 col[6]) {
-  const int tr = DIR == 0 ? 4 : 1;
-  if (c == tr) {
-    D5 w3 = (1.0 / 3.0) * (DIR == 0 ? R.v : R.u);
-    col[0] = dconst(0.0);
-    col[1] = dconst(DIR == 0 ? 0.0 : 1.0);
-    col[2] = dconst(DIR == 0 ? 1.0 : 0.0);
-    col[3] = col[4] = col[5] = w3;
-    return;
-  }
-  col[0] = dconst(1.0);
-  if (c == 0 || c == 5) {
-    col[1] = DIR == 0 ? (c == 0 ? R.u - R.cs : R.u + R.cs) : R.u;
-    col[2] = DIR == 1 ? (c == 0 ? R.v - R.cs : R.v + R.cs) : R.v;
-#pragma unroll
-    for (int r = 0; r < 3; ++r) col[3 + r] = c == 0 ? R.gk_[r] - R.pcs : R.gk_[r] + R.pcs;
-    return;
-  }
-  col[1] = R.u;
-  col[2] = R.v;
-  const int k = DIR == 0 ? c - 1 : c - 2;
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    if (j == k) {
-      col[3 + j] = (K.gt * K.inv6g[k]) * R.w2;
-    } else {
-      const int m = 3 - k - j;
-      col[3 + j] = dconst((j == (k + 1) % 3) ? K.g[m] : -K.g[m]);
-    }
-  }
-}
-
-template <int DIR>
-__device__ __forceinline__ D5 lambda_dual(const RoeD<DIR>& R, int c) {
-  D5 w = DIR == 0 ? R.u : R.v;
-  return c == 0 ? w - R.cs : (c == 5 ? w + R.cs : w);
-}
 
 // ----------------------------------------------------------- WENO5
 
-// value of weno5 and the gradient of s * weno5 w.r.t. its 5 inputs
+// Value of weno5(u) (src/weno.rs:846-886: alpha_k = d_k / (eps + beta_k)^2)
+// and g = s * d weno5 / d u.
 __device__ __forceinline__ double weno5_vjp(const double u[5], double eps, double s, double g[5]) {
   const double A0 = u[0] - 2.0 * u[1] + u[2], B0 = u[0] - 4.0 * u[1] + 3.0 * u[2];
   const double A1 = u[1] - 2.0 * u[2] + u[3], B1 = u[1] - u[3];
@@ -225,35 +32,34 @@ __device__ __forceinline__ double weno5_vjp(const double u[5], double eps, doubl
   const double beta[3] = {13.0 / 12.0 * A0 * A0 + 0.25 * B0 * B0, 13.0 / 12.0 * A1 * A1 + 0.25 * B1 * B1,
                           13.0 / 12.0 * A2 * A2 + 0.25 * B2 * B2};
   const double dk[3] = {0.1, 0.6, 0.3};
-  double a[3], S = 0.0;
+  double a[3], ie[3], S = 0.0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double e = eps + beta[k];
-    a[k] = dk[k] / (e * e);
+    ie[k] = 1.0 / (eps + beta[k]);
+    a[k] = dk[k] * ie[k] * ie[k];
     S += a[k];
   }
-  const double p[3] = {u[0] / 3.0 - 7.0 / 6.0 * u[1] + 11.0 / 6.0 * u[2],
-                       -u[1] / 6.0 + 5.0 / 6.0 * u[2] + u[3] / 3.0,
-                       u[2] / 3.0 + 5.0 / 6.0 * u[3] - u[4] / 6.0};
-  double W = 0.0;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) W += a[k] / S * p[k];
-  // p_k pullback
-  const double pb[3] = {s * a[0] / S, s * a[1] / S, s * a[2] / S};
-  g[0] = pb[0] / 3.0;
-  g[1] = -7.0 / 6.0 * pb[0] - pb[1] / 6.0;
-  g[2] = 11.0 / 6.0 * pb[0] + 5.0 / 6.0 * pb[1] + pb[2] / 3.0;
-  g[3] = pb[1] / 3.0 + 5.0 / 6.0 * pb[2];
-  g[4] = -pb[2] / 6.0;
-  // beta_k pullback: dW/dbeta_k = (da_k/dbeta_k) (p_k - W) / S,
-  // da_k/dbeta_k = -2 a_k / (eps + beta_k)
+  const double iS = 1.0 / S;
+  const double p[3] = {u[0] * (1.0 / 3.0) - 7.0 / 6.0 * u[1] + 11.0 / 6.0 * u[2],
+                       -u[1] * (1.0 / 6.0) + 5.0 / 6.0 * u[2] + u[3] * (1.0 / 3.0),
+                       u[2] * (1.0 / 3.0) + 5.0 / 6.0 * u[3] - u[4] * (1.0 / 6.0)};
+  const double W = (a[0] * p[0] + a[1] * p[1] + a[2] * p[2]) * iS;
+  // candidate polynomials
+  const double sS = s * iS;
+  const double pb[3] = {sS * a[0], sS * a[1], sS * a[2]};
+  g[0] = pb[0] * (1.0 / 3.0);
+  g[1] = -7.0 / 6.0 * pb[0] - pb[1] * (1.0 / 6.0);
+  g[2] = 11.0 / 6.0 * pb[0] + 5.0 / 6.0 * pb[1] + pb[2] * (1.0 / 3.0);
+  g[3] = pb[1] * (1.0 / 3.0) + 5.0 / 6.0 * pb[2];
+  g[4] = -pb[2] * (1.0 / 6.0);
+  // weights: dW/dbeta_k = (da_k/dbeta_k) (p_k - W) / S, da_k/dbeta_k = -2 a_k / (eps + beta_k)
   double bb[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) bb[k] = s * (-2.0 * a[k] / (eps + beta[k])) * (p[k] - W) / S;
+  for (int k = 0; k < 3; ++k) bb[k] = -2.0 * pb[k] * ie[k] * (p[k] - W);
   // beta = 13/12 A^2 + 1/4 B^2
-  const double a0 = bb[0] * 13.0 / 6.0 * A0, b0 = bb[0] * 0.5 * B0;
-  const double a1 = bb[1] * 13.0 / 6.0 * A1, b1 = bb[1] * 0.5 * B1;
-  const double a2 = bb[2] * 13.0 / 6.0 * A2, b2 = bb[2] * 0.5 * B2;
+  const double a0 = bb[0] * (13.0 / 6.0) * A0, b0 = bb[0] * 0.5 * B0;
+  const double a1 = bb[1] * (13.0 / 6.0) * A1, b1 = bb[1] * 0.5 * B1;
+  const double a2 = bb[2] * (13.0 / 6.0) * A2, b2 = bb[2] * 0.5 * B2;
   g[0] += a0 + b0;
   g[1] += -2.0 * a0 - 4.0 * b0 + a1 + b1;
   g[2] += a0 + 3.0 * b0 - 2.0 * a1 + a2 + 3.0 * b2;
@@ -264,278 +70,387 @@ __device__ __forceinline__ double weno5_vjp(const double u[5], double eps, doubl
 
 // ------------------------------------------------------- Derived VJP
 
-// db: (u, v, e_e, e_i, e_r, p_e, p_i, p_r, T_e, T_i, T_r) cotangents;
-// accumulates the state cotangent into sb.
-__device__ __forceinline__ void derived_vjp(const Phys& ph, const double s[6], const double db[11], double sb[6]) {
-  const double rho = s[0], u = s[1] / rho, v = s[2] / rho;
+// db: cotangents of (u, v, e_e, e_i, e_r, p_e, p_i, p_r, T_e, T_i, T_r) of
+// one point; accumulates the state cotangent into sb.
+__device__ __forceinline__ void derived_vjp(const Phys& ph, const double s[6], const double db[D_N], double sb[6]) {
+  const double rho = s[0], ir = 1.0 / rho, u = s[1] * ir, v = s[2] * ir;
   const double w2 = u * u + v * v;
   const double g[3] = {ph.ge, ph.gi, ph.gr};
-  const double X = s[5] - rho * w2 / 6.0;  // rho e_r, Tr = (X/A)^(1/4)
+  // T_r = (X / A)^(1/4), X = E_r - rho w2 / 6
+  const double X = s[5] - rho * w2 * (1.0 / 6.0);
   double Xb = 0.0;
-  if (db[10] != 0.0 && X > 0.0) {
-    const double tr = pow(X / ph.a_rad, 0.25);
-    Xb = db[10] * 0.25 * tr / X;
-  }
-  double w2b = 0.0;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) w2b += -db[2 + k] / 6.0 - db[5 + k] * g[k] * rho / 6.0;
-  w2b += -db[8] / (6.0 * ph.cve) - db[9] / (6.0 * ph.cvi) - Xb * rho / 6.0;
-  const double ub = db[0] + 2.0 * u * w2b, vb = db[1] + 2.0 * v * w2b;
-  sb[1] += ub / rho;
-  sb[2] += vb / rho;
-  double rb = -(u * ub + v * vb) / rho;
+  if (db[D_TR] != 0.0 && X > 0.0) Xb = db[D_TR] * 0.25 * pow(X / ph.a_rad, 0.25) / X;
+  const double icve = 1.0 / ph.cve, icvi = 1.0 / ph.cvi;
+  double w2b = -(db[D_EE] + db[D_EI] + db[D_ER]) * (1.0 / 6.0) -
+               (db[D_PE] * g[0] + db[D_PI] * g[1] + db[D_PR] * g[2]) * rho * (1.0 / 6.0) -
+               (db[D_TE] * icve + db[D_TI] * icvi) * (1.0 / 6.0) - Xb * rho * (1.0 / 6.0);
+  const double ub = db[D_U] + 2.0 * u * w2b, vb = db[D_V] + 2.0 * v * w2b;
+  double rb = -(u * ub + v * vb) * ir;
+  const double pk[3] = {db[D_PE], db[D_PI], db[D_PR]};
+  const double ek[3] = {db[D_EE], db[D_EI], db[D_ER]};
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    rb += -db[2 + k] * s[3 + k] / (rho * rho) - db[5 + k] * g[k] * w2 / 6.0;
-    sb[3 + k] += db[2 + k] / rho + db[5 + k] * g[k];
+    rb += -ek[k] * s[3 + k] * ir * ir - pk[k] * g[k] * w2 * (1.0 / 6.0);
+    sb[3 + k] += ek[k] * ir + pk[k] * g[k];
   }
-  rb += -db[8] * s[3] / (rho * rho * ph.cve) - db[9] * s[4] / (rho * rho * ph.cvi) - Xb * w2 / 6.0;
-  sb[3] += db[8] / (rho * ph.cve);
-  sb[4] += db[9] / (rho * ph.cvi);
-  sb[5] += Xb;
+  rb += -(db[D_TE] * s[3] * icve + db[D_TI] * s[4] * icvi) * ir * ir - Xb * w2 * (1.0 / 6.0);
   sb[0] += rb;
+  sb[1] += ub * ir;
+  sb[2] += vb * ir;
+  sb[3] += db[D_TE] * ir * icve;
+  sb[4] += db[D_TI] * ir * icvi;
+  sb[5] += Xb;
 }
+
+// ------------------------------------------------ Roe eigensystem VJP
+
+// Roe-state intermediates of build_l_plain / build_r_plain
+// (src/weno.rs:28-548) as functions of (u, v, e_e, e_i, e_r).
+struct RoeV {
+  double u, v, w, w2, cs2, cs, b, ics2, ib, igtcs2, gtcs2, cs2w2, pcs;
+  double e[3], h[3], gk[3];
+};
+
+// cotangents of those intermediates
+struct RoeBar {
+  double u, v, w2, cs, cs2, b, ics2, ib, igtcs2, pcs;
+  double h[3], gk[3];
+};
+
+template <int DIR>
+__device__ __forceinline__ void roe_values(const EigConst& K, double u, double v, const double e[3], RoeV& R) {
+  R.u = u;
+  R.v = v;
+  R.w = DIR == 0 ? u : v;
+  R.w2 = u * u + v * v;
+  R.cs2 = K.gamg[0] * e[0] + K.gamg[1] * e[1] + K.gamg[2] * e[2];
+  R.cs = sqrt(R.cs2);
+  const double ge = K.gam[0] * e[0], gi = K.gam[1] * e[1], gr = K.gam[2] * e[2];
+  R.h[0] = K.h6[0] * (gi - gr) + K.gam_gt[0] * e[0] * R.w2;
+  R.h[1] = K.h6[1] * (gr - ge) + K.gam_gt[1] * e[1] * R.w2;
+  R.h[2] = K.h6[2] * (ge - gi) + K.gam_gt[2] * e[2] * R.w2;
+  R.b = K.gt * (K.c36 + K.gt * R.w2 * R.w2) * R.cs2;
+  R.ics2 = 1.0 / R.cs2;
+  R.ib = 1.0 / R.b;
+  R.gtcs2 = K.gt * R.cs2;
+  R.igtcs2 = 1.0 / R.gtcs2;
+  R.cs2w2 = R.cs2 * R.w2;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    R.e[k] = e[k];
+    R.gk[k] = K.gam[k] * e[k] + R.w2 * (1.0 / 6.0);
+  }
+  R.pcs = R.w * R.cs * (1.0 / 3.0);
+}
+
+__device__ __forceinline__ double pick3(int k, double a0, double a1, double a2) {
+  return k == 0 ? a0 : (k == 1 ? a1 : a2);
+}
+
+// the bracket of entry (k, j) of the energy rows (src/weno.rs:110-233):
+// g_k h_k - gt cs2 w2 on the diagonal, h_k -/+ 6 g_m cs2 off it
+__device__ __forceinline__ double energy_inner(const EigConst& K, const RoeV& R, int k, int j, double gk,
+                                               double hk) {
+  if (j == k) return gk * hk - R.gtcs2 * R.w2;
+  const int m = 3 - k - j;
+  const double t6 = pick3(m, K.six_g[0], K.six_g[1], K.six_g[2]) * R.cs2;
+  return (j == (k + 1) % 3) ? hk - t6 : hk + t6;
+}
+
+// Row c of L, column c of R and lambda_c.  X rows: u-c, e, i, r,
+// transverse, u+c; Y rows: v-c, transverse, e, i, r, v+c.
+template <int DIR>
+__device__ __forceinline__ void row_col(const EigConst& K, const RoeV& R, int c, double row[6], double col[6],
+                                        double* lam) {
+  const int tr = DIR == 0 ? 4 : 1;
+  if (c == 0 || c == 5) {
+    const double sg = c == 0 ? 1.0 : -1.0;
+    const double iad = R.ics2 * (1.0 / 12.0);
+    row[0] = (K.gt * R.w2 + sg * 6.0 * R.w * R.cs) * iad;
+    row[1] = (K.m2gt * R.u - (DIR == 0 ? sg * 6.0 * R.cs : 0.0)) * iad;
+    row[2] = (K.m2gt * R.v - (DIR == 1 ? sg * 6.0 * R.cs : 0.0)) * iad;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) row[3 + k] = K.six_g[k] * iad;
+    col[0] = 1.0;
+    col[1] = DIR == 0 ? R.u - sg * R.cs : R.u;
+    col[2] = DIR == 1 ? R.v - sg * R.cs : R.v;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) col[3 + k] = R.gk[k] - sg * R.pcs;
+    *lam = R.w - sg * R.cs;
+    return;
+  }
+  if (c == tr) {
+    const double t = DIR == 0 ? R.v : R.u;
+    row[0] = -t;
+    row[1] = DIR == 0 ? 0.0 : 1.0;
+    row[2] = DIR == 0 ? 1.0 : 0.0;
+    row[3] = row[4] = row[5] = 0.0;
+    col[0] = 0.0;
+    col[1] = DIR == 0 ? 0.0 : 1.0;
+    col[2] = DIR == 0 ? 1.0 : 0.0;
+    col[3] = col[4] = col[5] = t * (1.0 / 3.0);
+    *lam = R.w;
+    return;
+  }
+  // energy row k (runtime k: values are picked, never indexed)
+  const int k = DIR == 0 ? c - 1 : c - 2;
+  const double gk = pick3(k, K.g[0], K.g[1], K.g[2]), hk = pick3(k, R.h[0], R.h[1], R.h[2]);
+  const double t = R.ib * (hk - R.cs2w2);
+  row[0] = pick3(k, K.g_over_gt[0], K.g_over_gt[1], K.g_over_gt[2]) - gk * (1.0 / 6.0) * R.w2 * R.ics2 -
+           pick3(k, K.g_gt[0], K.g_gt[1], K.g_gt[2]) * R.w2 * t;
+  const double cuv = gk * (1.0 / 3.0) * R.ics2 + pick3(k, K.two_g_gt[0], K.two_g_gt[1], K.two_g_gt[2]) * t;
+  row[1] = cuv * R.u;
+  row[2] = cuv * R.v;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double inner = energy_inner(K, R, k, j, gk, hk);
+    row[3 + j] = pick3(k, K.num[0][j], K.num[1][j], K.num[2][j]) * R.igtcs2 -
+                 pick3(k, K.coef[0][j], K.coef[1][j], K.coef[2][j]) * R.ib * inner;
+  }
+  col[0] = 1.0;
+  col[1] = R.u;
+  col[2] = R.v;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int m = 3 - k - j;
+    const double gm = pick3(m, K.g[0], K.g[1], K.g[2]);
+    col[3 + j] = j == k ? K.gt * pick3(k, K.inv6g[0], K.inv6g[1], K.inv6g[2]) * R.w2
+                        : ((j == (k + 1) % 3) ? gm : -gm);
+  }
+  *lam = R.w;
+}
+
+// Accumulates into B the pullback of (row c, column c, lambda_c) with
+// cotangents Lb, Rb, lamb.
+template <int DIR>
+__device__ __forceinline__ void row_col_bar(const EigConst& K, const RoeV& R, int c, const double Lb[6],
+                                            const double Rb[6], double lamb, RoeBar& B) {
+  const int tr = DIR == 0 ? 4 : 1;
+  double wb = 0.0;
+  if (c == 0 || c == 5) {
+    const double sg = c == 0 ? 1.0 : -1.0;
+    const double iad = R.ics2 * (1.0 / 12.0);
+    const double A0 = K.gt * R.w2 + sg * 6.0 * R.w * R.cs;
+    const double A1 = K.m2gt * R.u - (DIR == 0 ? sg * 6.0 * R.cs : 0.0);
+    const double A2 = K.m2gt * R.v - (DIR == 1 ? sg * 6.0 * R.cs : 0.0);
+    const double iadb = Lb[0] * A0 + Lb[1] * A1 + Lb[2] * A2 + Lb[3] * K.six_g[0] + Lb[4] * K.six_g[1] +
+                        Lb[5] * K.six_g[2];
+    const double Ab0 = Lb[0] * iad, Ab1 = Lb[1] * iad, Ab2 = Lb[2] * iad;
+    B.w2 += K.gt * Ab0;
+    wb += sg * 6.0 * R.cs * Ab0;
+    B.cs += sg * 6.0 * R.w * Ab0;
+    B.u += K.m2gt * Ab1;
+    B.v += K.m2gt * Ab2;
+    B.cs -= sg * 6.0 * (DIR == 0 ? Ab1 : Ab2);
+    B.ics2 += iadb * (1.0 / 12.0);
+    // column (1, u -/+ c, v, gk - pcs) or its Y form
+    B.u += Rb[1];
+    B.v += Rb[2];
+    B.cs -= sg * (DIR == 0 ? Rb[1] : Rb[2]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      B.gk[r] += Rb[3 + r];
+      B.pcs -= sg * Rb[3 + r];
+    }
+    wb += lamb;
+    B.cs -= sg * lamb;
+  } else if (c == tr) {
+    const double tb = -Lb[0] + (Rb[3] + Rb[4] + Rb[5]) * (1.0 / 3.0);
+    if (DIR == 0)
+      B.v += tb;
+    else
+      B.u += tb;
+    wb += lamb;
+  } else {
+    const int k = DIR == 0 ? c - 1 : c - 2;
+    const double gk = pick3(k, K.g[0], K.g[1], K.g[2]), hk = pick3(k, R.h[0], R.h[1], R.h[2]);
+    const double g_gt = pick3(k, K.g_gt[0], K.g_gt[1], K.g_gt[2]);
+    const double two_g_gt = pick3(k, K.two_g_gt[0], K.two_g_gt[1], K.two_g_gt[2]);
+    const double A = hk - R.cs2w2;
+    const double t = R.ib * A;
+    const double ulv = R.u * Lb[1] + R.v * Lb[2];
+    B.w2 += -(gk * (1.0 / 6.0) * R.ics2 + g_gt * t) * Lb[0];
+    B.ics2 += -gk * (1.0 / 6.0) * R.w2 * Lb[0] + gk * (1.0 / 3.0) * ulv;
+    const double tb = -g_gt * R.w2 * Lb[0] + two_g_gt * ulv;
+    const double cuv = gk * (1.0 / 3.0) * R.ics2 + two_g_gt * t;
+    B.u += cuv * Lb[1];
+    B.v += cuv * Lb[2];
+    double hb = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double l = Lb[3 + j];
+      const double coef = pick3(k, K.coef[0][j], K.coef[1][j], K.coef[2][j]);
+      const double inner = energy_inner(K, R, k, j, gk, hk);
+      const double innerb = -coef * R.ib * l;
+      if (j == k) {
+        hb += gk * innerb;
+        B.cs2 -= K.gt * R.w2 * innerb;
+        B.w2 -= R.gtcs2 * innerb;
+      } else {
+        const int m = 3 - k - j;
+        const double sgn = (j == (k + 1) % 3) ? -1.0 : 1.0;
+        hb += innerb;
+        B.cs2 += sgn * pick3(m, K.six_g[0], K.six_g[1], K.six_g[2]) * innerb;
+      }
+      B.igtcs2 += pick3(k, K.num[0][j], K.num[1][j], K.num[2][j]) * l;
+      B.ib -= coef * inner * l;
+    }
+    B.ib += tb * A;
+    const double Ab = tb * R.ib;
+    hb += Ab;
+    B.cs2 -= R.w2 * Ab;
+    B.w2 -= R.cs2 * Ab;
+    B.h[0] += k == 0 ? hb : 0.0;
+    B.h[1] += k == 1 ? hb : 0.0;
+    B.h[2] += k == 2 ? hb : 0.0;
+    // column (1, u, v, .., gt w2 / (6 g_k), ..)
+    B.u += Rb[1];
+    B.v += Rb[2];
+    B.w2 += K.gt * pick3(k, K.inv6g[0], K.inv6g[1], K.inv6g[2]) * pick3(k, Rb[3], Rb[4], Rb[5]);
+    wb += lamb;
+  }
+  if (DIR == 0)
+    B.u += wb;
+  else
+    B.v += wb;
+}
+
+// Chains the intermediate cotangents back to (u, v, e_e, e_i, e_r).
+template <int DIR>
+__device__ __forceinline__ void roe_bar_final(const EigConst& K, const RoeV& R, RoeBar B, double out[5]) {
+  B.b -= B.ib * R.ib * R.ib;
+  B.cs2 -= B.ics2 * R.ics2 * R.ics2 + B.igtcs2 * K.gt * R.igtcs2 * R.igtcs2;
+  // b = gt (c36 + gt w2^2) cs2
+  B.w2 += B.b * K.gt * K.gt * 2.0 * R.w2 * R.cs2;
+  B.cs2 += B.b * K.gt * (K.c36 + K.gt * R.w2 * R.w2);
+  // pcs = w cs / 3
+  const double wb = B.pcs * R.cs * (1.0 / 3.0);
+  if (DIR == 0)
+    B.u += wb;
+  else
+    B.v += wb;
+  B.cs += B.pcs * R.w * (1.0 / 3.0);
+  double eb[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    eb[r] = K.gam[r] * B.gk[r];
+    B.w2 += B.gk[r] * (1.0 / 6.0);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    eb[k1] += K.h6[k] * K.gam[k1] * B.h[k];
+    eb[k2] -= K.h6[k] * K.gam[k2] * B.h[k];
+    eb[k] += K.gam_gt[k] * R.w2 * B.h[k];
+    B.w2 += K.gam_gt[k] * R.e[k] * B.h[k];
+  }
+  B.cs2 += B.cs * 0.5 / R.cs;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) eb[k] += K.gamg[k] * B.cs2;
+  out[0] = B.u + 2.0 * R.u * B.w2;
+  out[1] = B.v + 2.0 * R.v * B.w2;
+  out[2] = eb[0];
+  out[3] = eb[1];
+  out[4] = eb[2];
+}
+
+// Keeps the compiler from carrying loaded stencil data across the phases of
+// interface_vjp in registers (it is re-read from L1/L2 instead).
+__device__ __forceinline__ void reload_barrier() { __asm__ volatile("" ::: "memory"); }
 
 // -------------------------------------------------- interface flux VJP
 
-// Pullback of one interface flux (interface between stencil points 2 and
-// 3).  Inputs as for interface_flux; ob: flux cotangent.  Outputs: Ub[q][6]
-// state cotangents and dbq[q][5] cotangents of (u, v, p_e, p_i, p_r) of
-// every point, eb2/eb3 cotangents of (e_e, e_i, e_r) at points 2 and 3.
-template <int DIR>
-__device__ __forceinline__ void interface_vjp(const EigConst& K, double eps, const double U[6][6],
-                                              const double dq[6][5], const double e3[3], const double e2[3],
-                                              const double ob[6], double Ub[6][6], double dbq[6][5], double eb3[3],
-                                              double eb2[3]) {
-  // physical fluxes
-  double F[6][6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const double u = dq[q][0], v = dq[q][1], ptot = dq[q][2] + dq[q][3] + dq[q][4];
-    const double w = DIR == 0 ? u : v;
-    F[q][0] = U[q][0] * w;
-    F[q][1] = DIR == 0 ? U[q][0] * u * u + ptot : U[q][0] * u * v;
-    F[q][2] = DIR == 0 ? U[q][0] * u * v : U[q][0] * v * v + ptot;
-    F[q][3] = (U[q][3] + dq[q][2]) * w;
-    F[q][4] = (U[q][4] + dq[q][3]) * w;
-    F[q][5] = (U[q][5] + dq[q][4]) * w;
-  }
-  // Roe state (points 3 and 2)
-  const double r1 = sqrt(U[3][0]), r2 = sqrt(U[2][0]), rs = r1 + r2;
-  const double ru = (dq[3][0] * r1 + dq[2][0] * r2) / rs;
-  const double rv = (dq[3][1] * r1 + dq[2][1] * r2) / rs;
-  double re[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) re[k] = (e3[k] * r1 + e2[k] * r2) / rs;
-  RoeD<DIR> R;
-  roe_dual<DIR>(K, ru, rv, re, R);
-  double roeb[5] = {0, 0, 0, 0, 0};
-  double Fb[6][6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q)
-#pragma unroll
-    for (int f = 0; f < 6; ++f) {
-      Fb[q][f] = 0.0;
-      Ub[q][f] = 0.0;
-    }
-  for (int c = 0; c < 6; ++c) {
-    D5 row[6], col[6];
-    l_row_dual<DIR>(K, R, c, row);
-    r_col_dual<DIR>(K, R, c, col);
-    const D5 lam = lambda_dual<DIR>(R, c);
-    const double a = fabs(lam.v);
-    double cf[6], cu[6], fp[6], fm[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      cf[q] = 0.0;
-      cu[q] = 0.0;
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        cf[q] += row[f].v * F[q][f];
-        cu[q] += row[f].v * U[q][f];
-      }
-      fp[q] = 0.5 * (cf[q] + a * cu[q]);
-      fm[q] = 0.5 * (cf[q] - a * cu[q]);
-    }
-    // forward value of this characteristic flux, its cotangent
-    double fcb = 0.0;
-#pragma unroll
-    for (int r = 0; r < 6; ++r) fcb += col[r].v * ob[r];
-    double gp[5], gm[5];
-    const double sp[5] = {fp[0], fp[1], fp[2], fp[3], fp[4]};
-    const double sm[5] = {fm[5], fm[4], fm[3], fm[2], fm[1]};
-    const double fc = weno5_vjp(sp, eps, fcb, gp) + weno5_vjp(sm, eps, fcb, gm);
-    double fpb[6] = {gp[0], gp[1], gp[2], gp[3], gp[4], 0.0};
-    double fmb[6] = {0.0, gm[4], gm[3], gm[2], gm[1], gm[0]};
-    double ab = 0.0;
-    double Lb[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      const double cfb = 0.5 * (fpb[q] + fmb[q]);
-      const double cub = 0.5 * a * (fpb[q] - fmb[q]);
-      ab += 0.5 * cu[q] * (fpb[q] - fmb[q]);
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        Lb[f] += cfb * F[q][f] + cub * U[q][f];
-        Fb[q][f] += cfb * row[f].v;
-        Ub[q][f] += cub * row[f].v;
-      }
-    }
-    const double lamb = lam.v >= 0.0 ? ab : -ab;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      double acc = lamb * lam.d[k];
-#pragma unroll
-      for (int f = 0; f < 6; ++f) acc += Lb[f] * row[f].d[k] + ob[f] * fc * col[f].d[k];
-      roeb[k] += acc;
-    }
-  }
-  // physical flux pullback -> U and (u, v, p_k) of each point
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const double u = dq[q][0], v = dq[q][1];
-    const double rho = U[q][0];
-    const double w = DIR == 0 ? u : v;
-    double ub = 0.0, vb = 0.0, wb = 0.0;
-    // F0 = rho w
-    Ub[q][0] += Fb[q][0] * w;
-    wb += Fb[q][0] * rho;
-    if (DIR == 0) {
-      // F1 = rho u u + P, F2 = rho u v
-      Ub[q][0] += Fb[q][1] * u * u + Fb[q][2] * u * v;
-      ub += Fb[q][1] * 2.0 * rho * u + Fb[q][2] * rho * v;
-      vb += Fb[q][2] * rho * u;
-    } else {
-      Ub[q][0] += Fb[q][1] * u * v + Fb[q][2] * v * v;
-      ub += Fb[q][1] * rho * v;
-      vb += Fb[q][1] * rho * u + Fb[q][2] * 2.0 * rho * v;
-    }
-    const double Pb = DIR == 0 ? Fb[q][1] : Fb[q][2];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      Ub[q][3 + k] += Fb[q][3 + k] * w;
-      wb += Fb[q][3 + k] * (U[q][3 + k] + dq[q][2 + k]);
-      dbq[q][2 + k] = Pb + Fb[q][3 + k] * w;
-    }
-    if (DIR == 0)
-      ub += wb;
-    else
-      vb += wb;
-    dbq[q][0] = ub;
-    dbq[q][1] = vb;
-  }
-  // Roe average pullback
-  const double qv3[5] = {dq[3][0], dq[3][1], e3[0], e3[1], e3[2]};
-  const double qv2[5] = {dq[2][0], dq[2][1], e2[0], e2[1], e2[2]};
-  const double qr[5] = {ru, rv, re[0], re[1], re[2]};
-  double r1b = 0.0, r2b = 0.0;
-  double b3[5], b2[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    b3[k] = roeb[k] * r1 / rs;
-    b2[k] = roeb[k] * r2 / rs;
-    r1b += roeb[k] * (qv3[k] - qr[k]) / rs;
-    r2b += roeb[k] * (qv2[k] - qr[k]) / rs;
-  }
-  dbq[3][0] += b3[0];
-  dbq[3][1] += b3[1];
-  dbq[2][0] += b2[0];
-  dbq[2][1] += b2[1];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    eb3[k] = b3[2 + k];
-    eb2[k] = b2[2 + k];
-  }
-  Ub[3][0] += r1b * 0.5 / r1;
-  Ub[2][0] += r2b * 0.5 / r2;
+// Element at byte offset boff of a plane: a uniform (scalar) plane base plus
+// a 32-bit per-lane offset, so the stencil addresses of an interface cost one
+// VGPR per point instead of a 64-bit address per value.
+__device__ __forceinline__ double plane_at(const double* base, unsigned boff) {
+  return *(const double*)((const char*)base + boff);
 }
 
-// ----------------------------------------------------------- noncon VJP
+// State and physical flux (flux_from_derived, src/state.rs:56-85) of the
+// stencil point at byte offset boff of the padded planes.
+template <int DIR>
+__device__ __forceinline__ void load_point(const double* S, const double* D, long P, unsigned boff, double U[6],
+                                           double F[6]) {
+#pragma unroll
+  for (int f = 0; f < 6; ++f) U[f] = plane_at(S + f * P, boff);
+  const double u = plane_at(D + D_U * P, boff), v = plane_at(D + D_V * P, boff);
+  const double pe = plane_at(D + D_PE * P, boff), pi = plane_at(D + D_PI * P, boff);
+  const double pr = plane_at(D + D_PR * P, boff);
+  const double w = DIR == 0 ? u : v;
+  F[0] = U[0] * w;
+  F[1] = DIR == 0 ? U[0] * u * u + (pe + pi + pr) : U[0] * u * v;
+  F[2] = DIR == 0 ? U[0] * u * v : U[0] * v * v + (pe + pi + pr);
+  F[3] = (U[3] + pe) * w;
+  F[4] = (U[4] + pi) * w;
+  F[5] = (U[5] + pr) * w;
+}
 
-// Pullback of nonconservative_direction_pre's (ee, ei, er) output nb[3]
-// w.r.t. vel[k] (k = 3, 4, 5 only are used) and p_e/p_i/p_r at the 9 points.
-__device__ __forceinline__ void noncon_vjp(const double vel[9], const double pe[9], const double pi[9],
-                                           const double pr[9], double ds, const double nb[3], double velb[9],
-                                           double peb[9], double pib[9], double prb[9]) {
-  const double* P[3] = {pe, pi, pr};
-  double* PB[3] = {peb, pib, prb};
+// Characteristic c of the pullback of one interface flux (interface between
+// stencil points 2 and 3).  U, F: states and physical fluxes of the 6
+// points; ob: flux cotangent.  The cotangents of the split fluxes fp[0..5)
+// and fm[1..6) that enter the two WENO5 reconstructions are written to
+// cb[0..10) (their pullback to the stencil states is linear and is formed
+// by the point phase), and the cotangents of row c of L, column c of R and
+// lambda_c are accumulated into B.
+template <int DIR>
+__device__ __forceinline__ void characteristic_step(int c, const EigConst& K, double eps, const RoeV& R,
+                                                    const double U[6][6], const double F[6][6], const double ob[6],
+                                                    double cb[10], RoeBar& B) {
+  double row[6], col[6], lam;
+  row_col<DIR>(K, R, c, row, col, &lam);
+  const double a = fabs(lam);
+  double cu[6], fp[6], fm[6];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    velb[k] = 0.0;
-    peb[k] = pib[k] = prb[k] = 0.0;
-  }
-  const double v = vel[4];
-  double dp[3];
+  for (int q = 0; q < 6; ++q) {
+    double cf = 0.0, u_ = 0.0;
 #pragma unroll
-  for (int m = 0; m < 3; ++m) dp[m] = sixth(P[m][1], P[m][2], P[m][3], P[m][5], P[m][6], P[m][7], ds);
-  const double der[3] = {2.0 * dp[0] - dp[1] - dp[2], 2.0 * dp[1] - dp[0] - dp[2], 2.0 * dp[2] - dp[0] - dp[1]};
-  double N[3][9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    N[0][k] = 2.0 * pe[k] - pi[k] - pr[k];
-    N[1][k] = 2.0 * pi[k] - pe[k] - pr[k];
-    N[2][k] = 2.0 * pr[k] - pe[k] - pi[k];
-  }
-  const double m1 = fmax(v, vel[3]);
-  const double cl = fmax(m1, 0.0) / 3.0;
-  const double n1 = fmin(v, vel[5]);
-  const double cr = fmin(n1, 0.0) / 3.0;
-  double clb = 0.0, crb = 0.0, vb = 0.0;
-  double derb[3];
-  double Nb[3][9];
-#pragma unroll
-  for (int e = 0; e < 3; ++e) {
-    const double jm = jump(N[e]), jp = jump(N[e] + 1);
-    vb += nb[e] * der[e] / 3.0;
-    derb[e] = nb[e] * v / 3.0;
-    clb += nb[e] * jm / ds;
-    crb += nb[e] * jp / ds;
-    const double jmb = nb[e] * cl / ds, jpb = nb[e] * cr / ds;
-    // jump(a) = q(a7..a1) - q(a0..a6)
-    const double qc[7] = {-0.00714285714285723, 0.0595238095238118, -0.240476190476197, 0.759523809523819,
-                          0.509523809523803,    -0.0904761904761891, 0.00952380952380977};
-#pragma unroll
-    for (int k = 0; k < 9; ++k) Nb[e][k] = 0.0;
-#pragma unroll
-    for (int t = 0; t < 7; ++t) {
-      Nb[e][7 - t] += jmb * qc[t];
-      Nb[e][t] -= jmb * qc[t];
-      Nb[e][8 - t] += jpb * qc[t];
-      Nb[e][1 + t] -= jpb * qc[t];
+    for (int f = 0; f < 6; ++f) {
+      cf += row[f] * F[q][f];
+      u_ += row[f] * U[q][f];
     }
+    cu[q] = u_;
+    fp[q] = 0.5 * (cf + a * u_);
+    fm[q] = 0.5 * (cf - a * u_);
   }
-  const double dpb[3] = {2.0 * derb[0] - derb[1] - derb[2], -derb[0] + 2.0 * derb[1] - derb[2],
-                         -derb[0] - derb[1] + 2.0 * derb[2]};
-  const double inv = 1.0 / (60.0 * ds);
+  double fcb = 0.0;
 #pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    PB[m][1] += -dpb[m] * inv;
-    PB[m][2] += 9.0 * dpb[m] * inv;
-    PB[m][3] += -45.0 * dpb[m] * inv;
-    PB[m][5] += 45.0 * dpb[m] * inv;
-    PB[m][6] += -9.0 * dpb[m] * inv;
-    PB[m][7] += dpb[m] * inv;
-  }
+  for (int r = 0; r < 6; ++r) fcb += col[r] * ob[r];
+  double gp[5], gm[5];
+  const double sp[5] = {fp[0], fp[1], fp[2], fp[3], fp[4]};
+  const double sm[5] = {fm[5], fm[4], fm[3], fm[2], fm[1]};
+  const double fc = weno5_vjp(sp, eps, fcb, gp) + weno5_vjp(sm, eps, fcb, gm);
+  // fpb[q] = gp[q] (q < 5), fmb[q] = gm[5 - q] (q > 0)
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    peb[k] += 2.0 * Nb[0][k] - Nb[1][k] - Nb[2][k];
-    pib[k] += -Nb[0][k] + 2.0 * Nb[1][k] - Nb[2][k];
-    prb[k] += -Nb[0][k] - Nb[1][k] + 2.0 * Nb[2][k];
+  for (int k = 0; k < 5; ++k) {
+    cb[k] = gp[k];
+    cb[5 + k] = gm[4 - k];
   }
-  // cl = max(max(v, v3), 0)/3, cr = min(min(v, v5), 0)/3
-  if (m1 >= 0.0) {
-    if (v >= vel[3])
-      vb += clb / 3.0;
-    else
-      velb[3] += clb / 3.0;
+  double ab = 0.0;
+  double Lb[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double fpb = q < 5 ? gp[q] : 0.0, fmb = q > 0 ? gm[5 - q] : 0.0;
+    const double cfb = 0.5 * (fpb + fmb);
+    const double d = 0.5 * (fpb - fmb);
+    const double cub = a * d;
+    ab += cu[q] * d;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) Lb[f] += cfb * F[q][f] + cub * U[q][f];
   }
-  if (n1 <= 0.0) {
-    if (v <= vel[5])
-      vb += crb / 3.0;
-    else
-      velb[5] += crb / 3.0;
-  }
-  velb[4] += vb;
+  double Rb[6];
+#pragma unroll
+  for (int r = 0; r < 6; ++r) Rb[r] = fc * ob[r];
+  const double lamb = lam >= 0.0 ? ab : -ab;
+  row_col_bar<DIR>(K, R, c, Lb, Rb, lamb, B);
+}
+
+__device__ __forceinline__ void roe_bar_zero(RoeBar& B) {
+  B.u = B.v = B.w2 = B.cs = B.cs2 = B.b = B.ics2 = B.ib = B.igtcs2 = B.pcs = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) B.h[k] = B.gk[k] = 0.0;
 }
 
 }  // namespace rh3t_dev
+
+#pragma clang fp contract(off)

@@ -35,29 +35,17 @@
 #include <string>
 #include <vector>
 
-#include "flux.hpp"
-#include "kernels.hpp"
+#include "engine.hpp"
 #include "problem.hpp"
 
 using namespace rh3t_dev;
+using namespace rh3t_eng;
 
 namespace {
 
 constexpr int kStOff[6] = {0, 1, 5, 14, 30, 55};
 constexpr int kSOffD[6] = {0, 1, 17, 98, 354, 979};
 constexpr int kWOffD[6] = {0, 1, 9, 36, 100, 225};
-
-struct SolveDesc {
-  Geo g;
-  double* U[3];
-  double* D;
-  double* FX;
-  double* FY;
-  double* RHS;          // compact SoA nx*ny (l() output)
-  const uint8_t* pt;    // 1 fluid, 2 ghost
-  Scalars* sc;
-  const long* gpad;     // ghost slots by sorted ghost id
-};
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; give
 // each XCD a contiguous run of tiles so that halo rows shared by
@@ -86,17 +74,6 @@ __device__ __forceinline__ void raise_error(Scalars* sc, unsigned phase, unsigne
 }
 
 // ------------------------------------------------------------ ghosts
-
-struct CheapGhosts {
-  int n;
-  const int* sid;       // solve
-  const long* pad;      // destination slot
-  const long* src;      // source slot (reflective / zeroth / periodic)
-  const int* kind;      // rh3t_bc_kind
-  const double* normal; // 2 per ghost
-  const double* cstate; // 6 per ghost (Constant)
-  const int* gid;       // sorted ghost id within its solve
-};
 
 __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict__ descs, CheapGhosts cg, int buf,
                                                      unsigned phase) {
@@ -130,24 +107,6 @@ __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict
   }
   if (!ok) raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)cg.gid[k]);
 }
-
-struct HeavyGhosts {
-  int n;
-  const int* sid;        // solve
-  const long* pad;       // destination slot
-  const int* kind;       // Wall / Outflow / FarField
-  const int* gid;        // sorted ghost id within its solve
-  const long* nearest;   // slot of GhostBC::nearest
-  const double* normal;  // 2 per ghost
-  const double* dist;    // signed distance D
-  const double* bcstate; // 6 per ghost (FarField u_inf)
-  const double* outflow; // 3 per ghost (p_inf, sigma, l_domain)
-  const int* tab;        // GhostBC table row of the ghost (shared by clones)
-  const double* d5;      // 5 per table row
-  const long* stencil;   // 55 slots per table row
-  const double* S;       // 979 per table row
-  const double* W;       // 225 per table row
-};
 
 __device__ __forceinline__ void matvec6(const double M[6][6], const double x[6], double y[6]) {
 #pragma unroll
@@ -468,12 +427,6 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
 
 // --------------------------------------------------------- assemble
 
-struct StageSpec {
-  int mode;        // 0: write rhs, 1..3: SSP-RK3 stage
-  int base, add, dst;  // buffer indices into SolveDesc::U
-  unsigned phase;
-};
-
 __global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ descs, const int* __restrict__ tiles,
                                                   int nsolve, Phys ph, StageSpec sa) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -706,120 +659,13 @@ __global__ __launch_bounds__(256) void k_gather_ghosts(SolveDesc sd, int n, doub
 // Engine: device state of a batch of solves
 // ======================================================================
 
-enum KernelId { K_GHOST_CHEAP, K_GHOST_HEAVY, K_DERIVED, K_FLUX_X, K_FLUX_Y, K_ASSEMBLE, K_DT_PARTIAL,
-                K_DT_FINAL, K_CAPTURE, K_COUNT };
-static const char* kKernelNames[K_COUNT] = {"ghost_cheap", "ghost_heavy", "derived", "flux_x", "flux_y",
-                                            "assemble", "dt_partial", "dt_final", "capture"};
-
-struct ProfRec {
-  int kernel;
-  hipEvent_t a, b;
-};
-
-struct SolveHost {
-  Geo g;
-  long n_ghost = 0;
-  std::vector<uint8_t> fluid;
-  std::vector<double> nonfluid;  // AoS values of non-fluid cells (kept as given)
-};
-
-struct Engine {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  Phys phys{};
-  EigConst eig{};
-  bool fast = true;
-  int nplanes = 8;
-  int nsolve = 0;
-  std::vector<SolveHost> solves;
-  std::vector<SolveDesc> h_desc;
-  SolveDesc* d_desc = nullptr;
-  Scalars* d_sc = nullptr;
-  CheapGhosts cheap{};
-  HeavyGhosts heavy{};
-
-  // launch shapes (autotuned): flux / assemble workgroup (bx, by), heavy
-  // ghosts per workgroup
-  int bx = 64, by = 4, ghosts_per_block = 1;
-  std::vector<int> t_fx, t_fy, t_as, t_der, t_dt;  // host prefix tables
-  int* d_tabs = nullptr;  // device copy: fx | fy | as | der | dt, each nsolve+1
-  double* d_partial = nullptr;
-  double* d_aos = nullptr;   // staging (largest solve)
-  double* d_gvals = nullptr;
-  std::vector<void*> allocs;
-
-  hipGraphExec_t graph = nullptr;
-  bool profiling = false;
-  std::vector<ProfRec> prof;
-  double prof_ms[K_COUNT] = {0};
-  long long prof_n[K_COUNT] = {0};
-
-  std::vector<rh3t_error> last;  // per solve
-  char message[256] = {0};
-
-  const int* tab(int k) const { return d_tabs + k * (nsolve + 1); }
-};
-
-struct rh3t_ctx {
-  Engine e;
-};
-
-struct rh3t_batch {
-  Engine e;
-};
+namespace rh3t_eng {
+const char* const kKernelNames[K_COUNT] = {"ghost_cheap", "ghost_heavy", "derived", "flux_x", "flux_y",
+                                           "assemble", "dt_partial", "dt_final", "capture", "adj_iface_x",
+                                           "adj_iface_y", "adj_gather", "adj_ghost", "adj_param"};
+}  // namespace rh3t_eng
 
 namespace {
-
-#define HIP_TRY(expr)                                                                     \
-  do {                                                                                    \
-    hipError_t e_ = (expr);                                                               \
-    if (e_ != hipSuccess) {                                                               \
-      std::snprintf(E.message, sizeof(E.message), "%s: %s", #expr, hipGetErrorString(e_)); \
-      return RH3T_ERR_HIP;                                                                \
-    }                                                                                     \
-  } while (0)
-
-template <class T>
-int dev_alloc(Engine& E, T** p, size_t count) {
-  void* q = nullptr;
-  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
-  if (e != hipSuccess) {
-    std::snprintf(E.message, sizeof(E.message), "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
-    return RH3T_ERR_HIP;
-  }
-  E.allocs.push_back(q);
-  *p = (T*)q;
-  return RH3T_OK;
-}
-
-template <class T>
-int upload(Engine& E, T** p, const std::vector<T>& v) {
-  int st = dev_alloc(E, p, v.size());
-  if (st) return st;
-  if (!v.empty()) HIP_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-  return RH3T_OK;
-}
-
-struct Launcher {
-  Engine& E;
-  int kid;
-  hipEvent_t a = nullptr, b = nullptr;
-  Launcher(Engine& e, int k) : E(e), kid(k) {
-    if (E.profiling) {
-      (void)hipEventCreate(&a);
-      (void)hipEventCreate(&b);
-      (void)hipEventRecord(a, E.stream);
-    }
-  }
-  ~Launcher() {
-    if (E.profiling) {
-      (void)hipEventRecord(b, E.stream);
-      E.prof.push_back(ProfRec{kid, a, b});
-    }
-  }
-};
-
-int total(const std::vector<int>& t) { return t.back(); }
 
 // prefix tables for the current launch shape
 int build_tables(Engine& E) {
@@ -852,7 +698,11 @@ int build_tables(Engine& E) {
   return RH3T_OK;
 }
 
-void launch_ghosts(Engine& E, int buf, unsigned phase) {
+}  // namespace
+
+namespace rh3t_eng {
+
+void launch_ghosts(Engine& E, int buf, unsigned phase, bool capture) {
   if (E.cheap.n > 0) {
     Launcher L(E, K_GHOST_CHEAP);
     k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.cheap, buf, phase);
@@ -862,17 +712,19 @@ void launch_ghosts(Engine& E, int buf, unsigned phase) {
     int g = E.ghosts_per_block;
     k_ghost_heavy<<<(E.heavy.n + g - 1) / g, dim3(64, g), 0, E.stream>>>(E.d_desc, E.heavy, E.phys, buf, phase);
   }
+  if (!capture) return;
   Launcher L(E, K_CAPTURE);
   k_capture<<<(E.nsolve + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.nsolve, phase, buf);
 }
 
-// ghosts -> derived -> x/y fluxes -> assemble (one l() + stage) on buffer src
+void launch_derived(Engine& E, int buf) {
+  Launcher L(E, K_DERIVED);
+  k_derived<<<total(E.t_der), 256, 0, E.stream>>>(E.d_desc, E.tab(3), E.nsolve, E.phys, buf, E.nplanes);
+}
+
 void launch_stage(Engine& E, int src, const StageSpec& sa) {
   launch_ghosts(E, src, sa.mode == 0 ? 2u : (unsigned)(2 * sa.mode));
-  {
-    Launcher L(E, K_DERIVED);
-    k_derived<<<total(E.t_der), 256, 0, E.stream>>>(E.d_desc, E.tab(3), E.nsolve, E.phys, src, E.nplanes);
-  }
+  launch_derived(E, src);
   dim3 blk(E.bx, E.by);
   {
     Launcher L(E, K_FLUX_X);
@@ -958,6 +810,10 @@ int reset_errors(Engine& E) {
   return RH3T_OK;
 }
 
+}  // namespace rh3t_eng
+
+namespace {
+
 Phys make_phys(const rh3t_physics& p) {
   Phys ph;
   ph.gamma_e = p.gamma_e;
@@ -1023,6 +879,7 @@ void free_engine(Engine& E) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
+  free_adjoint(E);
   for (void* p : E.allocs) (void)hipFree(p);
   if (E.stream) (void)hipStreamDestroy(E.stream);
   E.allocs.clear();
@@ -1146,8 +1003,8 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
       if ((st = dev_alloc(E, &d.U[b], 6 * (size_t)g.plane))) return st;
       HIP_TRY(hipMemset(d.U[b], 0, 6 * (size_t)g.plane * sizeof(double)));
     }
-    if ((st = dev_alloc(E, &d.D, (size_t)E.nplanes * g.plane))) return st;
-    HIP_TRY(hipMemset(d.D, 0, (size_t)E.nplanes * g.plane * sizeof(double)));
+    if ((st = dev_alloc(E, &d.D, (size_t)D_N * g.plane))) return st;
+    HIP_TRY(hipMemset(d.D, 0, (size_t)D_N * g.plane * sizeof(double)));
     if ((st = dev_alloc(E, &d.FX, 6 * (size_t)(g.nx + 1) * g.ny))) return st;
     if ((st = dev_alloc(E, &d.FY, 6 * (size_t)g.nx * (g.ny + 1)))) return st;
     if ((st = dev_alloc(E, &d.RHS, 6 * (size_t)ncell))) return st;
@@ -1157,6 +1014,11 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   if ((st = dev_alloc(E, &E.d_aos, 6 * (size_t)max_cells))) return st;
   if ((st = dev_alloc(E, &E.d_gvals, 6 * (size_t)max_ghost))) return st;
 
+  E.h_cheap_sid = c_sid;
+  E.h_cheap_kind = c_kind;
+  E.h_cheap_pad = c_pad;
+  E.h_cheap_src = c_src;
+  E.h_cheap_normal = c_n;
   E.cheap.n = (int)c_pad.size();
   E.heavy.n = (int)h_pad.size();
   long* lp;
@@ -1457,6 +1319,19 @@ int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode) {
     ctx->e.graph = nullptr;
   }
   ctx->e.fast = f;
+  return RH3T_OK;
+}
+
+int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys) {
+  if (!ctx || !phys) return RH3T_ERR_ARG;
+  Engine& E = ctx->e;
+  E.phys = make_phys(*phys);
+  E.eig = make_eig(E.phys);
+  E.nplanes = (E.adj || E.phys.diffusion || E.phys.source) ? D_N : 8;
+  if (E.graph) {
+    (void)hipGraphExecDestroy(E.graph);
+    E.graph = nullptr;
+  }
   return RH3T_OK;
 }
 

@@ -1,0 +1,205 @@
+// Device context shared by the forward engine (engine.hip) and the adjoint
+// (adjoint.hip): per-solve descriptors, the ghost work lists, the Engine
+// that owns a batch's device memory and HIP stream, and the launch helpers
+// of one l() evaluation / RK3 stage.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "flux.hpp"
+#include "kernels.hpp"
+#include "rh3t.h"
+
+namespace rh3t_eng {
+
+using namespace rh3t_dev;
+
+struct SolveDesc {
+  Geo g;
+  double* U[3];
+  double* D;
+  double* FX;
+  double* FY;
+  double* RHS;          // compact SoA nx*ny (l() output)
+  const uint8_t* pt;    // 1 fluid, 2 ghost
+  Scalars* sc;
+  const long* gpad;     // ghost slots by sorted ghost id
+};
+
+struct CheapGhosts {
+  int n;
+  const int* sid;       // solve
+  const long* pad;      // destination slot
+  const long* src;      // source slot (reflective / zeroth / periodic)
+  const int* kind;      // rh3t_bc_kind
+  const double* normal; // 2 per ghost
+  const double* cstate; // 6 per ghost (Constant)
+  const int* gid;       // sorted ghost id within its solve
+};
+
+struct HeavyGhosts {
+  int n;
+  const int* sid;        // solve
+  const long* pad;       // destination slot
+  const int* kind;       // Wall / Outflow / FarField
+  const int* gid;        // sorted ghost id within its solve
+  const long* nearest;   // slot of GhostBC::nearest
+  const double* normal;  // 2 per ghost
+  const double* dist;    // signed distance D
+  const double* bcstate; // 6 per ghost (FarField u_inf)
+  const double* outflow; // 3 per ghost (p_inf, sigma, l_domain)
+  const int* tab;        // GhostBC table row of the ghost (shared by clones)
+  const double* d5;      // 5 per table row
+  const long* stencil;   // 55 slots per table row
+  const double* S;       // 979 per table row
+  const double* W;       // 225 per table row
+};
+
+struct StageSpec {
+  int mode;        // 0: write rhs, 1..3: SSP-RK3 stage
+  int base, add, dst;  // buffer indices into SolveDesc::U
+  unsigned phase;
+};
+
+enum KernelId {
+  K_GHOST_CHEAP, K_GHOST_HEAVY, K_DERIVED, K_FLUX_X, K_FLUX_Y, K_ASSEMBLE, K_DT_PARTIAL, K_DT_FINAL, K_CAPTURE,
+  K_ADJ_IFACE_X, K_ADJ_IFACE_Y, K_ADJ_GATHER, K_ADJ_GHOST, K_ADJ_PARAM, K_COUNT
+};
+extern const char* const kKernelNames[K_COUNT];
+
+struct ProfRec {
+  int kernel;
+  hipEvent_t a, b;
+};
+
+struct SolveHost {
+  Geo g;
+  long n_ghost = 0;
+  std::vector<uint8_t> fluid;
+  std::vector<double> nonfluid;  // AoS values of non-fluid cells (kept as given)
+};
+
+struct AdjWork;  // adjoint workspace (adjoint.hip)
+
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Phys phys{};
+  EigConst eig{};
+  bool fast = true;
+  int nplanes = 8;
+  int nsolve = 0;
+  std::vector<SolveHost> solves;
+  std::vector<SolveDesc> h_desc;
+  SolveDesc* d_desc = nullptr;
+  Scalars* d_sc = nullptr;
+  CheapGhosts cheap{};
+  HeavyGhosts heavy{};
+  // host copies of the cheap-ghost lists (the adjoint transposes them)
+  std::vector<int> h_cheap_sid, h_cheap_kind;
+  std::vector<long> h_cheap_pad, h_cheap_src;
+  std::vector<double> h_cheap_normal;
+
+  // launch shapes (autotuned): flux / assemble workgroup (bx, by), heavy
+  // ghosts per workgroup
+  int bx = 64, by = 4, ghosts_per_block = 1;
+  std::vector<int> t_fx, t_fy, t_as, t_der, t_dt;  // host prefix tables
+  int* d_tabs = nullptr;  // device copy: fx | fy | as | der | dt, each nsolve+1
+  double* d_partial = nullptr;
+  double* d_aos = nullptr;   // staging (largest solve)
+  double* d_gvals = nullptr;
+  std::vector<void*> allocs;
+
+  hipGraphExec_t graph = nullptr;
+  bool profiling = false;
+  std::vector<ProfRec> prof;
+  double prof_ms[K_COUNT] = {0};
+  long long prof_n[K_COUNT] = {0};
+
+  std::vector<rh3t_error> last;  // per solve
+  char message[256] = {0};
+
+  AdjWork* adj = nullptr;
+
+  const int* tab(int k) const { return d_tabs + k * (nsolve + 1); }
+};
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      std::snprintf(E.message, sizeof(E.message), "%s: %s", #expr, hipGetErrorString(e_)); \
+      return RH3T_ERR_HIP;                                                                \
+    }                                                                                     \
+  } while (0)
+
+template <class T>
+int dev_alloc(Engine& E, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    std::snprintf(E.message, sizeof(E.message), "hipMalloc(%zu): %s", count * sizeof(T), hipGetErrorString(e));
+    return RH3T_ERR_HIP;
+  }
+  E.allocs.push_back(q);
+  *p = (T*)q;
+  return RH3T_OK;
+}
+
+template <class T>
+int upload(Engine& E, T** p, const std::vector<T>& v) {
+  int st = dev_alloc(E, p, v.size());
+  if (st) return st;
+  if (!v.empty()) HIP_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return RH3T_OK;
+}
+
+// Brackets one launch with events on the engine's stream when profiling.
+struct Launcher {
+  Engine& E;
+  int kid;
+  hipEvent_t a = nullptr, b = nullptr;
+  Launcher(Engine& e, int k) : E(e), kid(k) {
+    if (E.profiling) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, E.stream);
+    }
+  }
+  ~Launcher() {
+    if (E.profiling) {
+      (void)hipEventRecord(b, E.stream);
+      E.prof.push_back(ProfRec{kid, a, b});
+    }
+  }
+};
+
+inline int total(const std::vector<int>& t) { return t.back(); }
+
+// ghosts of every solve on buffer buf (+ the first-failure state capture
+// unless capture is false; the failure key is raised either way)
+void launch_ghosts(Engine& E, int buf, unsigned phase, bool capture = true);
+// Derived::from_state on every fluid cell and ghost of buffer buf
+void launch_derived(Engine& E, int buf);
+// ghosts -> derived -> x/y fluxes -> assemble (one l() + stage) on buffer src
+void launch_stage(Engine& E, int src, const StageSpec& sa);
+// one SSP-RK3 step with the dt held in the solves' scalars
+void launch_rk3(Engine& E);
+int check_errors(Engine& E);
+int reset_errors(Engine& E);
+// releases the adjoint workspace (host side; device memory is in allocs)
+void free_adjoint(Engine& E);
+
+}  // namespace rh3t_eng
+
+struct rh3t_ctx {
+  rh3t_eng::Engine e;
+};
+
+struct rh3t_batch {
+  rh3t_eng::Engine e;
+};

@@ -137,3 +137,66 @@ def translating_shock_state(field, seed=2):
 
 def cylinder(nx=121, mach=3.0):
     return Field.init_cylinder(nx, mach)
+
+
+# ----------------------------------------------- differentiation problem
+
+# Exchange coefficients of the differentiated workload: switched on (the
+# reference defaults are 0, src/constant.rs:17-20) so that the source term
+# and its parameters enter the gradient.
+ADJOINT_PHYSICS = dict(omega_ei=1.0, omega_er=0.1)
+
+
+def gaussian_window(field, seed=5):
+    """w = exp(-|x - x_c|^2 / (2 sigma^2)) on the cell nodes, with
+    x_c ~ U(0.25, 0.75)^2 and sigma ~ U(0.05, 0.15) drawn from the LCG."""
+    g = field.grid
+    rng = LCG(seed)
+    xc, yc = 0.25 + 0.5 * rng(), 0.25 + 0.5 * rng()
+    sig = 0.05 + 0.1 * rng()
+    x = g.x0 + np.arange(g.nx)[:, None] * g.dx + 0.0 * np.arange(g.ny)[None, :]
+    y = g.y0 + np.arange(g.ny)[None, :] * g.dy + 0.0 * np.arange(g.nx)[:, None]
+    return np.exp(-((x - xc) ** 2 + (y - yc) ** 2) / (2.0 * sig * sig))
+
+
+def window_loss_cotangent(field, seed=5):
+    """dJ/du^N of J = sum(w * E_r) (the sixth state field)."""
+    lam = np.zeros((field.grid.nx, field.grid.ny, 6))
+    lam[..., 5] = gaussian_window(field, seed)
+    lam[~field.fluid] = 0.0
+    return lam
+
+
+def state_direction(u, fluid, seed):
+    """Seeded direction in state space, scaled per field by the mean
+    magnitude of that field, zero on non-fluid cells."""
+    rng = np.random.default_rng(seed)
+    du = rng.standard_normal(u.shape) * np.abs(u[fluid]).mean(axis=0)
+    du[~fluid] = 0.0
+    return du
+
+
+def smooth_direction(field, u, seed):
+    """Seeded smooth direction in state space: per field a sum of 8 random
+    Fourier modes (as smooth_modes_state) scaled by the mean magnitude of
+    that field, zero on non-fluid cells."""
+    g = field.grid
+    x = g.x0 + np.arange(g.nx, dtype=np.float64)[:, None] * g.dx + 0.0 * np.arange(g.ny)[None, :]
+    y = g.y0 + np.arange(g.ny, dtype=np.float64)[None, :] * g.dy + 0.0 * np.arange(g.nx)[:, None]
+    rng = LCG(seed)
+    scale = np.abs(u[field.fluid]).mean(axis=0)
+    du = np.stack([scale[k] * fourier_field(x, y, rng) for k in range(6)], axis=-1)
+    du[~field.fluid] = 0.0
+    return du
+
+
+def walled_box(n):
+    """Unit square with ReflectiveWall bottom and top, Constant inflow on
+    the left and ZerothOrder on the right (the cheap ghost kinds)."""
+    grid = GridInfo(n, n, 1.0 / n, 1.0 / n, 0.0, 0.0)
+    inflow = primitive_to_state(np.array(1.0), np.array(0.3), np.array(0.0), np.array(1.0), np.array(1.0),
+                                np.array(1.0))
+    bcs = [BCType.reflective_wall(), BCType.zeroth_order(), BCType.reflective_wall(),
+           BCType.constant(inflow.tolist())]
+    f = Field(grid, _square(), bcs)
+    return f.build_ghosts()

@@ -11,7 +11,7 @@ hipEvents recorded around every kernel on the solver's own stream; the
 per-kernel averages give the roofline entry for the dominant kernel (the
 x-direction WENO flux).  The CPU baseline is the oracle (C restatement of
 the reference algorithm) timed on a bounded sample of the same workload:
-one RK3 step of the same 2048^2 box on the host's cores.
+whole RK3 steps of the same 2048^2 box on the host's cores, about 10 s.
 """
 import argparse
 import importlib
@@ -40,7 +40,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(field, u, workloads, max_threads=16):
+def cpu_baseline(field, u, workloads, max_threads=16, min_seconds=10.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bridge as ob
     try:
@@ -50,13 +50,17 @@ def cpu_baseline(field, u, workloads, max_threads=16):
     cores = max(1, min(cores, max_threads))
     prob = ob.OracleProblem(field)
     n_fluid = int(field.fluid.sum())
-    dt = 0.05 * ob.global_dt(prob, u)
-    t0 = time.perf_counter()
-    ob.rk3_step(prob, u, dt, nthreads=cores)
-    wall = time.perf_counter() - t0
-    return {"value": n_fluid * 3 / wall, "unit": "cell-stage/s", "cores": cores, "kind": "port",
-            "sample": f"1 SSP-RK3 step (3 l() + stages) of the same {field.grid.nx}x{field.grid.ny} box, "
-                      f"C oracle with OpenMP, {wall:.2f} s"}
+    # whole RK3 steps of the same box until about min_seconds of CPU work
+    steps, wall = 0, 0.0
+    while wall < min_seconds and steps < 64:
+        dt = 0.05 * ob.global_dt(prob, u)
+        t0 = time.perf_counter()
+        u, _ = ob.rk3_step(prob, u, dt, nthreads=cores)
+        wall += time.perf_counter() - t0
+        steps += 1
+    return {"value": n_fluid * 3 * steps / wall, "unit": "cell-stage/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} SSP-RK3 steps (3 l() + stages each) of the same {field.grid.nx}x{field.grid.ny} "
+                      f"box, C oracle with OpenMP on {cores} threads, {wall:.2f} s"}
 
 
 def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
@@ -124,6 +128,86 @@ def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
                                                                           "ms_per_step")}}
 
 
+def _gpu_ms(profile):
+    """Summed kernel time (ms) of a solver's per-kernel profile."""
+    return sum(avg * cnt for avg, cnt in profile.values())
+
+
+def adjoint_leg(rh3t, n=512, nsteps=20, every=5, fd_dirs=32):
+    """The differentiation workload: the smooth-mode periodic box at n^2
+    with the exchange terms on, nsteps RK3 steps with the dt schedule of a
+    base run frozen, J = sum(w * E_r(u^N)) with a seeded Gaussian window.
+    Times the checkpointing forward (state kept every `every` steps) and the
+    backward (segment recompute + reverse sweep), by wall clock and by summed
+    per-kernel GPU time, and checks the gradient against central
+    differences (h = 1e-6) of the forward on fd_dirs seeded smooth
+    directions over (u0, omega_ei, omega_er)."""
+    import numpy as np
+
+    workloads = importlib.import_module("3t_rh_amd.workloads")
+    f = workloads.periodic_box(n)
+    u = workloads.smooth_modes_state(f, seed=3)
+    phys = rh3t.Physics(**workloads.ADJOINT_PHYSICS)
+    s = f.solver(physics=phys)
+    s.set_state(u)
+    dts = []
+    for _ in range(nsteps):
+        dts.append(0.05 * s.calc_global_dt())
+        s.rk3_ssp(dts[-1])
+    lam = workloads.window_loss_cotangent(f, seed=5)
+    adj = s.adjoint(every)
+    s.set_state(u)
+    adj.forward(dts)
+    adj.backward(lam)  # warm-up (allocations, code objects)
+    s.set_state(u)
+    t0 = time.perf_counter()
+    adj.forward(dts)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gu, gp = adj.backward(lam)
+    t_bwd = time.perf_counter() - t0
+    # GPU-side time of the same two calls
+    s.set_state(u)
+    s.profile(True)
+    adj.forward(dts)
+    prof_f = s.profile_read()
+    s.profile(True)
+    adj.backward(lam)
+    prof_b = s.profile_read()
+    s.profile(False)
+    g_fwd, g_bwd = _gpu_ms(prof_f), _gpu_ms(prof_b)
+    w = lam[..., 5]
+
+    def J(x, pv):
+        s.set_physics(phys.with_params(pv))
+        s.set_state(x)
+        for d in dts:
+            s.rk3_ssp(d)
+        y, _ = s.get_state()
+        return float(np.sum(w * y[..., 5]))
+
+    h, worst = 1e-6, 0.0
+    p0 = phys.params()
+    for seed in range(5001, 5001 + fd_dirs):
+        rng = np.random.default_rng(seed)
+        du = workloads.smooth_direction(f, u, seed)
+        dp = np.zeros(5)
+        dp[:2] = rng.standard_normal(2) * np.array([1.0, 0.1])
+        fd = (J(u + h * du, p0 + h * dp) - J(u - h * du, p0 - h * dp)) / (2 * h)
+        ad = float(np.sum(gu * du) + np.sum(gp * dp))
+        worst = max(worst, abs(fd - ad) / abs(ad))
+    adj.close()
+    s.close()
+    kern = {k: round(v[0] * v[1], 4) for k, v in prof_b.items() if v[1]}
+    return {"workload": f"smooth-mode periodic box {n}x{n}, omega_ei=1, omega_er=0.1, {nsteps} SSP-RK3 steps, "
+                        f"J = sum(gaussian window * E_r)",
+            "checkpoint_every": every, "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+            "adjoint_over_forward": t_bwd / t_fwd, "forward_gpu_ms": g_fwd, "backward_gpu_ms": g_bwd,
+            "adjoint_over_forward_gpu": g_bwd / g_fwd, "backward_kernels_ms": kern,
+            "grad_params": [float(gp[0]), float(gp[1])], "fd_directions": fd_dirs, "fd_h": h,
+            "fd_max_rel_err": worst}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +219,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--batched-solves", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the differentiation leg (for profiling it); prints its JSON object")
+    ap.add_argument("--fd-dirs", type=int, default=32, help="central-difference directions of the adjoint leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
@@ -142,6 +230,9 @@ def main():
 
     rh3t = importlib.import_module("3t_rh_amd")
     workloads = importlib.import_module("3t_rh_amd.workloads")
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(rh3t, fd_dirs=args.fd_dirs)}), flush=True)
+        return
     n = args.size
     log(f"building {n}x{n} periodic box")
     field = workloads.periodic_box(n)
@@ -202,8 +293,11 @@ def main():
     if not args.no_batched:
         log("batched leg")
         line["batched"] = batched_leg(rh3t, args.batched_solves, args.batched_steps)
+    if not args.no_adjoint:
+        log("adjoint leg")
+        line["adjoint"] = adjoint_leg(rh3t, fd_dirs=args.fd_dirs)
     if not args.no_cpu:
-        log("cpu baseline (oracle, 1 step)")
+        log("cpu baseline (oracle, ~10 s of RK3 steps)")
         line["cpu_baseline"] = cpu_baseline(field, u0, workloads)
     print(json.dumps(line), flush=True)
 

@@ -13,8 +13,8 @@
  * 6 doubles (rho, mom_x, mom_y, ee, ei, er) per grid point at l = i*ny + j
  * (src/field1.rs:83-85, src/field1.rs:204-215).
  *
- * A Rust crate binds this header 1:1 (see INTEGRATION.md for the extern
- * block); the Python mirror lives in the 3t_rh_amd package.
+ * A Rust crate binds this header 1:1 through an extern "C" block; the
+ * Python mirror lives in the 3t_rh_amd package.
  */
 #ifndef RH3T_H
 #define RH3T_H
@@ -236,6 +236,11 @@ int rh3t_advance(rh3t_ctx* ctx, int32_t nsteps, double cfl_factor,
 #define RH3T_ARITH_FAST 1
 int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode);
 
+/* Replaces the physics constants (src/constant.rs:1-25 as a runtime
+ * struct) of an existing context, e.g. to evaluate the operator at other
+ * exchange or diffusion coefficients.  Problem tables are kept. */
+int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys);
+
 /* Workgroup shape of the WENO flux / assembly kernels (bx lanes along y,
  * by rows along x; bx a multiple of 16, bx*by <= 256) and heavy ghosts per
  * workgroup (1..4, one wavefront each).  Chosen by the autotuner in
@@ -285,6 +290,46 @@ int rh3t_batch_profile(rh3t_batch* b, int32_t enable);
 int rh3t_batch_profile_read(rh3t_batch* b, int32_t max_kernels, char* names,
                             int32_t names_cap, double* avg_ms,
                             int64_t* launches, int32_t* n_kernels);
+
+/* ------------------------------------------------------------------------ */
+/* Reverse mode.  No counterpart in the reference (which has no            */
+/* derivatives): vector-Jacobian products of l() and of rk3_ssp() with     */
+/* respect to the state and to the parameters                               */
+/*   pgrad[5] = (omega_ei, omega_er, kappa_e, kappa_i, kappa_r)             */
+/* (src/constant.rs:17-25), and a checkpoint-and-recompute adjoint of a     */
+/* sequence of RK3 steps with a frozen dt schedule.  Covered ghost kinds:  */
+/* Periodic, ReflectiveWall, ZerothOrder, Constant (others return          */
+/* RH3T_ERR_UNSUPPORTED); single-solve contexts only.  Cotangent arrays    */
+/* use the state's AoS layout; entries of non-fluid cells are ignored on   */
+/* input and zero on output.  pgrad may be NULL.                            */
+/* ------------------------------------------------------------------------ */
+
+/* ubar = (d l / d u)^T rbar at the context's state (Scratch::rhs cotangent
+ * rbar, src/main.rs:135-305). */
+int rh3t_l_vjp(rh3t_ctx* ctx, const double* rbar, double* ubar,
+               double* pgrad);
+
+/* ubar = (d u^{n+1} / d u^n)^T lam for one rk3_ssp step of size dt from the
+ * context's state (src/main.rs:351-373); the state is not advanced. */
+int rh3t_rk3_vjp(rh3t_ctx* ctx, double dt, const double* lam, double* ubar,
+                 double* pgrad);
+
+typedef struct rh3t_adjoint rh3t_adjoint;
+
+/* Adjoint of nsteps RK3 steps of ctx (which must outlive it): the forward
+ * keeps the state every checkpoint_every steps; the backward recomputes
+ * each segment from its checkpoint (taping the stage states) and sweeps it
+ * in reverse. */
+int rh3t_adjoint_create(rh3t_ctx* ctx, int32_t checkpoint_every,
+                        rh3t_adjoint** out);
+void rh3t_adjoint_destroy(rh3t_adjoint* adj);
+/* nsteps rk3_ssp steps from the context's state with dt[n]; Field.time
+ * advances by each dt. */
+int rh3t_adjoint_forward(rh3t_adjoint* adj, int32_t nsteps, const double* dt);
+/* lam_initial = (d u^N / d u^0)^T lam_final and the parameter cotangent,
+ * for the last forward; the context is left at u^N. */
+int rh3t_adjoint_backward(rh3t_adjoint* adj, const double* lam_final,
+                          double* lam_initial, double* pgrad);
 
 #ifdef __cplusplus
 }

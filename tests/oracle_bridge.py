@@ -251,3 +251,64 @@ def relerr(a, b, fluid=None):
         d = np.max(np.abs(a[:, f] - b[:, f]))
         out.append(d / scale if scale > 0 else d)
     return np.array(out)
+
+
+# ------------------------------------------------- forward-mode oracle
+
+DUAL_LIB = os.path.join(ORACLE_DIR, "_build", "librh3t_dual.so")
+_dual = None
+
+
+def dual_lib():
+    """oracle/dual.cpp: rh3t_oracle.c instantiated with dual numbers."""
+    global _dual
+    if _dual is None:
+        if not os.path.exists(DUAL_LIB):
+            build()
+        L = ctypes.CDLL(DUAL_LIB)
+        phys_p = ctypes.POINTER(ro_physics)
+        L.rod_l_jvp.argtypes = [ctypes.POINTER(ro_problem), phys_p, phys_p, _DP, _DP, _DP, _DP, ctypes.c_int]
+        L.rod_rk3_jvp.argtypes = [ctypes.POINTER(ro_problem), phys_p, phys_p, _DP, _DP, _DP, ctypes.c_int,
+                                  ctypes.c_int]
+        _dual = L
+    return _dual
+
+
+PARAMS = ("omega_ei", "omega_er", "kappa_e", "kappa_i", "kappa_r")
+
+
+def _dphys(dparams):
+    """ro_physics holding only the tangents of the five parameters."""
+    d = ro_physics(*([0.0] * 14))
+    if dparams is not None:
+        for name, v in zip(PARAMS, dparams):
+            setattr(d, name, float(v))
+    return d
+
+
+def l_jvp(prob, u, du, phys=None, dparams=None, nthreads=8):
+    """(l(u), d l(u)[du, dparams]) from the dual-number oracle."""
+    phys = phys or physics()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    du = np.ascontiguousarray(du, dtype=np.float64)
+    rhs, drhs = np.zeros_like(u), np.zeros_like(u)
+    dp = _dphys(dparams)
+    st = dual_lib().rod_l_jvp(ctypes.byref(prob.c), ctypes.byref(phys), ctypes.byref(dp), _dp(u), _dp(du),
+                              _dp(rhs), _dp(drhs), nthreads)
+    if st:
+        raise OracleError(st, ro_error())
+    return rhs, drhs
+
+
+def rk3_jvp(prob, u, du, dts, phys=None, dparams=None, nthreads=8):
+    """State and tangent after len(dts) rk3_ssp steps with frozen dts."""
+    phys = phys or physics()
+    u = np.array(u, dtype=np.float64, copy=True, order="C")
+    du = np.array(du, dtype=np.float64, copy=True, order="C")
+    dts = np.ascontiguousarray(dts, dtype=np.float64)
+    dp = _dphys(dparams)
+    st = dual_lib().rod_rk3_jvp(ctypes.byref(prob.c), ctypes.byref(phys), ctypes.byref(dp), _dp(u), _dp(du),
+                                _dp(dts), len(dts), nthreads)
+    if st:
+        raise OracleError(st, ro_error())
+    return u, du

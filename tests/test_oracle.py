@@ -37,6 +37,10 @@ a transposed operand fails:
   ro_rk3_step                linearity in dt of stage 1 for a single step
                              is not applicable (non-linear); pinned by the
                              constant-state fixed point and conservation
+  dual build (dual.cpp)      value part bitwise equal to the plain oracle;
+                             tangent against central differences of the
+                             plain oracle (l() and RK3 steps, state and
+                             parameter directions)
 """
 import math
 
@@ -417,3 +421,69 @@ def test_cylinder_step_is_ill_conditioned(rh3t):
     assert np.abs(up - u2).max() < 1e-14
     assert np.abs(gb - ga).max() > 1e-10
     assert np.abs(rb - ra).max() > 1e-9
+
+
+KW_ACTIVE = dict(omega_ei=1.0, omega_er=0.1, kappa_e=0.01, kappa_i=0.02, kappa_r=0.005)
+
+
+def _direction(u, fluid, seed):
+    """Random state direction scaled per field, zero off the fluid cells,
+    and a random parameter direction."""
+    rng = np.random.default_rng(seed)
+    du = rng.standard_normal(u.shape) * np.abs(u[fluid]).mean(axis=0)
+    du[~fluid] = 0.0
+    dpar = rng.standard_normal(5) * np.array([1.0, 0.1, 0.01, 0.02, 0.005])
+    return du, dpar
+
+
+def _shifted(kw, dpar, s):
+    return ob.physics(**{k: v + s * d for (k, v), d in zip(kw.items(), dpar)})
+
+
+@pytest.mark.parametrize("kw", [KW_ACTIVE, {k: 0.0 for k in KW_ACTIVE}], ids=["active", "zero-coefficients"])
+def test_dual_oracle_l_matches_central_differences(rh3t, workloads, kw):
+    """dual.cpp's tangent of l() against central differences (h = 1e-6) of
+    the plain oracle on a 16x16 periodic box, in a random state + parameter
+    direction; at zero exchange/diffusion coefficients the parameter tangent
+    must still switch the terms on."""
+    f = workloads.periodic_box(16)
+    u = workloads.smooth_modes_state(f, seed=3)
+    p = ob.OracleProblem(f)
+    du, dpar = _direction(u, f.fluid, 11)
+    rhs, drhs = ob.l_jvp(p, u, du, ob.physics(**kw), dpar)
+    assert np.array_equal(rhs, ob.l(p, u, phys=ob.physics(**kw)))
+    h = 1e-6
+    fd = (ob.l(p, u + h * du, phys=_shifted(kw, dpar, h)) - ob.l(p, u - h * du, phys=_shifted(kw, dpar, -h))) / (2 * h)
+    err = np.abs(fd - drhs).max() / np.abs(drhs).max()
+    print("dual vs FD", err)
+    assert err < 1e-6
+    # the parameter part alone is linear in the coefficients
+    _, dpo = ob.l_jvp(p, u, np.zeros_like(u), ob.physics(**kw), dpar)
+    lin = (ob.l(p, u, phys=_shifted(kw, dpar, 1.0)) - ob.l(p, u, phys=ob.physics(**kw)))
+    assert np.abs(lin - dpo).max() <= 1e-9 * np.abs(dpo).max()
+
+
+def test_dual_oracle_rk3_matches_central_differences(rh3t, workloads):
+    """Tangent of three RK3 steps (frozen dt) against central differences."""
+    f = workloads.periodic_box(16)
+    u = workloads.smooth_modes_state(f, seed=3)
+    p = ob.OracleProblem(f)
+    phys = ob.physics(**KW_ACTIVE)
+    dt = 0.05 * ob.global_dt(p, u, phys)
+    dts = [dt, 1.1 * dt, 0.9 * dt]
+    du, dpar = _direction(u, f.fluid, 12)
+    un, dun = ob.rk3_jvp(p, u, du, dts, phys, dpar)
+
+    def run(s):
+        x, ph = u + s * du, _shifted(KW_ACTIVE, dpar, s)
+        for d in dts:
+            x, _ = ob.rk3_step(p, x, d, phys=ph)
+        return x
+
+    ref = run(0.0)
+    assert np.array_equal(un, ref)
+    h = 1e-6
+    fd = (run(h) - run(-h)) / (2 * h)
+    err = np.abs(fd - dun).max() / np.abs(dun).max()
+    print("dual rk3 vs FD", err)
+    assert err < 1e-6
