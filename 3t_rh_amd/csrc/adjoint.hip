@@ -230,38 +230,44 @@ constexpr double jp_w(int t) { return t >= 1 ? jm_w(t - 1) : 0.0; }
 constexpr double kS6[9] = {0.0, -1.0, 9.0, -45.0, 0.0, 45.0, -9.0, 1.0, 0.0};
 constexpr double kLap[7] = {2.0, -27.0, 270.0, -490.0, 270.0, -27.0, 2.0};
 
-// One line (DIR 0: x, DIR 1: y) of cell-term pullbacks into point p: the
-// noncon terms of the 9 cells whose stencils hold p (pressure and velocity
-// cotangents) and the diffusion fluxes of the 7 cells around it
-// (temperature cotangents, T_r^4 for the radiation field).  The pressures
-// and velocities of the 11-point line are read once.
+// The gather stages, per (kGy x kGx) tile of points, the pressures, the two
+// velocities, the scaled energy cotangents rb = rscale * LAM[3..6) and the
+// point types of the tile plus a kGh-point halo in LDS; the noncon and
+// diffusion line pullbacks read their 11-point lines from there.
+constexpr int kGx = 64, kGy = 4, kGh = 5;
+constexpr int kGr = kGy + 2 * kGh, kGc = kGx + 2 * kGh;
+enum { T_PE = 0, T_PI, T_PR, T_U, T_V, T_R3, T_R4, T_R5, T_N };
+struct GatherTile {
+  double d[T_N][kGr][kGc];
+  uint8_t pt[kGr][kGc];
+};
+
+// One line (DIR 0: x, DIR 1: y) of cell-term pullbacks into the point at
+// tile position (ty, tx), grid index (i, j): the noncon terms of the 9 cells
+// whose stencils hold it (pressure and velocity cotangents) and the
+// diffusion fluxes of the 7 cells around it (temperature cotangents, T_r^4
+// for the radiation field).
 template <int DIR>
-__device__ __forceinline__ void line_gather(const AdjDesc& A, const Phys& ph, long p, int i, int j, double db[D_N],
-                                            double tb[3]) {
-  const Geo& g = A.g;
-  const long P = g.plane;
-  const long st = DIR == 0 ? g.PY : 1;
+__device__ __forceinline__ void line_gather(const GatherTile& T, const Geo& g, const Phys& ph, int ty, int tx,
+                                            int i, int j, double db[D_N], double tb[3]) {
   const double ds = DIR == 0 ? g.dx : g.dy;
   const double ids = 1.0 / ds;
   const int VP = DIR == 0 ? D_U : D_V;
   const int along = DIR == 0 ? i : j, across = DIR == 0 ? j : i;
   const int n_along = DIR == 0 ? g.nx : g.ny, n_across = DIR == 0 ? g.ny : g.nx;
   if (across < 0 || across >= n_across) return;
-  const double* D = A.D;
+  auto row = [&](int o) { return ty + kGh + (DIR == 0 ? o : 0); };
+  auto col = [&](int o) { return tx + kGh + (DIR == 0 ? 0 : o); };
   // p_e, p_i, p_r and the velocity at offsets -5..5 (0 outside the padded
   // planes; such points are never in the stencil of a fluid cell)
   double Pv[3][11], V[11];
 #pragma unroll
   for (int o = -5; o <= 5; ++o) {
-    const int a = along + o;
-    const bool in = a >= -4 && a < n_along + 4;
-    const long q = p + o * st;
 #pragma unroll
-    for (int m = 0; m < 3; ++m) Pv[m][o + 5] = in ? D[(D_PE + m) * P + q] : 0.0;
-    V[o + 5] = in ? D[VP * P + q] : 0.0;
+    for (int m = 0; m < 3; ++m) Pv[m][o + 5] = T.d[T_PE + m][row(o)][col(o)];
+    V[o + 5] = T.d[DIR == 0 ? T_U : T_V][row(o)][col(o)];
   }
-  // rho_m(c) = sum_e rb_e(c) M[e][m], M = [[2,-1,-1],[-1,2,-1],[-1,-1,2]],
-  // rb = rscale * LAM of the energy equations of cell c
+  // rho_m(c) = sum_e rb_e(c) M[e][m], M = [[2,-1,-1],[-1,2,-1],[-1,-1,2]]
   double Pb[3] = {0.0, 0.0, 0.0};
   double rc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // rho_m of cells -1, 0, +1
   bool fc[3] = {false, false, false};
@@ -270,14 +276,12 @@ __device__ __forceinline__ void line_gather(const AdjDesc& A, const Phys& ph, lo
 #pragma unroll
   for (int o = -4; o <= 4; ++o) {
     const int a = along + o;
-    const long c = p + o * st;
-    const bool fl = a >= 0 && a < n_along && A.pt[c] == 1;
+    const bool fl = a >= 0 && a < n_along && T.pt[row(o)][col(o)] == 1;
     if (!fl) continue;
-    const double r0 = A.rscale * A.LAM[3 * P + c], r1 = A.rscale * A.LAM[4 * P + c];
-    const double r2 = A.rscale * A.LAM[5 * P + c];
+    const double r0 = T.d[T_R3][row(o)][col(o)], r1 = T.d[T_R4][row(o)][col(o)], r2 = T.d[T_R5][row(o)][col(o)];
     const double sr = r0 + r1 + r2;
     const double rh[3] = {3.0 * r0 - sr, 3.0 * r1 - sr, 3.0 * r2 - sr};
-    const int t = 4 - o;  // position of p in the stencil of c
+    const int t = 4 - o;  // position of the point in the stencil of cell o
     const double vc = V[o + 5], vm = V[o + 4], vp = V[o + 6];
     const double cl = fmax(fmax(vc, vm), 0.0) * (1.0 / 3.0);
     const double cr = fmin(fmin(vc, vp), 0.0) * (1.0 / 3.0);
@@ -299,9 +303,9 @@ __device__ __forceinline__ void line_gather(const AdjDesc& A, const Phys& ph, lo
   db[D_PE] += Pb[0];
   db[D_PI] += Pb[1];
   db[D_PR] += Pb[2];
-  // velocity: the v/3 factor of cell p and the max/min selections of cl
-  // (cells p, p+1) and cr (cells p-1, p).  sh: offset of a cell's stencil
-  // origin in the line.
+  // velocity: the v/3 factor of the point's own cell and the max/min
+  // selections of cl (cells 0, +1) and cr (cells -1, 0).  sh: offset of a
+  // cell's stencil origin in the line.
   auto jump_sum = [&](const double r[3], int sh, bool plus) {
     double acc = 0.0;
 #pragma unroll
@@ -355,14 +359,34 @@ __device__ __forceinline__ void laplacian(const AdjDesc& A, long c, double lap[3
 }
 
 __global__ __launch_bounds__(256) void k_adj_gather(AdjDesc A, Phys ph) {
-  __shared__ double red[kNParam][256];
+  __shared__ GatherTile T;
   const Geo g = A.g;
   const long P = g.plane;
-  const long p = (long)blockIdx.x * 256 + threadIdx.x;
+  const int tiles_c = (g.PY + kGx - 1) / kGx;
+  const int ip0 = (int)(blockIdx.x / tiles_c) * kGy, jp0 = (int)(blockIdx.x % tiles_c) * kGx;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * kGx + tx;
+  for (int idx = tid; idx < kGr * kGc; idx += kGx * kGy) {
+    const int r = idx / kGc, c = idx % kGc;
+    const int ip = ip0 - kGh + r, jp = jp0 - kGh + c;
+    const bool in = ip >= 0 && ip < g.nx + 8 && jp >= 0 && jp < g.PY;
+    const long q = (long)ip * g.PY + jp;
+    T.pt[r][c] = in ? A.pt[q] : 0;
+    T.d[T_PE][r][c] = in ? A.D[D_PE * P + q] : 0.0;
+    T.d[T_PI][r][c] = in ? A.D[D_PI * P + q] : 0.0;
+    T.d[T_PR][r][c] = in ? A.D[D_PR * P + q] : 0.0;
+    T.d[T_U][r][c] = in ? A.D[D_U * P + q] : 0.0;
+    T.d[T_V][r][c] = in ? A.D[D_V * P + q] : 0.0;
+    T.d[T_R3][r][c] = in ? A.rscale * A.LAM[3 * P + q] : 0.0;
+    T.d[T_R4][r][c] = in ? A.rscale * A.LAM[4 * P + q] : 0.0;
+    T.d[T_R5][r][c] = in ? A.rscale * A.LAM[5 * P + q] : 0.0;
+  }
+  __syncthreads();
+  const int ip = ip0 + ty, jp = jp0 + tx;
+  const long p = (long)ip * g.PY + jp;
+  const int kind = (ip < g.nx + 8 && jp < g.PY) ? T.pt[ty + kGh][tx + kGh] : 0;
   double pg[kNParam] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  const int kind = p < P ? A.pt[p] : 0;
   if (kind != 0) {
-    const int i = (int)(p / g.PY) - 4, j = (int)(p % g.PY) - 4;
+    const int i = ip - 4, j = jp - 4;
     double sb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     // the 6 x and 6 y interfaces whose stencils hold p
     const long fxpl = (long)(g.nx + 1) * g.ny, fypl = (long)g.nx * (g.ny + 1);
@@ -390,16 +414,16 @@ __global__ __launch_bounds__(256) void k_adj_gather(AdjDesc A, Phys ph) {
 #pragma unroll
     for (int k = 0; k < D_N; ++k) db[k] = 0.0;
     double tb[3] = {0.0, 0.0, 0.0};
-    line_gather<0>(A, ph, p, i, j, db, tb);
-    line_gather<1>(A, ph, p, i, j, db, tb);
+    line_gather<0>(T, g, ph, ty, tx, i, j, db, tb);
+    line_gather<1>(T, g, ph, ty, tx, i, j, db, tb);
     if (ph.diffusion) {
       db[D_TE] += tb[0];
       db[D_TI] += tb[1];
       db[D_TR] += tb[2] * 4.0 * cube(A.D[D_TR * P + p]);
     }
     if (kind == 1) {
-      const double rb3 = A.rscale * A.LAM[3 * P + p], rb4 = A.rscale * A.LAM[4 * P + p];
-      const double rb5 = A.rscale * A.LAM[5 * P + p];
+      const double rb3 = T.d[T_R3][ty + kGh][tx + kGh], rb4 = T.d[T_R4][ty + kGh][tx + kGh];
+      const double rb5 = T.d[T_R5][ty + kGh][tx + kGh];
       const double te = A.D[D_TE * P + p], ti = A.D[D_TI * P + p], tr = A.D[D_TR * P + p];
       if (ph.source) {  // src/source.rs:7-21
         const double te3 = 4.0 * cube(te), tr3 = 4.0 * cube(tr);
@@ -433,18 +457,21 @@ __global__ __launch_bounds__(256) void k_adj_gather(AdjDesc A, Phys ph) {
       for (int f = 0; f < 6; ++f) A.SB[f * P + p] = sb[f];
     }
   }
-#pragma unroll
-  for (int k = 0; k < kNParam; ++k) red[k][threadIdx.x] = pg[k];
+  // block partial sums of the parameter cotangents (the tile's LDS is reused)
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
+  double(*red)[kGx * kGy] = reinterpret_cast<double(*)[kGx * kGy]>(&T.d[0][0][0]);
 #pragma unroll
-      for (int k = 0; k < kNParam; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+  for (int k = 0; k < kNParam; ++k) red[k][tid] = pg[k];
+  __syncthreads();
+  for (int w = kGx * kGy / 2; w > 0; w >>= 1) {
+    if (tid < w)
+#pragma unroll
+      for (int k = 0; k < kNParam; ++k) red[k][tid] += red[k][tid + w];
     __syncthreads();
   }
   // each block owns its slots: partial sums accumulate over the VJPs of a
   // call in launch order
-  if (threadIdx.x < kNParam) A.PG[blockIdx.x * kNParam + threadIdx.x] += red[threadIdx.x][0];
+  if (tid < kNParam) A.PG[blockIdx.x * kNParam + tid] += red[tid][0];
 }
 
 // One lane per source cell: adds the cotangents of the cheap ghosts copied
@@ -629,7 +656,7 @@ int init_adjoint(Engine& E) {
   HIP_TRY(hipMemset(W->SB, 0, plane6 * sizeof(double)));
   if ((st = dev_alloc(E, &W->XB, 36 * (size_t)(g.nx + 1) * g.ny))) return st;
   if ((st = dev_alloc(E, &W->YB, 36 * (size_t)g.nx * (g.ny + 1)))) return st;
-  W->pg_blocks = (int)((g.plane + 255) / 256);
+  W->pg_blocks = ((g.nx + 8 + kGy - 1) / kGy) * ((g.PY + kGx - 1) / kGx);
   if ((st = dev_alloc(E, &W->PG, (size_t)kNParam * W->pg_blocks))) return st;
   if ((st = dev_alloc(E, &W->pacc, (size_t)kNParam))) return st;
   if ((st = dev_alloc(E, &W->aos, 6 * (size_t)g.nx * g.ny))) return st;
@@ -690,7 +717,7 @@ void launch_vjp(Engine& E, int buf, const double* LAM, double rscale, double a, 
   }
   {
     Launcher L(E, K_ADJ_GATHER);
-    k_adj_gather<<<W.pg_blocks, 256, 0, E.stream>>>(A, E.phys);
+    k_adj_gather<<<W.pg_blocks, dim3(kGx, kGy), 0, E.stream>>>(A, E.phys);
   }
   if (W.nrows > 0) {
     Launcher L(E, K_ADJ_GHOST);
