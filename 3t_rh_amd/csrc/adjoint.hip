@@ -90,7 +90,8 @@ constexpr int kIfBx = 64;
 constexpr int kCb = 60;
 
 template <int DIR>
-__global__ __launch_bounds__(kIfBx) void k_adj_iface(AdjDesc A, EigConst K, Phys ph) {
+__global__ __launch_bounds__(kIfBx) void k_adj_iface(AdjDesc A, const EigConst* __restrict__ Kp, Phys ph) {
+  const EigConst& K = *Kp;
   __shared__ double s_cb[kCb][kIfBx];
   const Geo g = A.g;
   const int ni = DIR == 0 ? g.nx + 1 : g.nx;
@@ -708,12 +709,12 @@ void launch_vjp(Engine& E, int buf, const double* LAM, double rscale, double a, 
   {
     Launcher L(E, K_ADJ_IFACE_X);
     const int tiles = ((g.ny + kIfBx - 1) / kIfBx) * (g.nx + 1);
-    k_adj_iface<0><<<tiles, kIfBx, 0, E.stream>>>(A, E.eig, E.phys);
+    k_adj_iface<0><<<tiles, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys);
   }
   {
     Launcher L(E, K_ADJ_IFACE_Y);
     const int tiles = ((g.ny + 1 + kIfBx - 1) / kIfBx) * g.nx;
-    k_adj_iface<1><<<tiles, kIfBx, 0, E.stream>>>(A, E.eig, E.phys);
+    k_adj_iface<1><<<tiles, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys);
   }
   {
     Launcher L(E, K_ADJ_GATHER);

@@ -16,6 +16,7 @@
 
 #include "device_math.hpp"
 #include "flux.hpp"
+#include "kernels.hpp"
 
 #pragma clang fp contract(fast)
 
@@ -359,13 +360,6 @@ __device__ __forceinline__ void roe_bar_final(const EigConst& K, const RoeV& R, 
 __device__ __forceinline__ void reload_barrier() { __asm__ volatile("" ::: "memory"); }
 
 // -------------------------------------------------- interface flux VJP
-
-// Element at byte offset boff of a plane: a uniform (scalar) plane base plus
-// a 32-bit per-lane offset, so the stencil addresses of an interface cost one
-// VGPR per point instead of a 64-bit address per value.
-__device__ __forceinline__ double plane_at(const double* base, unsigned boff) {
-  return *(const double*)((const char*)base + boff);
-}
 
 // State and physical flux (flux_from_derived, src/state.rs:56-85) of the
 // stencil point at byte offset boff of the padded planes.

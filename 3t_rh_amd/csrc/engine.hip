@@ -898,6 +898,8 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   E.last.assign(n, rh3t_error{});
   int st;
   if ((st = dev_alloc(E, &E.d_sc, (size_t)n))) return st;
+  if ((st = dev_alloc(E, &E.d_eig, (size_t)1))) return st;
+  HIP_TRY(hipMemcpy(E.d_eig, &E.eig, sizeof(EigConst), hipMemcpyHostToDevice));
   {
     std::vector<Scalars> sc(n);
     for (auto& s : sc) {
@@ -1325,8 +1327,11 @@ int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode) {
 int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys) {
   if (!ctx || !phys) return RH3T_ERR_ARG;
   Engine& E = ctx->e;
+  HIP_TRY(hipSetDevice(E.device));
+  HIP_TRY(hipStreamSynchronize(E.stream));
   E.phys = make_phys(*phys);
   E.eig = make_eig(E.phys);
+  HIP_TRY(hipMemcpy(E.d_eig, &E.eig, sizeof(EigConst), hipMemcpyHostToDevice));
   E.nplanes = (E.adj || E.phys.diffusion || E.phys.source) ? D_N : 8;
   if (E.graph) {
     (void)hipGraphExecDestroy(E.graph);

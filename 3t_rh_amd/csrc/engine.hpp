@@ -90,6 +90,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   Phys phys{};
   EigConst eig{};
+  EigConst* d_eig = nullptr;  // device copy for kernels that read it through a pointer
   bool fast = true;
   int nplanes = 8;
   int nsolve = 0;

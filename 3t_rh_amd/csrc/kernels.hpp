@@ -27,6 +27,16 @@ __host__ __device__ inline long pad_index(const Geo& g, long i, long j) {
   return (i + 4) * (long)g.PY + (j + 4);
 }
 
+// Element at byte offset boff of a plane in global memory: a uniform
+// (scalar) plane base plus a 32-bit per-lane offset, so a stencil's addresses
+// cost one VGPR per point instead of a 64-bit flat address per value (planes
+// are far below 4 GiB).
+typedef __attribute__((address_space(1))) const char gchar;
+typedef __attribute__((address_space(1))) const double gdouble;
+__device__ __forceinline__ double plane_at(const double* base, unsigned boff) {
+  return *(gdouble*)((gchar*)base + boff);
+}
+
 // Error key: step (20 bits) | phase (4) | kind (8) | index (32); the
 // smallest key of a call is its first failure in program order.
 // Phases: 1 dt, 2/3 stage-1 ghosts/update, 4/5 stage 2, 6/7 stage 3.

@@ -34,6 +34,14 @@ PEAK_HBM = 8.0e12      # MI355X HBM3E, B/s
 FLOP_PER_INTERFACE = 2252
 FLOP_PER_CELL_STAGE = 5250
 BYTES_PER_CELL_STAGE = 128  # stage 1: read U^k, write U (96 B); stages 2-3: + U^n (144 B)
+# HBM traffic of one k_flux<0> launch on the 2048^2 box, from two rocprofv3
+# --pmc passes over this bench (FETCH_SIZE and WRITE_SIZE, KiB per dispatch,
+# mean of 15 dispatches), each scaled by what the same counter reads for a
+# plain 1 GiB device-to-device copy (`--copy-calibration`: FETCH_SIZE reads
+# 524,304 KiB -> x2.0, WRITE_SIZE 1,048,576 KiB -> x1.0).
+FLUX_X_FETCH_KIB, FLUX_X_WRITE_KIB = 239769, 229192
+COPY_FETCH_SCALE, COPY_WRITE_SCALE = 2.0, 1.0
+FLUX_X_TRAFFIC_BYTES = (FLUX_X_FETCH_KIB * COPY_FETCH_SCALE + FLUX_X_WRITE_KIB * COPY_WRITE_SCALE) * 1024
 
 
 def log(msg):
@@ -223,11 +231,24 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the differentiation leg (for profiling it); prints its JSON object")
     ap.add_argument("--fd-dirs", type=int, default=32, help="central-difference directions of the adjoint leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (three launches), the reference for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        import torch
+        n = 1 << 27  # doubles: 1 GiB
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 3}), flush=True)
+        return
     rh3t = importlib.import_module("3t_rh_amd")
     workloads = importlib.import_module("3t_rh_amd.workloads")
     if args.adjoint_only:
@@ -287,7 +308,8 @@ def main():
         # same for MFMA and vector FP64 on MI355X); the kernel uses VALU FP64.
         "roofline": {"bound": "mfma", "pipe": "fp64 valu", "kernel": "flux_x", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                     "traffic": None, "flop_per_interface": FLOP_PER_INTERFACE, "interfaces": interfaces,
+                     "traffic": FLUX_X_TRAFFIC_BYTES if n == 2048 else None,
+                     "flop_per_interface": FLOP_PER_INTERFACE, "interfaces": interfaces,
                      "avg_launch_ms": fx_ms},
     }
     if not args.no_batched:
