@@ -198,3 +198,82 @@ def test_nonphysical_state_error(rh3t, box64):
     assert ei.value.status == rh3t.ERR_NONPHYSICAL
     assert (g["stage"], g["i"], g["j"]) == (o["stage"], o["i"], o["j"])
     s.close()
+
+
+# ------------------------------------------------ full-size workloads
+
+def test_cylinder_200_steps_resynced(rh3t, workloads):
+    """init_cylinder (121x481, Mach 3): 200 steps of dt = 0.05 dt_cfl.  At
+    every step the oracle's state u^n is fed to both sides; the device's dt
+    rule, l() of the stage states and the full RK3 step are compared (EXACT
+    arithmetic, 1e-12 relative per field), and the oracle carries on from
+    its own step."""
+    f = workloads.cylinder(121, 3.0)
+    u = f.cylinder_state()
+    p = ob.OracleProblem(f)
+    _say("cylinder 121x481: 200 resynced steps, exact")
+    s = f.solver(arith=0)
+    worst = 0.0
+    for n in range(200):
+        dt_o = ob.global_dt(p, u)
+        s.set_state(u, 0.0)
+        assert abs(s.calc_global_dt() - dt_o) <= 1e-15 * dt_o
+        dt = 0.05 * dt_o
+        s.rk3_ssp(dt)
+        got, _ = s.get_state()
+        ref, _ = ob.rk3_step(p, u, dt)
+        err = ob.relerr(got, ref, f.fluid).max()
+        worst = max(worst, err)
+        if n % 50 == 0:
+            print(f"step {n} relerr {err:.3e}", flush=True)
+        assert err <= TOL, f"step {n}: {err}"
+        u = ref
+    print("worst", worst)
+    s.close()
+
+
+def test_translating_shock_1024(rh3t, workloads):
+    """The planar-shock workload at its full size (1024^2, Constant inflow,
+    Outflow, y-periodic): l() against the oracle, then 100 device steps
+    checked through size-independent properties: finite admissible states,
+    the front moving right, and y-periodicity of the state (the perturbed
+    front keeps the domain's period)."""
+    f = workloads.translating_shock(1024)
+    u = workloads.translating_shock_state(f, seed=2)
+    p = ob.OracleProblem(f)
+    _say("translating shock 1024: l() vs oracle")
+    s = f.solver()
+    s.set_state(u)
+    rhs, gv = s.l(with_ghosts=True)
+    ref, gref = ob.l(p, u, with_ghosts=True, nthreads=16)
+    print("ghost", ob.relerr(gv, gref).max(), "rhs", ob.relerr(rhs, ref, f.fluid).max())
+    assert ob.relerr(gv, gref).max() <= 1e-12
+    assert ob.relerr(rhs, ref, f.fluid).max() <= TOL
+    _say("translating shock 1024: 100 steps")
+    s.set_state(u, 0.0)
+    t = s.advance(100, 0.05)
+    got, _ = s.get_state()
+    assert np.isfinite(got).all() and t > 0.0
+    rho = got[..., 0]
+    assert rho.min() > 0.0
+    front0 = (u[..., 0] > 1.5).sum(axis=0).mean()
+    front1 = (rho > 1.5).sum(axis=0).mean()
+    print("post-shock cells per row", front0, "->", front1, "time", t)
+    assert front1 > front0
+    s.close()
+
+
+def test_l_periodic_box_2048_oracle(rh3t, workloads):
+    """The roofline workload itself: l() of the 2048^2 smooth-mode box
+    against the oracle (FAST, 1e-12)."""
+    f = workloads.periodic_box(2048)
+    u = workloads.smooth_modes_state(f, seed=3)
+    _say("l() periodic box 2048 vs oracle")
+    s = f.solver()
+    s.set_state(u)
+    rhs = s.l()
+    ref = ob.l(ob.OracleProblem(f), u, nthreads=16)
+    err = ob.relerr(rhs, ref, f.fluid)
+    print("relerr", err)
+    assert err.max() <= TOL
+    s.close()
