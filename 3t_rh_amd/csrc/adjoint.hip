@@ -912,6 +912,7 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
   }
   HIP_TRY(hipMemcpyAsync(A->final_state, d.U[0], bytes, hipMemcpyDeviceToDevice, E.stream));
   HIP_TRY(hipGetLastError());
+  E.dt_partials_valid = true;
   return check_errors(E);
 }
 
@@ -956,9 +957,11 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
   }
   HIP_TRY(hipGetLastError());
   if ((st = download_cot(E, lam, lam_initial, pgrad))) return st;
-  // leave the context at the forward's final state
+  // leave the context at the forward's final state (the recompute's stage 3
+  // overwrote the dt minima)
   HIP_TRY(hipMemcpyAsync(d.U[0], A->final_state, bytes, hipMemcpyDeviceToDevice, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
+  E.dt_partials_valid = false;
   return check_errors(E);
 }
 

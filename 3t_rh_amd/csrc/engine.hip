@@ -20,10 +20,13 @@
 //   k_assemble      flux differences + noncon x/y + source + diffusion
 //                   (src/main.rs:251-304) fused with the RK stage combination
 //                   and the admissibility check (src/main.rs:307-349).
-//   k_dt_partial / k_dt_final   calc_global_dt as a two-level fmin
-//                   reduction per solve (src/main.rs:375-408, src/dt.rs:5-34)
-//                   and dt = cfl_factor * dt_cfl (src/main.rs:461), on the
-//                   device.
+//   k_dt_final      calc_global_dt (src/main.rs:375-408, src/dt.rs:5-34) as
+//                   wavefront fmin reductions: the last RK stage's assembly
+//                   leaves one minimum of get_local_dt of the new state per
+//                   workgroup (k_dt_partial does the same from U0 when the
+//                   state came from the host), one workgroup per solve
+//                   reduces them and sets dt = cfl_factor * dt_cfl
+//                   (src/main.rs:461), on the device.
 // Every grid-shaped kernel enumerates the tiles of all solves in one launch:
 // a per-kernel prefix table maps a (remapped) workgroup index to its solve.
 #include <hip/hip_runtime.h>
@@ -427,17 +430,12 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
 
 // --------------------------------------------------------- assemble
 
-__global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ descs, const int* __restrict__ tiles,
-                                                  int nsolve, Phys ph, StageSpec sa) {
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int s = find_solve(tiles, nsolve, t);
-  const SolveDesc& sd = descs[s];
+// One cell (i, j) of solve sd: l()'s right-hand side and, for sa.mode >= 1,
+// the stage update.  Returns get_local_dt of the updated state after stage 3
+// (the dt rule of the next step), +inf otherwise.
+__device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys& ph, const StageSpec& sa, int i,
+                                                int j) {
   const Geo g = sd.g;
-  const int tiles_j = (g.ny + blockDim.x - 1) / blockDim.x;
-  const int lt = t - tiles[s];
-  const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
-  const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
-  if (i >= g.nx || j >= g.ny) return;
   const long c0 = pad_index(g, i, j);
   const long lin = (long)i * g.ny + j;
   const long ncell = (long)g.nx * g.ny;
@@ -445,7 +443,7 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ 
     if (sa.mode == 0)
 #pragma unroll
       for (int f = 0; f < 6; ++f) sd.RHS[f * ncell + lin] = 0.0;
-    return;
+    return INFINITY;
   }
   const double* __restrict__ D = sd.D;
   const double* __restrict__ FX = sd.FX;
@@ -513,7 +511,7 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ 
   if (sa.mode == 0) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) sd.RHS[f * ncell + lin] = r[f];
-    return;
+    return INFINITY;
   }
   // stage_update_rhs / stage_update_comb, src/main.rs:307-349
   const double dt = sd.sc->dt;
@@ -537,6 +535,42 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ 
 #pragma unroll
   for (int f = 0; f < 6; ++f) dst[f * g.plane + c0] = v[f];
   if (!admissible(v)) raise_error(sd.sc, sa.phase, RH3T_ERR_NONPHYSICAL, (unsigned)lin);
+  return sa.mode == 3 ? local_dt(ph, v, g.dx, g.dy) : INFINITY;
+}
+
+// fmin over the workgroup, written to partial[tile]: a wavefront reduction
+// (lane shuffles) per wavefront, then over the <= 4 wavefronts.  Every lane
+// of the workgroup takes part.  fmin is exact and ignores NaN like the
+// serial f64::min of src/main.rs:402-403, so the result does not depend on
+// the reduction order.
+__device__ __forceinline__ void put_group_min(double m, double* __restrict__ partial, int t) {
+  __shared__ double s_min[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x, nw = (blockDim.x * blockDim.y) >> 6;
+  if ((tid & 63) == 0) s_min[tid >> 6] = m;
+  __syncthreads();
+  if (tid == 0) {
+    double r = s_min[0];
+    for (int w = 1; w < nw; ++w) r = fmin(r, s_min[w]);
+    partial[t] = r;
+  }
+}
+
+// Assembly + stage update of every solve; the last RK stage also leaves the
+// per-workgroup minima of get_local_dt of the new state (src/dt.rs:5-34)
+// for the next step's dt.
+__global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ descs, const int* __restrict__ tiles,
+                                                  int nsolve, Phys ph, StageSpec sa, double* __restrict__ partial) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int s = find_solve(tiles, nsolve, t);
+  const SolveDesc& sd = descs[s];
+  const int tiles_j = (sd.g.ny + blockDim.x - 1) / blockDim.x;
+  const int lt = t - tiles[s];
+  const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
+  const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
+  const double m = (i < sd.g.nx && j < sd.g.ny) ? assemble_cell(sd, ph, sa, i, j) : INFINITY;
+  if (sa.mode == 3) put_group_min(m, partial, t);
 }
 
 // One thread per solve: copies the state of the first failure of `phase`
@@ -557,16 +591,22 @@ __global__ void k_capture(const SolveDesc* __restrict__ descs, int nsolve, unsig
 
 // ------------------------------------------------------------- dt
 
+// get_local_dt (src/dt.rs:5-34) of the state in U0 on every fluid cell,
+// reduced per workgroup; same tiles and partial layout as k_assemble.  Used
+// when the state did not come out of a stage 3 (first step, host writes).
 __global__ __launch_bounds__(256) void k_dt_partial(const SolveDesc* __restrict__ descs,
                                                     const int* __restrict__ tiles, int nsolve, Phys ph,
                                                     double* __restrict__ partial) {
-  __shared__ double red[256];
-  const int s = find_solve(tiles, nsolve, blockIdx.x);
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int s = find_solve(tiles, nsolve, t);
   const SolveDesc& sd = descs[s];
-  const long lin = (long)(blockIdx.x - tiles[s]) * 256 + threadIdx.x;
+  const int tiles_j = (sd.g.ny + blockDim.x - 1) / blockDim.x;
+  const int lt = t - tiles[s];
+  const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
+  const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
   double m = INFINITY;
-  if (lin < (long)sd.g.nx * sd.g.ny) {
-    long p = pad_index(sd.g, lin / sd.g.ny, lin % sd.g.ny);
+  if (i < sd.g.nx && j < sd.g.ny) {
+    const long p = pad_index(sd.g, i, j);
     if (sd.pt[p] == 1) {
       const double* S = sd.U[0];
       double st[6];
@@ -575,32 +615,28 @@ __global__ __launch_bounds__(256) void k_dt_partial(const SolveDesc* __restrict_
       m = local_dt(ph, st, sd.g.dx, sd.g.dy);
     }
   }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+  put_group_min(m, partial, t);
 }
 
-// One workgroup per solve.  fmin is exact, so the two-level reduction equals
-// the serial f64::min of src/main.rs:402-403 whatever the order.
-__global__ __launch_bounds__(256) void k_dt_final(const SolveDesc* __restrict__ descs, const int* __restrict__ tiles,
-                                                  const double* __restrict__ partial, int advance) {
-  __shared__ double red[256];
+// One workgroup (1024 lanes) per solve over its workgroup minima:
+// calc_global_dt (src/main.rs:375-408) and, when advancing,
+// dt = cfl_factor * dt_cfl (src/main.rs:461) or the fixed dt, the step
+// counter and Field.time.
+__global__ __launch_bounds__(1024) void k_dt_final(const SolveDesc* __restrict__ descs,
+                                                   const int* __restrict__ tiles,
+                                                   const double* __restrict__ partial, int advance) {
+  __shared__ double red[16];
   const int s = blockIdx.x;
   double m = INFINITY;
-  for (int k = tiles[s] + threadIdx.x; k < tiles[s + 1]; k += 256) m = fmin(m, partial[k]);
-  red[threadIdx.x] = m;
+  for (int k = tiles[s] + threadIdx.x; k < tiles[s + 1]; k += 1024) m = fmin(m, partial[k]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    Scalars* sc = descs[s].sc;
     double dt_cfl = red[0];
+    for (int w = 1; w < 16; ++w) dt_cfl = fmin(dt_cfl, red[w]);
+    Scalars* sc = descs[s].sc;
     sc->dt_cfl = dt_cfl;
     if (advance) {
       sc->step = sc->step + 1;
@@ -674,7 +710,6 @@ int build_tables(Engine& E) {
   E.t_fy.assign(n + 1, 0);
   E.t_as.assign(n + 1, 0);
   E.t_der.assign(n + 1, 0);
-  E.t_dt.assign(n + 1, 0);
   for (int s = 0; s < n; ++s) {
     const Geo& g = E.solves[s].g;
     auto tiles = [&](int ni, int nj) { return ((nj + E.bx - 1) / E.bx) * ((ni + E.by - 1) / E.by); };
@@ -682,15 +717,22 @@ int build_tables(Engine& E) {
     E.t_fy[s + 1] = E.t_fy[s] + tiles(g.nx, g.ny + 1);
     E.t_as[s + 1] = E.t_as[s] + tiles(g.nx, g.ny);
     E.t_der[s + 1] = E.t_der[s] + (int)((g.plane + 255) / 256);
-    E.t_dt[s + 1] = E.t_dt[s] + (int)(((long)g.nx * g.ny + 255) / 256);
   }
   std::vector<int> all;
-  for (auto* t : {&E.t_fx, &E.t_fy, &E.t_as, &E.t_der, &E.t_dt}) all.insert(all.end(), t->begin(), t->end());
+  for (auto* t : {&E.t_fx, &E.t_fy, &E.t_as, &E.t_der}) all.insert(all.end(), t->begin(), t->end());
   if (!E.d_tabs) {
     int st = dev_alloc(E, &E.d_tabs, all.size());
     if (st) return st;
   }
   HIP_TRY(hipMemcpy(E.d_tabs, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+  // one dt partial per assembly workgroup
+  const long need = (long)total(E.t_as);
+  if (need > E.partial_cap) {
+    int st = dev_alloc(E, &E.d_partial, (size_t)need);
+    if (st) return st;
+    E.partial_cap = need;
+  }
+  E.dt_partials_valid = false;
   if (E.graph) {
     (void)hipGraphExecDestroy(E.graph);
     E.graph = nullptr;
@@ -742,7 +784,7 @@ void launch_stage(Engine& E, int src, const StageSpec& sa) {
   }
   {
     Launcher L(E, K_ASSEMBLE);
-    k_assemble<<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa);
+    k_assemble<<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
   }
   if (sa.mode > 0) {
     Launcher L(E, K_CAPTURE);
@@ -750,13 +792,16 @@ void launch_stage(Engine& E, int src, const StageSpec& sa) {
   }
 }
 
-void launch_dt(Engine& E, int advance) {
-  {
+// dt of every solve from the wavefront minima; fresh: recompute them from
+// U0 first (otherwise the last stage 3 left them for the current state)
+void launch_dt(Engine& E, int advance, bool fresh) {
+  if (fresh) {
     Launcher L(E, K_DT_PARTIAL);
-    k_dt_partial<<<total(E.t_dt), 256, 0, E.stream>>>(E.d_desc, E.tab(4), E.nsolve, E.phys, E.d_partial);
+    k_dt_partial<<<total(E.t_as), dim3(E.bx, E.by), 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys,
+                                                                   E.d_partial);
   }
   Launcher L(E, K_DT_FINAL);
-  k_dt_final<<<E.nsolve, 256, 0, E.stream>>>(E.d_desc, E.tab(4), E.d_partial, advance);
+  k_dt_final<<<E.nsolve, 1024, 0, E.stream>>>(E.d_desc, E.tab(2), E.d_partial, advance);
 }
 
 // rk3_ssp, src/main.rs:351-373, with the state kept in U0:
@@ -1058,7 +1103,6 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   UP(h_W, dp) E.heavy.W = dp;
 #undef UP
   if ((st = build_tables(E))) return st;
-  if ((st = dev_alloc(E, &E.d_partial, (size_t)std::max(1, total(E.t_dt))))) return st;
   return RH3T_OK;
 }
 
@@ -1075,6 +1119,7 @@ int set_state(Engine& E, int s, const double* u, double time) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(&E.d_sc[s].time, &time, sizeof(double), hipMemcpyHostToDevice, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
+  E.dt_partials_valid = false;
   return RH3T_OK;
 }
 
@@ -1111,9 +1156,14 @@ int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
   }
   HIP_TRY(hipMemcpyAsync(E.d_sc, sc.data(), sizeof(Scalars) * E.nsolve, hipMemcpyHostToDevice, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
+  if (!E.dt_partials_valid) {
+    Launcher L(E, K_DT_PARTIAL);
+    k_dt_partial<<<total(E.t_as), dim3(E.bx, E.by), 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys,
+                                                                   E.d_partial);
+  }
   if (E.profiling) {
     for (int s = 0; s < nsteps; ++s) {
-      launch_dt(E, 1);
+      launch_dt(E, 1, false);
       launch_rk3(E);
     }
     HIP_TRY(hipGetLastError());
@@ -1121,7 +1171,7 @@ int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
     if (!E.graph) {
       hipGraph_t gr;
       HIP_TRY(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
-      launch_dt(E, 1);
+      launch_dt(E, 1, false);
       launch_rk3(E);
       HIP_TRY(hipStreamEndCapture(E.stream, &gr));
       hipError_t e = hipGraphInstantiate(&E.graph, gr, nullptr, nullptr, 0);
@@ -1134,6 +1184,8 @@ int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
     }
     for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(E.graph, E.stream));
   }
+  // the last stage 3 left the wavefront minima of the new state
+  E.dt_partials_valid = nsteps > 0 || E.dt_partials_valid;
   return check_errors(E);
 }
 
@@ -1181,7 +1233,9 @@ int profile_read(Engine& E, int max_kernels, char* names, int names_cap, double*
 }
 
 int set_launch(Engine& E, int bx, int by, int gpb) {
-  if (bx <= 0 || by <= 0 || bx * by > 256 || bx % 16 != 0 || gpb < 1 || gpb > kMaxGhostsPerBlock) return RH3T_ERR_ARG;
+  if (bx <= 0 || by <= 0 || bx * by > 256 || bx % 16 != 0 || (bx * by) % 64 != 0 || gpb < 1 ||
+      gpb > kMaxGhostsPerBlock)
+    return RH3T_ERR_ARG;
   E.bx = bx;
   E.by = by;
   E.ghosts_per_block = gpb;
@@ -1283,6 +1337,7 @@ int rh3t_rk3_step(rh3t_ctx* ctx, double dt) {
   HIP_TRY(hipMemcpyAsync(&E.d_sc[0].dt, &dt, sizeof(double), hipMemcpyHostToDevice, E.stream));
   launch_rk3(E);
   HIP_TRY(hipGetLastError());
+  E.dt_partials_valid = true;
   // Field.time advances by dt (u3.time = u.time + dt, src/main.rs:370)
   double t = 0.0;
   HIP_TRY(hipMemcpyAsync(&t, &E.d_sc[0].time, sizeof(double), hipMemcpyDeviceToHost, E.stream));
@@ -1296,7 +1351,8 @@ int rh3t_calc_global_dt(rh3t_ctx* ctx, double* dt_cfl) {
   if (!ctx || !dt_cfl) return RH3T_ERR_ARG;
   Engine& E = ctx->e;
   HIP_TRY(hipSetDevice(E.device));
-  launch_dt(E, 0);
+  launch_dt(E, 0, true);
+  E.dt_partials_valid = true;
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(dt_cfl, &E.d_sc[0].dt_cfl, sizeof(double), hipMemcpyDeviceToHost, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));

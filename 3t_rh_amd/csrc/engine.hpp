@@ -108,9 +108,11 @@ struct Engine {
   // launch shapes (autotuned): flux / assemble workgroup (bx, by), heavy
   // ghosts per workgroup
   int bx = 64, by = 4, ghosts_per_block = 1;
-  std::vector<int> t_fx, t_fy, t_as, t_der, t_dt;  // host prefix tables
-  int* d_tabs = nullptr;  // device copy: fx | fy | as | der | dt, each nsolve+1
-  double* d_partial = nullptr;
+  std::vector<int> t_fx, t_fy, t_as, t_der;  // host prefix tables
+  int* d_tabs = nullptr;  // device copy: fx | fy | as | der, each nsolve+1
+  double* d_partial = nullptr;  // get_local_dt minimum per assembly workgroup
+  long partial_cap = 0;
+  bool dt_partials_valid = false;  // d_partial holds the minima of the current U0
   double* d_aos = nullptr;   // staging (largest solve)
   double* d_gvals = nullptr;
   std::vector<void*> allocs;
