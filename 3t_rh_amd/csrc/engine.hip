@@ -163,18 +163,25 @@ __device__ bool solve6(const double A0[6][6], const double b0[6], double x[6]) {
 // projected data and the smoothness indicators beta_r do not depend on k, so
 // they are formed once here (same values, a fifth of the work).
 constexpr int kMaxGhostsPerBlock = 4;
+constexpr int kGhostLds = 336 + 336 + 150 + 30 + 30;  // doubles per ghost
 
 __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph,
                                                      int buf, unsigned phase) {
   const int w = threadIdx.y;
   const int h = blockIdx.x * blockDim.y + w;
   const int lane = threadIdx.x;
-  __shared__ double s_data[kMaxGhostsPerBlock][6][56];  // L . u6 at the 55 E_r points
-  __shared__ double s_part[kMaxGhostsPerBlock][330];    // data[c][a] * (S_r data[c])[a]
-  __shared__ double s_vk[kMaxGhostsPerBlock][5][6][5];  // [k][c][r]
-  __shared__ double s_alpha[kMaxGhostsPerBlock][6][5];  // [c][r]
-  __shared__ double s_v[kMaxGhostsPerBlock][5][6];      // V^(k) per characteristic
+  // per-ghost LDS (dynamic, kGhostLds doubles per wavefront):
+  //   data[6][56]  L . u6 at the 55 E_r points
+  //   part[6][56]  data[c][a] * (S_r data[c])[a], a by E_r row
+  //   vk[5][6][5]  [k][c][r];  alpha[6][5]  [c][r];  v[5][6]  V^(k) per characteristic
+  extern __shared__ double s_dyn[];
+  double* const s_data = s_dyn + (long)w * kGhostLds;
+  double* const s_part = s_data + 336;
+  double* const s_vk = s_part + 336;
+  double* const s_alpha = s_vk + 150;
+  double* const s_v = s_alpha + 30;
   const bool active = h < hg.n;
+  const long row = active ? hg.tab[h] : 0;
   const SolveDesc& sd = descs[active ? hg.sid[h] : hg.sid[0]];
   double* S = sd.U[buf];
   const long plane = sd.g.plane;
@@ -196,7 +203,6 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     build_eigen<0>(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V],
                    d1[D_EE], d1[D_EI], d1[D_ER], E);
   }
-  const long row = active ? hg.tab[h] : 0;
   if (active && lane < 55) {
     long q = hg.stencil[55L * row + lane];
     double s[6];
@@ -204,59 +210,81 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     for (int f = 0; f < 6; ++f) s[f] = S[f * plane + q];
     double u6[6] = {s[0], s[1] * nx + s[2] * ny, -s[1] * ny + s[2] * nx, s[3], s[4], s[5]};
 #pragma unroll
-    for (int c = 0; c < 6; ++c) s_data[w][c][lane] = dot6(E.L[c], u6);
+    for (int c = 0; c < 6; ++c) s_data[(c) * 56 + lane] = dot6(E.L[c], u6);
   }
   __syncthreads();
   if (active) {
+    // One lane per row a of S_r (the 55 E_r points, r = 0..4) applies the row
+    // to all six characteristic data vectors, and one lane per (k, r), k <= r,
+    // applies row k of w_r: every table entry is read once per ghost.  Sums
+    // run over b / a in increasing order, as in weno_extrapolation_pre.
     const double* Sh = hg.S + 979L * row;
     const double* Wh = hg.W + 225L * row;
-    for (int q = lane; q < 330; q += 64) {
-      int r = 0, base = 0;
-      while (q >= base + 6 * (r + 1) * (r + 1)) {
-        base += 6 * (r + 1) * (r + 1);
-        ++r;
+    if (lane < 55) {
+      int r = 0;
+      while (lane >= kStOff[r + 1]) ++r;
+      const int m = (r + 1) * (r + 1), a = lane - kStOff[r];
+      const double* srow = Sh + kSOffD[r] + a * m;
+      double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int b = 0; b < m; ++b) {
+        const double sv = srow[b];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] += sv * s_data[(c) * 56 + kStOff[r] + b];
       }
-      int m = (r + 1) * (r + 1);
-      int c = (q - base) / m, a = (q - base) % m;
-      const double* row = Sh + kSOffD[r] + a * m;
-      const double* dc = &s_data[w][c][kStOff[r]];
-      double acc = 0.0;
-      for (int b = 0; b < m; ++b) acc += row[b] * dc[b];
-      s_part[w][q] = dc[a] * acc;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s_part[(c) * 56 + lane] = s_data[(c) * 56 + lane] * acc[c];
     }
-    for (int q = lane; q < 150; q += 64) {
-      int k = q / 30, rc = q % 30, r = rc / 6, c = rc % 6;
-      int m = (r + 1) * (r + 1);
-      double vk = 0.0;
-      if (k <= r) {
-        const double* wv = Wh + kWOffD[r] + k * m;
-        const double* dc = &s_data[w][c][kStOff[r]];
-        for (int a = 0; a < m; ++a) vk += wv[a] * dc[a];
+    // (k, r) pairs: lanes 55..63 take r = 4 (k = 0..4) and r = 3 (k = 0..3),
+    // lanes 0..5 the six pairs of r <= 2 (after their light S rows)
+    int k = -1, r = 0;
+    if (lane >= 55) {
+      const int q = lane - 55;
+      r = q < 5 ? 4 : 3;
+      k = q < 5 ? q : q - 5;
+    } else if (lane < 6) {
+      const int rr[6] = {2, 2, 2, 1, 1, 0}, kk[6] = {0, 1, 2, 0, 1, 0};
+      r = rr[lane];
+      k = kk[lane];
+    }
+    if (k >= 0) {
+      const int m = (r + 1) * (r + 1);
+      const double* wv = Wh + kWOffD[r] + k * m;
+      double vk[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int a = 0; a < m; ++a) {
+        const double wa = wv[a];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) vk[c] += wa * s_data[(c) * 56 + kStOff[r] + a];
       }
-      s_vk[w][k][c][r] = vk;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s_vk[((k) * 6 + c) * 5 + r] = vk[c];
+    }
+    // v_k = 0 for k > r (weno_extrapolation_pre)
+    if (lane >= 6 && lane < 16) {
+      const int kr[10][2] = {{1, 0}, {2, 0}, {3, 0}, {4, 0}, {2, 1}, {3, 1}, {4, 1}, {3, 2}, {4, 2}, {4, 3}};
+      const int kz = kr[lane - 6][0], rz = kr[lane - 6][1];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s_vk[((kz) * 6 + c) * 5 + rz] = 0.0;
     }
   }
   __syncthreads();
   if (active && lane < 30) {  // beta_r and alpha_r = d_r / (eps + beta_r)^q
     int r = lane / 6, c = lane % 6;
     int m = (r + 1) * (r + 1);
-    int base = 0;
-    for (int rr = 0; rr < r; ++rr) base += 6 * (rr + 1) * (rr + 1);
-    const double* p = &s_part[w][base + c * m];
+    const double* p = &s_part[(c) * 56 + kStOff[r]];
     double beta = 0.0;
     for (int a = 0; a < m; ++a) beta += p[a];
-    s_alpha[w][c][r] = hg.d5[5L * row + r] / weno_pow(ph.eps + beta, ph.weno_q);
+    s_alpha[(c) * 5 + r] = hg.d5[5L * row + r] / weno_pow(ph.eps + beta, ph.weno_q);
   }
   __syncthreads();
   if (active && lane < 30) {  // V^(k)_c = sum_r (alpha_r / sum alpha) v_k,r
     int k = lane / 6, c = lane % 6;
     double asum = 0.0;
 #pragma unroll
-    for (int r = 0; r < 5; ++r) asum += s_alpha[w][c][r];
+    for (int r = 0; r < 5; ++r) asum += s_alpha[(c) * 5 + r];
     double res = 0.0;
 #pragma unroll
-    for (int r = 0; r < 5; ++r) res += (s_alpha[w][c][r] / asum) * s_vk[w][k][c][r];
-    s_v[w][k][c] = res;
+    for (int r = 0; r < 5; ++r) res += (s_alpha[(c) * 5 + r] / asum) * s_vk[((k) * 6 + c) * 5 + r];
+    s_v[(k) * 6 + c] = res;
   }
   __syncthreads();
   if (!active || lane != 0) return;
@@ -271,16 +299,16 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     matvec6(E.L, uloc, vinf);
     double rhs[6];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) rhs[m] = E.lam[m] < 0.0 ? vinf[m] : s_v[w][0][m];
+    for (int m = 0; m < 6; ++m) rhs[m] = E.lam[m] < 0.0 ? vinf[m] : s_v[(0) * 6 + m];
     matvec6(E.R, rhs, uk[0]);
     for (int k = 1; k <= 4; ++k) {
 #pragma unroll
-      for (int m = 0; m < 6; ++m) rhs[m] = E.lam[m] < 0.0 ? 0.0 : s_v[w][k][m];
+      for (int m = 0; m < 6; ++m) rhs[m] = E.lam[m] < 0.0 ? 0.0 : s_v[(k) * 6 + m];
       matvec6(E.R, rhs, uk[k]);
     }
   } else if (kind == RH3T_BC_OUTFLOW) {  // src/bc1.rs:760-828
     const double* of = hg.outflow + 3L * h;
-    matvec6(E.R, s_v[w][0], uk[0]);
+    matvec6(E.R, (s_v + (0) * 6), uk[0]);
     double rho0 = uk[0][0];
     double un0 = uk[0][1] / rho0, ut0 = uk[0][2] / rho0;
     double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
@@ -293,20 +321,20 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     double lambda1 = un0 - a;
     double rhs1[6];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) rhs1[m] = s_v[w][1][m];
+    for (int m = 0; m < 6; ++m) rhs1[m] = s_v[(1) * 6 + m];
     if (lambda1 < 0.0) {
       double ml = fabs(un0 / a);
       double l1 = of[1] * (1.0 - sq(ml)) * a / of[2] * (p0 - of[0]);
       rhs1[0] = l1 / lambda1;
     }
     matvec6(E.R, rhs1, uk[1]);
-    for (int k = 2; k <= 4; ++k) matvec6(E.R, s_v[w][k], uk[k]);
+    for (int k = 2; k <= 4; ++k) matvec6(E.R, (s_v + (k) * 6), uk[k]);
   } else {  // Wall-ILW, src/bc1.rs:691-756
     double A[6][6], rhs[6];
     for (int i = 0; i < 6; ++i)
       for (int j = 0; j < 6; ++j) A[i][j] = E.L[i][j];
     for (int c = 0; c < 6; ++c) A[0][c] = c == 1 ? 1.0 : 0.0;
-    for (int m = 0; m < 6; ++m) rhs[m] = s_v[w][0][m];
+    for (int m = 0; m < 6; ++m) rhs[m] = s_v[(0) * 6 + m];
     rhs[0] = 0.0;
     solved = solve6(A, rhs, uk[0]);
     double u1 = uk[0][0], u3 = uk[0][2];
@@ -316,10 +344,10 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     A[0][3] = -ph.ge;
     A[0][4] = -ph.gi;
     A[0][5] = -ph.gr;
-    for (int m = 0; m < 6; ++m) rhs[m] = s_v[w][1][m];
+    for (int m = 0; m < 6; ++m) rhs[m] = s_v[(1) * 6 + m];
     rhs[0] = 0.0;
     solved = solved && solve6(A, rhs, uk[1]);
-    for (int k = 2; k <= 4; ++k) matvec6(E.R, s_v[w][k], uk[k]);
+    for (int k = 2; k <= 4; ++k) matvec6(E.R, (s_v + (k) * 6), uk[k]);
   }
   // Taylor expansion to the ghost point, rotation back to (x, y)
   double ug[6];
@@ -752,7 +780,8 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, bool capture) {
   if (E.heavy.n > 0) {
     Launcher L(E, K_GHOST_HEAVY);
     int g = E.ghosts_per_block;
-    k_ghost_heavy<<<(E.heavy.n + g - 1) / g, dim3(64, g), 0, E.stream>>>(E.d_desc, E.heavy, E.phys, buf, phase);
+    k_ghost_heavy<<<(E.heavy.n + g - 1) / g, dim3(64, g), (size_t)g * kGhostLds * sizeof(double), E.stream>>>(
+        E.d_desc, E.heavy, E.phys, buf, phase);
   }
   if (!capture) return;
   Launcher L(E, K_CAPTURE);

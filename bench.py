@@ -113,6 +113,12 @@ def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
     t0 = time.perf_counter()
     batch.advance(steps)
     t_batch = time.perf_counter() - t0
+    # per-kernel breakdown of a few profiled (eager) batched steps
+    reset()
+    batch.profile(True)
+    batch.advance(3)
+    kern = {k: round(v[0] * v[1] / 3, 4) for k, v in batch.profile_read().items() if v[1]}
+    batch.profile(False)
     # the same solves back to back with the plain solver (graph per solve)
     solvers = [f.solver() for f in fields]
     for s, u in zip(solvers, states):
@@ -133,7 +139,8 @@ def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
             "batched_cell_stage_per_s": cells * 3 * steps / t_batch,
             "plain_cell_stage_per_s": cells * 3 * steps / t_plain,
             "autotune": None if tuned is None else {k: tuned[k] for k in ("bx", "by", "ghosts_per_block",
-                                                                          "ms_per_step")}}
+                                                                          "ms_per_step")},
+            "kernels_ms_per_step": kern}
 
 
 def _gpu_ms(profile):
@@ -231,6 +238,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the differentiation leg (for profiling it); prints its JSON object")
     ap.add_argument("--fd-dirs", type=int, default=32, help="central-difference directions of the adjoint leg")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched-vs-plain leg (for profiling it); prints its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (three launches), the reference for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -251,6 +260,9 @@ def main():
         return
     rh3t = importlib.import_module("3t_rh_amd")
     workloads = importlib.import_module("3t_rh_amd.workloads")
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(rh3t, args.batched_solves, args.batched_steps)}), flush=True)
+        return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(rh3t, fd_dirs=args.fd_dirs)}), flush=True)
         return
