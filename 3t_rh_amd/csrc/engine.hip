@@ -182,6 +182,18 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
   double* const s_v = s_alpha + 30;
   const bool active = h < hg.n;
   const long row = active ? hg.tab[h] : 0;
+  // everything the later phases read, requested before the first barrier
+  const int kind = active ? hg.kind[h] : RH3T_BC_FARFIELD;
+  const double dist = active ? hg.dist[h] : 0.0;
+  const long dst = active ? hg.pad[h] : 0;
+  double bcs[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, ofl[3] = {0.0, 0.0, 0.0};
+  if (active && kind == RH3T_BC_FARFIELD)
+#pragma unroll
+    for (int f = 0; f < 6; ++f) bcs[f] = hg.bcstate[6L * h + f];
+  if (active && kind == RH3T_BC_OUTFLOW)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) ofl[f] = hg.outflow[3L * h + f];
+  const double d5r = (active && lane < 30) ? hg.d5[5L * row + lane / 6] : 0.0;
   const SolveDesc& sd = descs[active ? hg.sid[h] : hg.sid[0]];
   double* S = sd.U[buf];
   const long plane = sd.g.plane;
@@ -225,11 +237,18 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
       while (lane >= kStOff[r + 1]) ++r;
       const int m = (r + 1) * (r + 1), a = lane - kStOff[r];
       const double* srow = Sh + kSOffD[r] + a * m;
+      const double* dat = s_data + kStOff[r];
       double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int b = 0; b < m; ++b) {
-        const double sv = srow[b];
+      // chunks of 5 loads in flight; the sum still runs over b in order
+      for (int b0 = 0; b0 < m; b0 += 5) {
+        double sv[5];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) acc[c] += sv * s_data[(c) * 56 + kStOff[r] + b];
+        for (int t = 0; t < 5; ++t) sv[t] = b0 + t < m ? srow[b0 + t] : 0.0;
+#pragma unroll
+        for (int t = 0; t < 5; ++t)
+          if (b0 + t < m)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] += sv[t] * dat[c * 56 + b0 + t];
       }
 #pragma unroll
       for (int c = 0; c < 6; ++c) s_part[(c) * 56 + lane] = s_data[(c) * 56 + lane] * acc[c];
@@ -249,11 +268,17 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     if (k >= 0) {
       const int m = (r + 1) * (r + 1);
       const double* wv = Wh + kWOffD[r] + k * m;
+      const double* dat = s_data + kStOff[r];
       double vk[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int a = 0; a < m; ++a) {
-        const double wa = wv[a];
+      for (int a0 = 0; a0 < m; a0 += 5) {
+        double wa[5];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) vk[c] += wa * s_data[(c) * 56 + kStOff[r] + a];
+        for (int t = 0; t < 5; ++t) wa[t] = a0 + t < m ? wv[a0 + t] : 0.0;
+#pragma unroll
+        for (int t = 0; t < 5; ++t)
+          if (a0 + t < m)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) vk[c] += wa[t] * dat[c * 56 + a0 + t];
       }
 #pragma unroll
       for (int c = 0; c < 6; ++c) s_vk[((k) * 6 + c) * 5 + r] = vk[c];
@@ -273,7 +298,7 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     const double* p = &s_part[(c) * 56 + kStOff[r]];
     double beta = 0.0;
     for (int a = 0; a < m; ++a) beta += p[a];
-    s_alpha[(c) * 5 + r] = hg.d5[5L * row + r] / weno_pow(ph.eps + beta, ph.weno_q);
+    s_alpha[(c) * 5 + r] = d5r / weno_pow(ph.eps + beta, ph.weno_q);
   }
   __syncthreads();
   if (active && lane < 30) {  // V^(k)_c = sum_r (alpha_r / sum alpha) v_k,r
@@ -290,10 +315,9 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
   if (!active || lane != 0) return;
 
   double uk[5][6];
-  const int kind = hg.kind[h];
   bool ok = true, solved = true;
   if (kind == RH3T_BC_FARFIELD) {  // src/bc1.rs:829-924
-    const double* ui = hg.bcstate + 6L * h;
+    const double* ui = bcs;
     double uloc[6] = {ui[0], ui[1] * nx + ui[2] * ny, -ui[1] * ny + ui[2] * nx, ui[3], ui[4], ui[5]};
     double vinf[6];
     matvec6(E.L, uloc, vinf);
@@ -307,7 +331,7 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
       matvec6(E.R, rhs, uk[k]);
     }
   } else if (kind == RH3T_BC_OUTFLOW) {  // src/bc1.rs:760-828
-    const double* of = hg.outflow + 3L * h;
+    const double* of = ofl;
     matvec6(E.R, (s_v + (0) * 6), uk[0]);
     double rho0 = uk[0][0];
     double un0 = uk[0][1] / rho0, ut0 = uk[0][2] / rho0;
@@ -354,14 +378,12 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
 #pragma unroll
   for (int c = 0; c < 6; ++c) ug[c] = uk[0][c];
   double coef = 1.0;
-  const double dist = hg.dist[h];
   for (int k = 1; k <= 4; ++k) {
     coef *= dist / (double)k;
 #pragma unroll
     for (int c = 0; c < 6; ++c) ug[c] = ug[c] + coef * uk[k][c];
   }
   double out[6] = {ug[0], ug[1] * nx - ug[2] * ny, ug[1] * ny + ug[2] * nx, ug[3], ug[4], ug[5]};
-  long dst = hg.pad[h];
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
     S[f * plane + dst] = out[f];
