@@ -206,6 +206,88 @@ __device__ __forceinline__ void build_eigen(const Phys& ph, double rho3, double 
   }
 }
 
+// The X-direction (DIR 0) eigensystem of build_eigen entry by entry, for
+// callers that need single rows of L or entries of R rather than the whole
+// 6x6 matrices (same expressions, hence the same values).
+struct RoeX {
+  double u, v, w2, ee, ei, er, cs2, cs, h[3], b, ad;
+};
+
+__device__ __forceinline__ void roe_x(const Phys& ph, double rho3, double rho2, double u3, double v3, double ee3,
+                                      double ei3, double er3, double u2, double v2, double ee2, double ei2,
+                                      double er2, RoeX& Q) {
+  double r1 = sqrt(rho3);
+  double r2 = sqrt(rho2);
+  double rs = r1 + r2;
+  Q.u = (u3 * r1 + u2 * r2) / rs;
+  Q.v = (v3 * r1 + v2 * r2) / rs;
+  Q.w2 = sq(Q.u) + sq(Q.v);
+  Q.ee = (ee3 * r1 + ee2 * r2) / rs;
+  Q.ei = (ei3 * r1 + ei2 * r2) / rs;
+  Q.er = (er3 * r1 + er2 * r2) / rs;
+  const double ge = ph.ge, gi = ph.gi, gr = ph.gr, gt = ph.gt;
+  Q.cs2 = ph.gamma_e * ge * Q.ee + ph.gamma_i * gi * Q.ei + ph.gamma_r * gr * Q.er;
+  Q.cs = sqrt(Q.cs2);
+  Q.h[0] = 6.0 * gi * gr * (ph.gamma_i * Q.ei - ph.gamma_r * Q.er) + ph.gamma_e * gt * Q.ee * Q.w2;
+  Q.h[1] = 6.0 * ge * gr * (ph.gamma_r * Q.er - ph.gamma_e * Q.ee) + ph.gamma_i * gt * Q.ei * Q.w2;
+  Q.h[2] = 6.0 * ge * gi * (ph.gamma_e * Q.ee - ph.gamma_i * Q.ei) + ph.gamma_r * gt * Q.er * Q.w2;
+  Q.b = gt * (36.0 * ge * gi * gr + gt * sq(Q.w2)) * Q.cs2;
+  Q.ad = 12.0 * Q.cs2;
+}
+
+// row c of L (X rows: u-c, electron, ion, radiation, transverse, u+c)
+__device__ __forceinline__ void lx_row(const Phys& ph, const RoeX& Q, int c, double row[6]) {
+  const double gt = ph.gt;
+  if (c == 0 || c == 5) {
+    if (c == 0) {
+      row[0] = (gt * Q.w2 + 6.0 * Q.u * Q.cs) / Q.ad;
+      row[1] = (-2.0 * gt * Q.u - 6.0 * Q.cs) / Q.ad;
+    } else {
+      row[0] = (gt * Q.w2 - 6.0 * Q.u * Q.cs) / Q.ad;
+      row[1] = (-2.0 * gt * Q.u + 6.0 * Q.cs) / Q.ad;
+    }
+    row[2] = (-2.0 * gt * Q.v) / Q.ad;
+    row[3] = 6.0 * ph.ge / Q.ad;
+    row[4] = 6.0 * ph.gi / Q.ad;
+    row[5] = 6.0 * ph.gr / Q.ad;
+  } else if (c == 4) {
+    row[0] = -Q.v;
+    row[1] = 0.0;
+    row[2] = 1.0;
+    row[3] = row[4] = row[5] = 0.0;
+  } else {
+    energy_row(c - 1, Q.h[c - 1], Q.u, Q.v, Q.w2, Q.cs2, gt, Q.b, ph.ge, ph.gi, ph.gr, row);
+  }
+}
+
+// entry (a, c) of R (X)
+__device__ __forceinline__ double rx_entry(const Phys& ph, const RoeX& Q, int a, int c) {
+  const double u = Q.u, v = Q.v, cs = Q.cs, w2 = Q.w2, gt = ph.gt;
+  if (a == 0) return c == 4 ? 0.0 : 1.0;
+  if (a == 1) return c == 0 ? u - cs : (c == 5 ? u + cs : (c == 4 ? 0.0 : u));
+  if (a == 2) return c == 4 ? 1.0 : v;
+  const int k = a - 3;  // energy row e, i, r
+  if (c == 4) return v / 3.0;
+  if (c == 0 || c == 5) {
+    const double gam = k == 0 ? ph.gamma_e : (k == 1 ? ph.gamma_i : ph.gamma_r);
+    const double ek = k == 0 ? Q.ee : (k == 1 ? Q.ei : Q.er);
+    const double g_ = gam * ek + w2 / 6.0;
+    const double ucs = u * cs / 3.0;
+    return c == 0 ? g_ - ucs : g_ + ucs;
+  }
+  const int j = c - 1;  // energy column
+  const double g[3] = {ph.ge, ph.gi, ph.gr};
+  if (j == k) return gt * w2 / (6.0 * g[k]);
+  // off-diagonal: rows (e, i, r) x columns (e, i, r) = [., -gr, gi; gr, ., -ge; -gi, ge, .]
+  if (k == 0) return j == 1 ? -ph.gr : ph.gi;
+  if (k == 1) return j == 0 ? ph.gr : -ph.ge;
+  return j == 0 ? -ph.gi : ph.ge;
+}
+
+__device__ __forceinline__ double lamx(const RoeX& Q, int c) {
+  return c == 0 ? Q.u - Q.cs : (c == 5 ? Q.u + Q.cs : Q.u);
+}
+
 // l_dot / r_dot row, src/weno.rs:823-843
 __device__ __forceinline__ double dot6(const double r[6], const double x[6]) {
   return r[0] * x[0] + r[1] * x[1] + r[2] * x[2] + r[3] * x[3] + r[4] * x[4] + r[5] * x[5];

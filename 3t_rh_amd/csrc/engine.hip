@@ -157,63 +157,81 @@ __device__ bool solve6(const double A0[6][6], const double b0[6], double x[6]) {
   return true;
 }
 
+// y = R x and y = L x of the X eigensystem with the entries formed on the fly
+// (matvec6's order).
+__device__ __forceinline__ void rx_apply(const Phys& ph, const RoeX& Q, const double x[6], double y[6]) {
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    double r[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) r[c] = rx_entry(ph, Q, a, c);
+    y[a] = dot6(r, x);
+  }
+}
+
+__device__ __forceinline__ void lx_apply(const Phys& ph, const RoeX& Q, const double x[6], double y[6]) {
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double r[6];
+    lx_row(ph, Q, c, r);
+    y[c] = dot6(r, x);
+  }
+}
+
 // One 64-lane wavefront per heavy ghost (blockDim = 64 x G ghosts).
 // set_ghost_point_value's fast path (src/bc1.rs:663-928) evaluates
 // weno_extrapolation_pre five times (k = 0..4) on identical data; the
 // projected data and the smoothness indicators beta_r do not depend on k, so
 // they are formed once here (same values, a fifth of the work).
+// WALL = false serves the FarField / Outflow ghosts, true the Wall ghosts
+// (whose ILW solves need the full matrices and would otherwise set the
+// register budget of the common case); the engine keeps the two kinds in
+// separate ranges [h0, h0 + count) of one list.
 constexpr int kMaxGhostsPerBlock = 4;
-constexpr int kGhostLds = 336 + 336 + 150 + 30 + 30;  // doubles per ghost
+constexpr int kGhostLds = 336 + 336 + 150 + 30 + 30 + 16;  // doubles per ghost
 
-__global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph,
-                                                     int buf, unsigned phase) {
-  const int w = threadIdx.y;
-  const int h = blockIdx.x * blockDim.y + w;
+template <bool WALL>
+__device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs, const HeavyGhosts& hg,
+                                            const Phys& ph, int buf, unsigned phase, int h0, int count) {
+  // the ghost index is uniform over the wavefront: per-ghost scalars come
+  // through scalar loads
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int hl = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.y) + w;
   const int lane = threadIdx.x;
   // per-ghost LDS (dynamic, kGhostLds doubles per wavefront):
   //   data[6][56]  L . u6 at the 55 E_r points
   //   part[6][56]  data[c][a] * (S_r data[c])[a], a by E_r row
   //   vk[5][6][5]  [k][c][r];  alpha[6][5]  [c][r];  v[5][6]  V^(k) per characteristic
+  //   Roe scalars of the nearest state
   extern __shared__ double s_dyn[];
   double* const s_data = s_dyn + (long)w * kGhostLds;
   double* const s_part = s_data + 336;
   double* const s_vk = s_part + 336;
   double* const s_alpha = s_vk + 150;
   double* const s_v = s_alpha + 30;
-  const bool active = h < hg.n;
-  const long row = active ? hg.tab[h] : 0;
-  // everything the later phases read, requested before the first barrier
-  const int kind = active ? hg.kind[h] : RH3T_BC_FARFIELD;
-  const double dist = active ? hg.dist[h] : 0.0;
-  const long dst = active ? hg.pad[h] : 0;
-  double bcs[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, ofl[3] = {0.0, 0.0, 0.0};
-  if (active && kind == RH3T_BC_FARFIELD)
-#pragma unroll
-    for (int f = 0; f < 6; ++f) bcs[f] = hg.bcstate[6L * h + f];
-  if (active && kind == RH3T_BC_OUTFLOW)
-#pragma unroll
-    for (int f = 0; f < 3; ++f) ofl[f] = hg.outflow[3L * h + f];
-  const double d5r = (active && lane < 30) ? hg.d5[5L * row + lane / 6] : 0.0;
-  const SolveDesc& sd = descs[active ? hg.sid[h] : hg.sid[0]];
+  RoeX* const s_q = (RoeX*)(s_v + 30);
+  const bool active = hl < count;
+  const int h = h0 + (active ? hl : 0);
+  const long row = hg.tab[h];
+  const double d5r = lane < 30 ? hg.d5[5L * row + lane / 6] : 0.0;
+  const SolveDesc& sd = descs[hg.sid[h]];
   double* S = sd.U[buf];
   const long plane = sd.g.plane;
-  const double nx = active ? hg.normal[2L * h] : 1.0, ny = active ? hg.normal[2L * h + 1] : 0.0;
+  const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
   // build_local_plain, src/bc1.rs:294-312: rotate the nearest state to the
   // (n, t) frame; X-direction eigenvectors of the Roe average of that state
   // with itself.
-  Eigen E;
+  RoeX Q;
   {
-    double val[6] = {1.0, 0.0, 0.0, 1.0, 1.0, 1.0};
-    if (active) {
-      long q = hg.nearest[h];
+    double val[6];
+    const long q = hg.nearest[h];
 #pragma unroll
-      for (int f = 0; f < 6; ++f) val[f] = S[f * plane + q];
-    }
+    for (int f = 0; f < 6; ++f) val[f] = S[f * plane + q];
     double s1[6] = {val[0], val[1] * nx + val[2] * ny, -val[1] * ny + val[2] * nx, val[3], val[4], val[5]};
     double d1[D_N];
     derived_from_state(ph, s1, d1, false);
-    build_eigen<0>(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V],
-                   d1[D_EE], d1[D_EI], d1[D_ER], E);
+    roe_x(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI],
+          d1[D_ER], Q);
   }
   if (active && lane < 55) {
     long q = hg.stencil[55L * row + lane];
@@ -222,8 +240,13 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     for (int f = 0; f < 6; ++f) s[f] = S[f * plane + q];
     double u6[6] = {s[0], s[1] * nx + s[2] * ny, -s[1] * ny + s[2] * nx, s[3], s[4], s[5]};
 #pragma unroll
-    for (int c = 0; c < 6; ++c) s_data[(c) * 56 + lane] = dot6(E.L[c], u6);
+    for (int c = 0; c < 6; ++c) {
+      double lr[6];
+      lx_row(ph, Q, c, lr);
+      s_data[(c) * 56 + lane] = dot6(lr, u6);
+    }
   }
+  if (active && lane == 0) *s_q = Q;
   __syncthreads();
   if (active) {
     // One lane per row a of S_r (the 55 E_r points, r = 0..4) applies the row
@@ -314,49 +337,79 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
   __syncthreads();
   if (!active || lane != 0) return;
 
-  double uk[5][6];
+  const RoeX Qs = *s_q;
+  const int kind = hg.kind[h];
+  const double dist = hg.dist[h];
+  // Taylor expansion to the ghost point, sum_k dist^k / k! u^(k)
+  double ug[6];
   bool ok = true, solved = true;
-  if (kind == RH3T_BC_FARFIELD) {  // src/bc1.rs:829-924
-    const double* ui = bcs;
-    double uloc[6] = {ui[0], ui[1] * nx + ui[2] * ny, -ui[1] * ny + ui[2] * nx, ui[3], ui[4], ui[5]};
-    double vinf[6];
-    matvec6(E.L, uloc, vinf);
-    double rhs[6];
+  if (!WALL) {
+    // FarField, src/bc1.rs:829-924: incoming characteristics of u^(0) from
+    // u_inf, of u^(k >= 1) zero.  Outflow, src/bc1.rs:760-828: the u - c
+    // characteristic of u^(1) from the pressure relaxation when incoming.
+    double vinf[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (kind == RH3T_BC_FARFIELD) {
+      double ui[6];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) rhs[m] = E.lam[m] < 0.0 ? vinf[m] : s_v[(0) * 6 + m];
-    matvec6(E.R, rhs, uk[0]);
-    for (int k = 1; k <= 4; ++k) {
-#pragma unroll
-      for (int m = 0; m < 6; ++m) rhs[m] = E.lam[m] < 0.0 ? 0.0 : s_v[(k) * 6 + m];
-      matvec6(E.R, rhs, uk[k]);
+      for (int f = 0; f < 6; ++f) ui[f] = hg.bcstate[6L * h + f];
+      double uloc[6] = {ui[0], ui[1] * nx + ui[2] * ny, -ui[1] * ny + ui[2] * nx, ui[3], ui[4], ui[5]};
+      lx_apply(ph, Qs, uloc, vinf);
     }
-  } else if (kind == RH3T_BC_OUTFLOW) {  // src/bc1.rs:760-828
-    const double* of = ofl;
-    matvec6(E.R, (s_v + (0) * 6), uk[0]);
-    double rho0 = uk[0][0];
-    double un0 = uk[0][1] / rho0, ut0 = uk[0][2] / rho0;
-    double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
-    double ee1 = uk[0][3] / rho0 - w2 / 6.0;
-    double ei1 = uk[0][4] / rho0 - w2 / 6.0;
-    double er1 = uk[0][5] / rho0 - w2 / 6.0;
-    double p0 = rho0 * (ph.ge * ee1 + ph.gi * ei1 + ph.gr * er1);
-    double a2 = ph.gamma_e * ph.ge * ee1 + ph.gamma_i * ph.gi * ei1 + ph.gamma_r * ph.gr * er1;
-    double a = sqrt(a2);
-    double lambda1 = un0 - a;
-    double rhs1[6];
+    bool fix1 = false;
+    double rhs10 = 0.0;
+    double coef = 1.0;
+    for (int k = 0; k <= 4; ++k) {
+      double rhs[6], uk[6];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) rhs1[m] = s_v[(1) * 6 + m];
-    if (lambda1 < 0.0) {
-      double ml = fabs(un0 / a);
-      double l1 = of[1] * (1.0 - sq(ml)) * a / of[2] * (p0 - of[0]);
-      rhs1[0] = l1 / lambda1;
+      for (int m = 0; m < 6; ++m) rhs[m] = s_v[(k) * 6 + m];
+      if (kind == RH3T_BC_FARFIELD) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+          if (lamx(Qs, m) < 0.0) rhs[m] = k == 0 ? vinf[m] : 0.0;
+      } else if (k == 1 && fix1) {
+        rhs[0] = rhs10;
+      }
+      rx_apply(ph, Qs, rhs, uk);
+      if (k == 0) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ug[c] = uk[c];
+        if (kind == RH3T_BC_OUTFLOW) {
+          double rho0 = uk[0];
+          double un0 = uk[1] / rho0, ut0 = uk[2] / rho0;
+          double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
+          double ee1 = uk[3] / rho0 - w2 / 6.0;
+          double ei1 = uk[4] / rho0 - w2 / 6.0;
+          double er1 = uk[5] / rho0 - w2 / 6.0;
+          double p0 = rho0 * (ph.ge * ee1 + ph.gi * ei1 + ph.gr * er1);
+          double a2 = ph.gamma_e * ph.ge * ee1 + ph.gamma_i * ph.gi * ei1 + ph.gamma_r * ph.gr * er1;
+          double a = sqrt(a2);
+          double lambda1 = un0 - a;
+          if (lambda1 < 0.0) {
+            const double p_inf = hg.outflow[3L * h], sigma = hg.outflow[3L * h + 1],
+                         l_dom = hg.outflow[3L * h + 2];
+            double ml = fabs(un0 / a);
+            double l1 = sigma * (1.0 - sq(ml)) * a / l_dom * (p0 - p_inf);
+            fix1 = true;
+            rhs10 = l1 / lambda1;
+          }
+        }
+      } else {
+        coef *= dist / (double)k;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ug[c] = ug[c] + coef * uk[c];
+      }
     }
-    matvec6(E.R, rhs1, uk[1]);
-    for (int k = 2; k <= 4; ++k) matvec6(E.R, (s_v + (k) * 6), uk[k]);
   } else {  // Wall-ILW, src/bc1.rs:691-756
+    double L[6][6], R[6][6], uk[5][6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) lx_row(ph, Qs, c, L[c]);
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) R[a][c] = rx_entry(ph, Qs, a, c);
     double A[6][6], rhs[6];
     for (int i = 0; i < 6; ++i)
-      for (int j = 0; j < 6; ++j) A[i][j] = E.L[i][j];
+      for (int j = 0; j < 6; ++j) A[i][j] = L[i][j];
     for (int c = 0; c < 6; ++c) A[0][c] = c == 1 ? 1.0 : 0.0;
     for (int m = 0; m < 6; ++m) rhs[m] = s_v[(0) * 6 + m];
     rhs[0] = 0.0;
@@ -371,18 +424,18 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     for (int m = 0; m < 6; ++m) rhs[m] = s_v[(1) * 6 + m];
     rhs[0] = 0.0;
     solved = solved && solve6(A, rhs, uk[1]);
-    for (int k = 2; k <= 4; ++k) matvec6(E.R, (s_v + (k) * 6), uk[k]);
-  }
-  // Taylor expansion to the ghost point, rotation back to (x, y)
-  double ug[6];
+    for (int k = 2; k <= 4; ++k) matvec6(R, (s_v + (k) * 6), uk[k]);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) ug[c] = uk[0][c];
-  double coef = 1.0;
-  for (int k = 1; k <= 4; ++k) {
-    coef *= dist / (double)k;
+    for (int c = 0; c < 6; ++c) ug[c] = uk[0][c];
+    double coef = 1.0;
+    for (int k = 1; k <= 4; ++k) {
+      coef *= dist / (double)k;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) ug[c] = ug[c] + coef * uk[k][c];
+      for (int c = 0; c < 6; ++c) ug[c] = ug[c] + coef * uk[k][c];
+    }
   }
+  // rotation back to (x, y)
+  const long dst = hg.pad[h];
   double out[6] = {ug[0], ug[1] * nx - ug[2] * ny, ug[1] * ny + ug[2] * nx, ug[3], ug[4], ug[5]};
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
@@ -393,6 +446,18 @@ __global__ __launch_bounds__(256) void k_ghost_heavy(const SolveDesc* __restrict
     raise_error(sd.sc, phase, RH3T_ERR_SINGULAR, (unsigned)hg.gid[h]);
   else if (!ok)
     raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
+}
+
+// FarField / Outflow: four wavefronts per SIMD (128 VGPRs) to hide the
+// latency of the table reads
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ghost_open(
+    const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
+  ghost_heavy<false>(descs, hg, ph, buf, phase, h0, count);
+}
+
+__global__ __launch_bounds__(256) void k_ghost_wall(const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph,
+                                                    int buf, unsigned phase, int h0, int count) {
+  ghost_heavy<true>(descs, hg, ph, buf, phase, h0, count);
 }
 
 // ----------------------------------------------------------- derived
@@ -801,9 +866,14 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, bool capture) {
   }
   if (E.heavy.n > 0) {
     Launcher L(E, K_GHOST_HEAVY);
-    int g = E.ghosts_per_block;
-    k_ghost_heavy<<<(E.heavy.n + g - 1) / g, dim3(64, g), (size_t)g * kGhostLds * sizeof(double), E.stream>>>(
-        E.d_desc, E.heavy, E.phys, buf, phase);
+    const int g = E.ghosts_per_block, n_open = E.heavy.n_open, n_wall = E.heavy.n - n_open;
+    const size_t lds = (size_t)g * kGhostLds * sizeof(double);
+    if (n_open > 0)
+      k_ghost_open<<<(n_open + g - 1) / g, dim3(64, g), lds, E.stream>>>(E.d_desc, E.heavy, E.phys, buf,
+                                                                                  phase, 0, n_open);
+    if (n_wall > 0)
+      k_ghost_wall<<<(n_wall + g - 1) / g, dim3(64, g), lds, E.stream>>>(E.d_desc, E.heavy, E.phys, buf,
+                                                                                 phase, n_open, n_wall);
   }
   if (!capture) return;
   Launcher L(E, K_CAPTURE);
@@ -1119,6 +1189,30 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   E.h_cheap_normal = c_n;
   E.cheap.n = (int)c_pad.size();
   E.heavy.n = (int)h_pad.size();
+  {
+    // stable partition: FarField / Outflow ghosts, then Wall ghosts
+    std::vector<int> order;
+    for (int pass = 0; pass < 2; ++pass)
+      for (int k = 0; k < E.heavy.n; ++k)
+        if ((h_kind[k] == RH3T_BC_WALL) == (pass == 1)) order.push_back(k);
+    E.heavy.n_open = 0;
+    for (int k = 0; k < E.heavy.n; ++k) E.heavy.n_open += h_kind[k] != RH3T_BC_WALL;
+    auto permute = [&](auto& v, int width) {
+      auto src = v;
+      for (size_t k = 0; k < order.size(); ++k)
+        for (int q = 0; q < width; ++q) v[k * width + q] = src[(size_t)order[k] * width + q];
+    };
+    permute(h_sid, 1);
+    permute(h_tab, 1);
+    permute(h_pad, 1);
+    permute(h_kind, 1);
+    permute(h_gid, 1);
+    permute(h_near, 1);
+    permute(h_n, 2);
+    permute(h_dist, 1);
+    permute(h_state, 6);
+    permute(h_of, 3);
+  }
   long* lp;
   int* ip;
   double* dp;

@@ -43,6 +43,7 @@ struct CheapGhosts {
 
 struct HeavyGhosts {
   int n;
+  int n_open;            // FarField / Outflow ghosts first, Wall ghosts after
   const int* sid;        // solve
   const long* pad;       // destination slot
   const int* kind;       // Wall / Outflow / FarField
