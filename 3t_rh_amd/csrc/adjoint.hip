@@ -6,7 +6,7 @@
 //   OUT = a LAM + b1 X1 + b2 X2 + c (dl/dS)^T LAM
 // in gather form (every output written by exactly one lane, every sum in a
 // fixed order: results are bitwise reproducible):
-//   k_adj_iface<0|1> per interface: the pullback of the characteristic WENO5
+//   k_adj_iface      per interface (x and y): the pullback of the characteristic WENO5
 //                    flux (src/weno.rs:685-802) through the eigensystem,
 //                    the physical fluxes and the Derived values of its 6
 //                    stencil points (36 state cotangents per interface,
@@ -89,17 +89,18 @@ __device__ __forceinline__ void roe_average(const double* S, const double* D, lo
 constexpr int kIfBx = 64;
 constexpr int kCb = 60;
 
+// Interfaces of a direction are numbered i-major, t = i * nj + j, and a
+// workgroup takes 64 consecutive ones (a workgroup may span two lines).
 template <int DIR>
-__global__ __launch_bounds__(kIfBx) void k_adj_iface(AdjDesc A, const EigConst* __restrict__ Kp, Phys ph) {
+__device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __restrict__ Kp, const Phys& ph,
+                                          int tile, double (*s_cb)[kIfBx]) {
   const EigConst& K = *Kp;
-  __shared__ double s_cb[kCb][kIfBx];
   const Geo g = A.g;
   const int ni = DIR == 0 ? g.nx + 1 : g.nx;
   const int nj = DIR == 0 ? g.ny : g.ny + 1;
-  const int tiles_j = (nj + kIfBx - 1) / kIfBx;
-  const int j = (int)(blockIdx.x % tiles_j) * kIfBx + threadIdx.x;
-  const int i = (int)(blockIdx.x / tiles_j);
-  if (i >= ni || j >= nj) return;
+  const long t = (long)tile * kIfBx + threadIdx.x;
+  if (t >= (long)ni * nj) return;
+  const int i = (int)(t / nj), j = (int)(t - (long)i * nj);
   const long fpl = DIR == 0 ? (long)(g.nx + 1) * g.ny : (long)g.nx * (g.ny + 1);
   const long fidx = DIR == 0 ? (long)i * g.ny + j : (long)i * (g.ny + 1) + j;
   double* __restrict__ out = DIR == 0 ? A.XB : A.YB;
@@ -139,6 +140,8 @@ __global__ __launch_bounds__(kIfBx) void k_adj_iface(AdjDesc A, const EigConst* 
     }
     roe_bar_final<DIR>(K, R, B, roeb);
   }
+  // the split-flux cotangents are read back from LDS, not kept in registers
+  reload_barrier();
   // ---- point phase
   double L[6][6], a[6];
 #pragma unroll
@@ -216,6 +219,18 @@ __global__ __launch_bounds__(kIfBx) void k_adj_iface(AdjDesc A, const EigConst* 
 #pragma unroll
     for (int f = 0; f < 6; ++f) out[(q * 6 + f) * fpl + fidx] = sb[f];
   }
+}
+
+// x interfaces in workgroups [0, nbx), y interfaces after them: one launch,
+// so that the two directions share a single partial last round of
+// workgroups
+__global__ __launch_bounds__(kIfBx) void k_adj_iface(AdjDesc A, const EigConst* __restrict__ Kp, Phys ph,
+                                                     int nbx) {
+  __shared__ double s_cb[kCb][kIfBx];
+  if ((int)blockIdx.x < nbx)
+    adj_iface<0>(A, Kp, ph, blockIdx.x, s_cb);
+  else
+    adj_iface<1>(A, Kp, ph, blockIdx.x - nbx, s_cb);
 }
 
 // Weights of N_t in jump(N[0..8)) and jump(N[1..9)) (n_jump_pre,
@@ -707,14 +722,10 @@ void launch_vjp(Engine& E, int buf, const double* LAM, double rscale, double a, 
   A.rscale = rscale;
   A.PG = W.PG;
   {
-    Launcher L(E, K_ADJ_IFACE_X);
-    const int tiles = ((g.ny + kIfBx - 1) / kIfBx) * (g.nx + 1);
-    k_adj_iface<0><<<tiles, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys);
-  }
-  {
-    Launcher L(E, K_ADJ_IFACE_Y);
-    const int tiles = ((g.ny + 1 + kIfBx - 1) / kIfBx) * g.nx;
-    k_adj_iface<1><<<tiles, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys);
+    Launcher L(E, K_ADJ_IFACE);
+    const int nbx = (int)(((long)(g.nx + 1) * g.ny + kIfBx - 1) / kIfBx);
+    const int nby = (int)(((long)g.nx * (g.ny + 1) + kIfBx - 1) / kIfBx);
+    k_adj_iface<<<nbx + nby, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys, nbx);
   }
   {
     Launcher L(E, K_ADJ_GATHER);
@@ -788,14 +799,26 @@ void launch_rk3_reverse(Engine& E, double dt, const double* lam, double* s2b, do
 
 }  // namespace
 
+// Buffers of a checkpointed rollout with checkpoint interval K.  Step n
+// (segment n / K, slot m = n % K) reads its state U^n from base(n) and
+// writes its stage states to s1[m], s2[m] and U^{n+1} to base(n + 1), where
+// base(n) is the checkpoint ckpt[n / K] when m == 0, mid[m] otherwise, and
+// base(N) is the context's own state.  Every step has its own device
+// descriptor with these buffers as U[0..3], so neither the forward nor the
+// recompute copies a state: the forward leaves the last segment on the tape
+// and the backward recomputes only the segments before it.
 struct rh3t_adjoint {
   rh3t_ctx* ctx = nullptr;
   int every = 5;
   int nsteps = 0;
   std::vector<double> dts;
-  std::vector<double*> ckpt;  // U^{kK}, k = 0, 1, ...
-  std::vector<double*> tape;  // 3 stage states per step of one segment
-  double* final_state = nullptr;
+  std::vector<double*> ckpt;     // U^{kK}, k = 0, 1, ...
+  std::vector<double*> s1, s2;   // stage states of slot m
+  std::vector<double*> mid;      // U^{kK + m}, m = 1 .. K-1 (mid[0] unused)
+  std::vector<SolveDesc> h_steps;
+  SolveDesc* d_steps = nullptr;
+  int steps_cap = 0;
+  int tape_seg = -1;             // segment whose stage states the tape holds
   std::vector<void*> allocs;
 };
 
@@ -810,9 +833,32 @@ int adj_alloc(rh3t_adjoint* A, double** p, size_t n) {
     return RH3T_ERR_HIP;
   }
   A->allocs.push_back(q);
+  // cells outside the fluid and its ghosts are never written by a step
+  e = hipMemset(q, 0, n * sizeof(double));
+  if (e != hipSuccess) {
+    std::snprintf(E.message, sizeof(E.message), "hipMemset: %s", hipGetErrorString(e));
+    return RH3T_ERR_HIP;
+  }
   *p = (double*)q;
   return RH3T_OK;
 }
+
+// Launches of step n run on that step's buffers while this is in scope.
+struct OnStep {
+  Engine& E;
+  SolveDesc* d0;
+  SolveDesc h0;
+  OnStep(Engine& e, const rh3t_adjoint* A, int n) : E(e), d0(e.d_desc), h0(e.h_desc[0]) {
+    E.d_desc = A->d_steps + n;
+    E.h_desc[0] = A->h_steps[n];
+  }
+  ~OnStep() {
+    E.d_desc = d0;
+    E.h_desc[0] = h0;
+  }
+};
+
+const StageSpec kStage1{1, 0, 0, 1, 3}, kStage2{2, 0, 1, 2, 5}, kStage3{3, 0, 2, 3, 7};
 
 }  // namespace
 
@@ -863,15 +909,17 @@ int rh3t_adjoint_create(rh3t_ctx* ctx, int32_t checkpoint_every, rh3t_adjoint** 
   A->ctx = ctx;
   A->every = checkpoint_every;
   const size_t plane6 = 6 * (size_t)E.h_desc[0].g.plane;
-  A->tape.assign(3 * (size_t)checkpoint_every, nullptr);
-  for (auto& t : A->tape)
-    if ((st = adj_alloc(A, &t, plane6))) {
+  A->s1.assign(checkpoint_every, nullptr);
+  A->s2.assign(checkpoint_every, nullptr);
+  A->mid.assign(checkpoint_every, nullptr);
+  for (int m = 0; m < checkpoint_every; ++m) {
+    st = adj_alloc(A, &A->s1[m], plane6);
+    if (!st) st = adj_alloc(A, &A->s2[m], plane6);
+    if (!st && m > 0) st = adj_alloc(A, &A->mid[m], plane6);
+    if (st) {
       rh3t_adjoint_destroy(A);
       return st;
     }
-  if ((st = adj_alloc(A, &A->final_state, plane6))) {
-    rh3t_adjoint_destroy(A);
-    return st;
   }
   *out = A;
   return RH3T_OK;
@@ -892,26 +940,52 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
   Engine& E = A->ctx->e;
   int st = begin_adjoint_call(E);
   if (st) return st;
-  const SolveDesc& d = E.h_desc[0];
-  const size_t bytes = 6 * (size_t)d.g.plane * sizeof(double);
-  const int nck = (nsteps + A->every - 1) / A->every;
+  const SolveDesc d = E.h_desc[0];
+  const size_t plane6 = 6 * (size_t)d.g.plane;
+  const int K = A->every;
+  const int nck = (nsteps + K - 1) / K;
   while ((int)A->ckpt.size() < nck) {
     double* p;
-    if ((st = adj_alloc(A, &p, bytes / sizeof(double)))) return st;
+    if ((st = adj_alloc(A, &p, plane6))) return st;
     A->ckpt.push_back(p);
+  }
+  if (A->steps_cap < nsteps) {
+    void* q = nullptr;
+    HIP_TRY(hipMalloc(&q, sizeof(SolveDesc) * nsteps));
+    A->allocs.push_back(q);
+    A->d_steps = (SolveDesc*)q;
+    A->steps_cap = nsteps;
+  }
+  auto base = [&](int n) { return n == nsteps ? d.U[0] : (n % K == 0 ? A->ckpt[n / K] : A->mid[n % K]); };
+  std::vector<SolveDesc> steps(nsteps, d);
+  for (int n = 0; n < nsteps; ++n) {
+    SolveDesc& sd = steps[n];
+    sd.U[0] = base(n);
+    sd.U[1] = A->s1[n % K];
+    sd.U[2] = A->s2[n % K];
+    sd.U[3] = base(n + 1);
+  }
+  if (steps.size() != A->h_steps.size() ||
+      std::memcmp(steps.data(), A->h_steps.data(), sizeof(SolveDesc) * nsteps) != 0) {
+    HIP_TRY(hipStreamSynchronize(E.stream));
+    HIP_TRY(hipMemcpy(A->d_steps, steps.data(), sizeof(SolveDesc) * nsteps, hipMemcpyHostToDevice));
+    A->h_steps.swap(steps);
   }
   if ((st = ensure_dts(E, nsteps))) return st;
   A->dts.assign(dt, dt + nsteps);
   A->nsteps = nsteps;
+  A->tape_seg = -1;
   HIP_TRY(hipMemcpyAsync(E.adj->dts, dt, nsteps * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  HIP_TRY(hipMemcpyAsync(A->ckpt[0], d.U[0], plane6 * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
   for (int n = 0; n < nsteps; ++n) {
-    if (n % A->every == 0)
-      HIP_TRY(hipMemcpyAsync(A->ckpt[n / A->every], d.U[0], bytes, hipMemcpyDeviceToDevice, E.stream));
     k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, E.adj->dts, n, 1);
-    launch_rk3(E);
+    OnStep on(E, A, n);
+    launch_stage(E, 0, kStage1);
+    launch_stage(E, 1, kStage2);
+    launch_stage(E, 2, kStage3);
   }
-  HIP_TRY(hipMemcpyAsync(A->final_state, d.U[0], bytes, hipMemcpyDeviceToDevice, E.stream));
   HIP_TRY(hipGetLastError());
+  A->tape_seg = nck - 1;
   E.dt_partials_valid = true;
   return check_errors(E);
 }
@@ -923,8 +997,6 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
   int st = begin_adjoint_call(E);
   if (st) return st;
   AdjWork& W = *E.adj;
-  const SolveDesc& d = E.h_desc[0];
-  const size_t bytes = 6 * (size_t)d.g.plane * sizeof(double);
   double* lam = W.lam[0];
   double* nxt = W.lam[3];
   if ((st = upload_cot(E, lam_final, lam))) return st;
@@ -932,35 +1004,29 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
   const int nseg = (A->nsteps + A->every - 1) / A->every;
   for (int k = nseg - 1; k >= 0; --k) {
     const int n0 = k * A->every, n1 = std::min(A->nsteps, n0 + A->every);
-    // recompute the segment from its checkpoint, taping the stage states
-    HIP_TRY(hipMemcpyAsync(d.U[0], A->ckpt[k], bytes, hipMemcpyDeviceToDevice, E.stream));
-    for (int n = n0; n < n1; ++n) {
-      const int m = n - n0;
-      k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, W.dts, n, 0);
-      HIP_TRY(hipMemcpyAsync(A->tape[3 * m], d.U[0], bytes, hipMemcpyDeviceToDevice, E.stream));
-      launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3});
-      HIP_TRY(hipMemcpyAsync(A->tape[3 * m + 1], d.U[1], bytes, hipMemcpyDeviceToDevice, E.stream));
-      launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5});
-      HIP_TRY(hipMemcpyAsync(A->tape[3 * m + 2], d.U[2], bytes, hipMemcpyDeviceToDevice, E.stream));
-      if (n + 1 < n1) launch_stage(E, 2, StageSpec{3, 0, 2, 0, 7});
+    if (A->tape_seg != k) {
+      // recompute the segment from its checkpoint onto the tape (U^{n1} is
+      // the next checkpoint already)
+      A->tape_seg = -1;
+      for (int n = n0; n < n1; ++n) {
+        k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, W.dts, n, 0);
+        OnStep on(E, A, n);
+        launch_stage(E, 0, kStage1);
+        launch_stage(E, 1, kStage2);
+        if (n + 1 < n1) launch_stage(E, 2, kStage3);
+      }
+      A->tape_seg = k;
     }
     // reverse sweep over the segment
     for (int n = n1 - 1; n >= n0; --n) {
-      const int m = n - n0;
-      if (n + 1 < n1) {
-        for (int b = 0; b < 3; ++b)
-          HIP_TRY(hipMemcpyAsync(d.U[b], A->tape[3 * m + b], bytes, hipMemcpyDeviceToDevice, E.stream));
-      }
+      OnStep on(E, A, n);
       launch_rk3_reverse(E, A->dts[n], lam, W.lam[1], W.lam[2], nxt);
       std::swap(lam, nxt);
     }
   }
   HIP_TRY(hipGetLastError());
   if ((st = download_cot(E, lam, lam_initial, pgrad))) return st;
-  // leave the context at the forward's final state (the recompute's stage 3
-  // overwrote the dt minima)
-  HIP_TRY(hipMemcpyAsync(d.U[0], A->final_state, bytes, hipMemcpyDeviceToDevice, E.stream));
-  HIP_TRY(hipStreamSynchronize(E.stream));
+  // the recompute's stage 3 overwrote the dt minima
   E.dt_partials_valid = false;
   return check_errors(E);
 }

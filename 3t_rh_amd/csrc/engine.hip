@@ -196,7 +196,7 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
   // the ghost index is uniform over the wavefront: per-ghost scalars come
   // through scalar loads
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int hl = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.y) + w;
+  const int hl = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * blockDim.y) + w;
   const int lane = threadIdx.x;
   // per-ghost LDS (dynamic, kGhostLds doubles per wavefront):
   //   data[6][56]  L . u6 at the 55 E_r points
@@ -812,8 +812,8 @@ __global__ __launch_bounds__(256) void k_gather_ghosts(SolveDesc sd, int n, doub
 
 namespace rh3t_eng {
 const char* const kKernelNames[K_COUNT] = {"ghost_cheap", "ghost_heavy", "derived", "flux_x", "flux_y",
-                                           "assemble", "dt_partial", "dt_final", "capture", "adj_iface_x",
-                                           "adj_iface_y", "adj_gather", "adj_ghost", "adj_param"};
+                                           "assemble", "dt_partial", "dt_final", "capture", "adj_iface",
+                                           "adj_gather", "adj_ghost", "adj_param"};
 }  // namespace rh3t_eng
 
 namespace {
@@ -1171,6 +1171,7 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
       if ((st = dev_alloc(E, &d.U[b], 6 * (size_t)g.plane))) return st;
       HIP_TRY(hipMemset(d.U[b], 0, 6 * (size_t)g.plane * sizeof(double)));
     }
+    d.U[3] = d.U[0];
     if ((st = dev_alloc(E, &d.D, (size_t)D_N * g.plane))) return st;
     HIP_TRY(hipMemset(d.D, 0, (size_t)D_N * g.plane * sizeof(double)));
     if ((st = dev_alloc(E, &d.FX, 6 * (size_t)(g.nx + 1) * g.ny))) return st;
@@ -1190,11 +1191,17 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   E.cheap.n = (int)c_pad.size();
   E.heavy.n = (int)h_pad.size();
   {
-    // stable partition: FarField / Outflow ghosts, then Wall ghosts
-    std::vector<int> order;
-    for (int pass = 0; pass < 2; ++pass)
-      for (int k = 0; k < E.heavy.n; ++k)
-        if ((h_kind[k] == RH3T_BC_WALL) == (pass == 1)) order.push_back(k);
+    // FarField / Outflow ghosts, then Wall ghosts; within each, ghosts that
+    // share a table row (the same ghost of clones of one problem) next to
+    // each other, so that with xcd_tile's mapping one XCD's L2 serves the
+    // row to all of them
+    std::vector<int> order(E.heavy.n);
+    for (int k = 0; k < E.heavy.n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      const bool wa = h_kind[a] == RH3T_BC_WALL, wb = h_kind[b] == RH3T_BC_WALL;
+      if (wa != wb) return wb;
+      return h_tab[a] < h_tab[b];
+    });
     E.heavy.n_open = 0;
     for (int k = 0; k < E.heavy.n; ++k) E.heavy.n_open += h_kind[k] != RH3T_BC_WALL;
     auto permute = [&](auto& v, int width) {

@@ -20,7 +20,8 @@ using namespace rh3t_dev;
 
 struct SolveDesc {
   Geo g;
-  double* U[3];
+  double* U[4];         // U0, U1, U2; U[3]: stage-3 destination (U0 itself
+                        // except where a step writes a new buffer)
   double* D;
   double* FX;
   double* FY;
@@ -68,7 +69,7 @@ struct StageSpec {
 
 enum KernelId {
   K_GHOST_CHEAP, K_GHOST_HEAVY, K_DERIVED, K_FLUX_X, K_FLUX_Y, K_ASSEMBLE, K_DT_PARTIAL, K_DT_FINAL, K_CAPTURE,
-  K_ADJ_IFACE_X, K_ADJ_IFACE_Y, K_ADJ_GATHER, K_ADJ_GHOST, K_ADJ_PARAM, K_COUNT
+  K_ADJ_IFACE, K_ADJ_GATHER, K_ADJ_GHOST, K_ADJ_PARAM, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 

@@ -327,7 +327,10 @@ void rh3t_adjoint_destroy(rh3t_adjoint* adj);
  * advances by each dt. */
 int rh3t_adjoint_forward(rh3t_adjoint* adj, int32_t nsteps, const double* dt);
 /* lam_initial = (d u^N / d u^0)^T lam_final and the parameter cotangent,
- * for the last forward; the context is left at u^N. */
+ * for the last forward.  The context's state is not touched (it stays at
+ * u^N unless the caller changed it); the stage states of the forward's last
+ * checkpoint segment are kept, so the backward recomputes only the segments
+ * before it. */
 int rh3t_adjoint_backward(rh3t_adjoint* adj, const double* lam_final,
                           double* lam_initial, double* pgrad);
 
