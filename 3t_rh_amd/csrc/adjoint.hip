@@ -94,7 +94,6 @@ constexpr int kCb = 60;
 template <int DIR>
 __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __restrict__ Kp, const Phys& ph,
                                           int tile, double (*s_cb)[kIfBx]) {
-  const EigConst& K = *Kp;
   const Geo g = A.g;
   const int ni = DIR == 0 ? g.nx + 1 : g.nx;
   const int nj = DIR == 0 ? g.ny : g.ny + 1;
@@ -118,7 +117,7 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
   RoeAvg ra;
   roe_average(S, D, P, c0, c0 - step, ra);
   RoeV R;
-  roe_values<DIR>(K, ra.qr[0], ra.qr[1], ra.qr + 2, R);
+  roe_values<DIR>(*fresh(Kp), ra.qr[0], ra.qr[1], ra.qr + 2, R);
   double roeb[5];
   {
     // flux F[i] enters r(i) with +1/ds and r(i-1) with -1/ds (src/main.rs:266-270)
@@ -134,11 +133,11 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       double cb[10];
-      characteristic_step<DIR>(c, K, ph.eps, R, U, F, ob, cb, B);
+      characteristic_step<DIR>(c, *fresh(Kp), ph.eps, R, U, F, ob, cb, B);
 #pragma unroll
       for (int k = 0; k < 10; ++k) cbs[(c * 10 + k) * kIfBx] = cb[k];
     }
-    roe_bar_final<DIR>(K, R, B, roeb);
+    roe_bar_final<DIR>(*fresh(Kp), R, B, roeb);
   }
   // the split-flux cotangents are read back from LDS, not kept in registers
   reload_barrier();
@@ -147,7 +146,7 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     double col[6], lam;
-    row_col<DIR>(K, R, c, L[c], col, &lam);
+    row_col<DIR>(*fresh(Kp), R, c, L[c], col, &lam);
     a[c] = fabs(lam);
   }
   double b3[5], b2[5], r1b = 0.0, r2b = 0.0;
@@ -258,30 +257,69 @@ struct GatherTile {
   uint8_t pt[kGr][kGc];
 };
 
-// One line (DIR 0: x, DIR 1: y) of cell-term pullbacks into the point at
-// tile position (ty, tx), grid index (i, j): the noncon terms of the 9 cells
-// whose stencils hold it (pressure and velocity cotangents) and the
-// diffusion fluxes of the 7 cells around it (temperature cotangents, T_r^4
-// for the radiation field).
+// The tile planes along a line through a point, at offset o: from the LDS
+// tile (TileLine) or straight from the padded planes (PlaneLine, same values).
 template <int DIR>
-__device__ __forceinline__ void line_gather(const GatherTile& T, const Geo& g, const Phys& ph, int ty, int tx,
-                                            int i, int j, double db[D_N], double tb[3]) {
+struct TileLine {
+  const GatherTile& T;
+  int ty, tx;
+  __device__ double d(int plane, int o) const {
+    return T.d[plane][ty + kGh + (DIR == 0 ? o : 0)][tx + kGh + (DIR == 0 ? 0 : o)];
+  }
+  __device__ int pt(int o) const { return T.pt[ty + kGh + (DIR == 0 ? o : 0)][tx + kGh + (DIR == 0 ? 0 : o)]; }
+};
+
+template <int DIR>
+struct PlaneLine {
+  const AdjDesc& A;
+  int ip, jp;
+  __device__ bool at(int o, long& q) const {
+    const int a = ip + (DIR == 0 ? o : 0), b = jp + (DIR == 0 ? 0 : o);
+    q = (long)a * A.g.PY + b;
+    return a >= 0 && a < A.g.nx + 8 && b >= 0 && b < A.g.PY;
+  }
+  __device__ double d(int plane, int o) const {
+    long q;
+    if (!at(o, q)) return 0.0;
+    const long P = A.g.plane;
+    switch (plane) {
+      case T_PE: return A.D[D_PE * P + q];
+      case T_PI: return A.D[D_PI * P + q];
+      case T_PR: return A.D[D_PR * P + q];
+      case T_U: return A.D[D_U * P + q];
+      case T_V: return A.D[D_V * P + q];
+      case T_R3: return A.rscale * A.LAM[3 * P + q];
+      case T_R4: return A.rscale * A.LAM[4 * P + q];
+      default: return A.rscale * A.LAM[5 * P + q];
+    }
+  }
+  __device__ int pt(int o) const {
+    long q;
+    return at(o, q) ? A.pt[q] : 0;
+  }
+};
+
+// One line (DIR 0: x, DIR 1: y) of cell-term pullbacks into the point with
+// grid index (i, j): the noncon terms of the 9 cells whose stencils hold it
+// (pressure and velocity cotangents) and the diffusion fluxes of the 7 cells
+// around it (temperature cotangents, T_r^4 for the radiation field).
+template <int DIR, class Line>
+__device__ __forceinline__ void line_gather(const Line& T, const Geo& g, const Phys& ph, int i, int j,
+                                            double db[D_N], double tb[3]) {
   const double ds = DIR == 0 ? g.dx : g.dy;
   const double ids = 1.0 / ds;
   const int VP = DIR == 0 ? D_U : D_V;
   const int along = DIR == 0 ? i : j, across = DIR == 0 ? j : i;
   const int n_along = DIR == 0 ? g.nx : g.ny, n_across = DIR == 0 ? g.ny : g.nx;
   if (across < 0 || across >= n_across) return;
-  auto row = [&](int o) { return ty + kGh + (DIR == 0 ? o : 0); };
-  auto col = [&](int o) { return tx + kGh + (DIR == 0 ? 0 : o); };
   // p_e, p_i, p_r and the velocity at offsets -5..5 (0 outside the padded
   // planes; such points are never in the stencil of a fluid cell)
   double Pv[3][11], V[11];
 #pragma unroll
   for (int o = -5; o <= 5; ++o) {
 #pragma unroll
-    for (int m = 0; m < 3; ++m) Pv[m][o + 5] = T.d[T_PE + m][row(o)][col(o)];
-    V[o + 5] = T.d[DIR == 0 ? T_U : T_V][row(o)][col(o)];
+    for (int m = 0; m < 3; ++m) Pv[m][o + 5] = T.d(T_PE + m, o);
+    V[o + 5] = T.d(DIR == 0 ? T_U : T_V, o);
   }
   // rho_m(c) = sum_e rb_e(c) M[e][m], M = [[2,-1,-1],[-1,2,-1],[-1,-1,2]]
   double Pb[3] = {0.0, 0.0, 0.0};
@@ -292,9 +330,9 @@ __device__ __forceinline__ void line_gather(const GatherTile& T, const Geo& g, c
 #pragma unroll
   for (int o = -4; o <= 4; ++o) {
     const int a = along + o;
-    const bool fl = a >= 0 && a < n_along && T.pt[row(o)][col(o)] == 1;
+    const bool fl = a >= 0 && a < n_along && T.pt(o) == 1;
     if (!fl) continue;
-    const double r0 = T.d[T_R3][row(o)][col(o)], r1 = T.d[T_R4][row(o)][col(o)], r2 = T.d[T_R5][row(o)][col(o)];
+    const double r0 = T.d(T_R3, o), r1 = T.d(T_R4, o), r2 = T.d(T_R5, o);
     const double sr = r0 + r1 + r2;
     const double rh[3] = {3.0 * r0 - sr, 3.0 * r1 - sr, 3.0 * r2 - sr};
     const int t = 4 - o;  // position of the point in the stencil of cell o
@@ -374,12 +412,95 @@ __device__ __forceinline__ void laplacian(const AdjDesc& A, long c, double lap[3
   }
 }
 
+// The gather of one point (ip, jp) of kind 1 (fluid: OUT) or 2 (ghost: its
+// state cotangent SB); lx / ly: its x and y lines.  pg: the point's
+// parameter cotangents.
+template <class LX, class LY>
+__device__ __forceinline__ void gather_point(const AdjDesc& A, const Phys& ph, int ip, int jp, int kind,
+                                             const LX& lx, const LY& ly, double pg[kNParam]) {
+  const Geo& g = A.g;
+  const long P = g.plane;
+  const long p = (long)ip * g.PY + jp;
+  const int i = ip - 4, j = jp - 4;
+  double sb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  // the 6 x and 6 y interfaces whose stencils hold p
+  const long fxpl = (long)(g.nx + 1) * g.ny, fypl = (long)g.nx * (g.ny + 1);
+  if (j >= 0 && j < g.ny) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int ix = i + 3 - q;
+      if (ix < 0 || ix > g.nx) continue;
+      const long fi = (long)ix * g.ny + j;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) sb[f] += A.XB[(q * 6 + f) * fxpl + fi];
+    }
+  }
+  if (i >= 0 && i < g.nx) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const int jy = j + 3 - q;
+      if (jy < 0 || jy > g.ny) continue;
+      const long fi = (long)i * (g.ny + 1) + jy;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) sb[f] += A.YB[(q * 6 + f) * fypl + fi];
+    }
+  }
+  double db[D_N];
+#pragma unroll
+  for (int k = 0; k < D_N; ++k) db[k] = 0.0;
+  double tb[3] = {0.0, 0.0, 0.0};
+  line_gather<0>(lx, g, ph, i, j, db, tb);
+  line_gather<1>(ly, g, ph, i, j, db, tb);
+  if (ph.diffusion) {
+    db[D_TE] += tb[0];
+    db[D_TI] += tb[1];
+    db[D_TR] += tb[2] * 4.0 * cube(A.D[D_TR * P + p]);
+  }
+  if (kind == 1) {
+    const double rb3 = lx.d(T_R3, 0), rb4 = lx.d(T_R4, 0), rb5 = lx.d(T_R5, 0);
+    const double te = A.D[D_TE * P + p], ti = A.D[D_TI * P + p], tr = A.D[D_TR * P + p];
+    if (ph.source) {  // src/source.rs:7-21
+      const double te3 = 4.0 * cube(te), tr3 = 4.0 * cube(tr);
+      db[D_TE] += rb3 * (-ph.omega_ei - ph.omega_er * te3) + rb4 * ph.omega_ei + rb5 * ph.omega_er * te3;
+      db[D_TI] += (rb3 - rb4) * ph.omega_ei;
+      db[D_TR] += (rb3 - rb5) * ph.omega_er * tr3;
+    }
+    pg[0] = (te - ti) * (rb4 - rb3);
+    pg[1] = (pow4(te) - pow4(tr)) * (rb5 - rb3);
+    double lap[3] = {0.0, 0.0, 0.0};
+    laplacian<0>(A, p, lap);
+    laplacian<1>(A, p, lap);
+    pg[2] = rb3 * lap[0];
+    pg[3] = rb4 * lap[1];
+    pg[4] = rb5 * lap[2];
+  }
+  double s[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) s[f] = A.S[f * P + p];
+  derived_vjp(ph, s, db, sb);
+  if (kind == 1) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      double v = A.a * A.LAM[f * P + p] + sb[f];
+      if (A.X1) v += A.b1 * A.X1[f * P + p];
+      if (A.X2) v += A.b2 * A.X2[f * P + p];
+      A.OUT[f * P + p] = v;
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) A.SB[f * P + p] = sb[f];
+  }
+}
+
+// Tiles of kGy x kGx points over the fluid box i in [0, nx), j in [0, ny)
+// (its fluid cells and any ghosts inside it); the ghosts around the box go
+// to k_adj_gather_ghosts.
 __global__ __launch_bounds__(256) void k_adj_gather(AdjDesc A, Phys ph) {
   __shared__ GatherTile T;
   const Geo g = A.g;
   const long P = g.plane;
-  const int tiles_c = (g.PY + kGx - 1) / kGx;
-  const int ip0 = (int)(blockIdx.x / tiles_c) * kGy, jp0 = (int)(blockIdx.x % tiles_c) * kGx;
+  const int tiles_c = (g.ny + kGx - 1) / kGx;
+  const int ip0 = 4 + (int)(blockIdx.x / tiles_c) * kGy, jp0 = 4 + (int)(blockIdx.x % tiles_c) * kGx;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * kGx + tx;
   for (int idx = tid; idx < kGr * kGc; idx += kGx * kGy) {
     const int r = idx / kGc, c = idx % kGc;
@@ -398,81 +519,10 @@ __global__ __launch_bounds__(256) void k_adj_gather(AdjDesc A, Phys ph) {
   }
   __syncthreads();
   const int ip = ip0 + ty, jp = jp0 + tx;
-  const long p = (long)ip * g.PY + jp;
-  const int kind = (ip < g.nx + 8 && jp < g.PY) ? T.pt[ty + kGh][tx + kGh] : 0;
+  const int kind = (ip < g.nx + 4 && jp < g.ny + 4) ? T.pt[ty + kGh][tx + kGh] : 0;
   double pg[kNParam] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (kind != 0) {
-    const int i = ip - 4, j = jp - 4;
-    double sb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    // the 6 x and 6 y interfaces whose stencils hold p
-    const long fxpl = (long)(g.nx + 1) * g.ny, fypl = (long)g.nx * (g.ny + 1);
-    if (j >= 0 && j < g.ny) {
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const int ix = i + 3 - q;
-        if (ix < 0 || ix > g.nx) continue;
-        const long fi = (long)ix * g.ny + j;
-#pragma unroll
-        for (int f = 0; f < 6; ++f) sb[f] += A.XB[(q * 6 + f) * fxpl + fi];
-      }
-    }
-    if (i >= 0 && i < g.nx) {
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        const int jy = j + 3 - q;
-        if (jy < 0 || jy > g.ny) continue;
-        const long fi = (long)i * (g.ny + 1) + jy;
-#pragma unroll
-        for (int f = 0; f < 6; ++f) sb[f] += A.YB[(q * 6 + f) * fypl + fi];
-      }
-    }
-    double db[D_N];
-#pragma unroll
-    for (int k = 0; k < D_N; ++k) db[k] = 0.0;
-    double tb[3] = {0.0, 0.0, 0.0};
-    line_gather<0>(T, g, ph, ty, tx, i, j, db, tb);
-    line_gather<1>(T, g, ph, ty, tx, i, j, db, tb);
-    if (ph.diffusion) {
-      db[D_TE] += tb[0];
-      db[D_TI] += tb[1];
-      db[D_TR] += tb[2] * 4.0 * cube(A.D[D_TR * P + p]);
-    }
-    if (kind == 1) {
-      const double rb3 = T.d[T_R3][ty + kGh][tx + kGh], rb4 = T.d[T_R4][ty + kGh][tx + kGh];
-      const double rb5 = T.d[T_R5][ty + kGh][tx + kGh];
-      const double te = A.D[D_TE * P + p], ti = A.D[D_TI * P + p], tr = A.D[D_TR * P + p];
-      if (ph.source) {  // src/source.rs:7-21
-        const double te3 = 4.0 * cube(te), tr3 = 4.0 * cube(tr);
-        db[D_TE] += rb3 * (-ph.omega_ei - ph.omega_er * te3) + rb4 * ph.omega_ei + rb5 * ph.omega_er * te3;
-        db[D_TI] += (rb3 - rb4) * ph.omega_ei;
-        db[D_TR] += (rb3 - rb5) * ph.omega_er * tr3;
-      }
-      pg[0] = (te - ti) * (rb4 - rb3);
-      pg[1] = (pow4(te) - pow4(tr)) * (rb5 - rb3);
-      double lap[3] = {0.0, 0.0, 0.0};
-      laplacian<0>(A, p, lap);
-      laplacian<1>(A, p, lap);
-      pg[2] = rb3 * lap[0];
-      pg[3] = rb4 * lap[1];
-      pg[4] = rb5 * lap[2];
-    }
-    double s[6];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) s[f] = A.S[f * P + p];
-    derived_vjp(ph, s, db, sb);
-    if (kind == 1) {
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        double v = A.a * A.LAM[f * P + p] + sb[f];
-        if (A.X1) v += A.b1 * A.X1[f * P + p];
-        if (A.X2) v += A.b2 * A.X2[f * P + p];
-        A.OUT[f * P + p] = v;
-      }
-    } else {
-#pragma unroll
-      for (int f = 0; f < 6; ++f) A.SB[f * P + p] = sb[f];
-    }
-  }
+  if (kind != 0)
+    gather_point(A, ph, ip, jp, kind, TileLine<0>{T, ty, tx}, TileLine<1>{T, ty, tx}, pg);
   // block partial sums of the parameter cotangents (the tile's LDS is reused)
   __syncthreads();
   double(*red)[kGx * kGy] = reinterpret_cast<double(*)[kGx * kGy]>(&T.d[0][0][0]);
@@ -488,6 +538,16 @@ __global__ __launch_bounds__(256) void k_adj_gather(AdjDesc A, Phys ph) {
   // each block owns its slots: partial sums accumulate over the VJPs of a
   // call in launch order
   if (tid < kNParam) A.PG[blockIdx.x * kNParam + tid] += red[tid][0];
+}
+
+// The ghosts outside the fluid box, one lane each (pads listed by the host).
+__global__ __launch_bounds__(64) void k_adj_gather_ghosts(AdjDesc A, Phys ph, int n, const long* __restrict__ pads) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const long p = pads[k];
+  const int ip = (int)(p / A.g.PY), jp = (int)(p % A.g.PY);
+  double pg[kNParam] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  gather_point(A, ph, ip, jp, 2, PlaneLine<0>{A, ip, jp}, PlaneLine<1>{A, ip, jp}, pg);
 }
 
 // One lane per source cell: adds the cotangents of the cheap ghosts copied
@@ -590,6 +650,8 @@ struct AdjWork {
   double* SB = nullptr;
   double* PG = nullptr;
   int pg_blocks = 0;
+  int n_out = 0;           // ghosts outside the fluid box
+  long* out_pads = nullptr;
   double* pacc = nullptr;  // kNParam
   double* aos = nullptr;   // staging
   int nrows = 0;
@@ -672,7 +734,15 @@ int init_adjoint(Engine& E) {
   HIP_TRY(hipMemset(W->SB, 0, plane6 * sizeof(double)));
   if ((st = dev_alloc(E, &W->XB, 36 * (size_t)(g.nx + 1) * g.ny))) return st;
   if ((st = dev_alloc(E, &W->YB, 36 * (size_t)g.nx * (g.ny + 1)))) return st;
-  W->pg_blocks = ((g.nx + 8 + kGy - 1) / kGy) * ((g.PY + kGx - 1) / kGx);
+  W->pg_blocks = ((g.nx + kGy - 1) / kGy) * ((g.ny + kGx - 1) / kGx);
+  std::vector<long> out_pads;
+  for (int k = 0; k < E.cheap.n; ++k) {
+    const long p = E.h_cheap_pad[k];
+    const long ip = p / g.PY, jp = p % g.PY;
+    if (ip < 4 || ip >= g.nx + 4 || jp < 4 || jp >= g.ny + 4) out_pads.push_back(p);
+  }
+  W->n_out = (int)out_pads.size();
+  if (W->n_out > 0 && (st = upload(E, &W->out_pads, out_pads))) return st;
   if ((st = dev_alloc(E, &W->PG, (size_t)kNParam * W->pg_blocks))) return st;
   if ((st = dev_alloc(E, &W->pacc, (size_t)kNParam))) return st;
   if ((st = dev_alloc(E, &W->aos, 6 * (size_t)g.nx * g.ny))) return st;
@@ -696,19 +766,23 @@ int init_adjoint(Engine& E) {
 }
 
 // OUT = a LAM + b1 X1 + b2 X2 + rscale (dl/dS)^T LAM for the state in
-// buffer buf; parameter cotangents accumulate into W.pacc.
-void launch_vjp(Engine& E, int buf, const double* LAM, double rscale, double a, const double* X1, double b1,
-                const double* X2, double b2, double* OUT) {
+// buffer buf; parameter cotangents accumulate into W.pacc.  taped: the
+// state's ghost points and Derived planes are still those the forward stage
+// that read it formed (otherwise they are formed first).
+void launch_vjp(Engine& E, int buf, bool taped, const double* LAM, double rscale, double a, const double* X1,
+                double b1, const double* X2, double b2, double* OUT) {
   AdjWork& W = *E.adj;
   const SolveDesc& d = E.h_desc[0];
   const Geo& g = d.g;
-  launch_ghosts(E, buf, 2u, false);
-  launch_derived(E, buf);
+  if (!taped) {
+    launch_ghosts(E, buf, 2u, false);
+    launch_derived(E, buf);
+  }
   AdjDesc A;
   A.g = g;
   A.pt = d.pt;
   A.S = d.U[buf];
-  A.D = d.D;
+  A.D = d.D[buf];
   A.LAM = LAM;
   A.XB = W.XB;
   A.YB = W.YB;
@@ -730,6 +804,8 @@ void launch_vjp(Engine& E, int buf, const double* LAM, double rscale, double a, 
   {
     Launcher L(E, K_ADJ_GATHER);
     k_adj_gather<<<W.pg_blocks, dim3(kGx, kGy), 0, E.stream>>>(A, E.phys);
+    if (W.n_out > 0)
+      k_adj_gather_ghosts<<<(W.n_out + 63) / 64, 64, 0, E.stream>>>(A, E.phys, W.n_out, W.out_pads);
   }
   if (W.nrows > 0) {
     Launcher L(E, K_ADJ_GHOST);
@@ -791,10 +867,11 @@ int ensure_dts(Engine& E, int n) {
 //   S2b  = 2/3 lam + (2dt/3) J(S2)^T lam
 //   S1b  = 1/4 S2b + (dt/4) J(S1)^T S2b
 //   out  = 1/3 lam + 3/4 S2b + S1b + dt J(U^n)^T S1b
-void launch_rk3_reverse(Engine& E, double dt, const double* lam, double* s2b, double* s1b, double* out) {
-  launch_vjp(E, 2, lam, 2.0 * dt / 3.0, 2.0 / 3.0, nullptr, 0.0, nullptr, 0.0, s2b);
-  launch_vjp(E, 1, s2b, dt / 4.0, 0.25, nullptr, 0.0, nullptr, 0.0, s1b);
-  launch_vjp(E, 0, s1b, dt, 1.0, lam, 1.0 / 3.0, s2b, 0.75, out);
+void launch_rk3_reverse(Engine& E, bool taped, double dt, const double* lam, double* s2b, double* s1b,
+                        double* out) {
+  launch_vjp(E, 2, taped, lam, 2.0 * dt / 3.0, 2.0 / 3.0, nullptr, 0.0, nullptr, 0.0, s2b);
+  launch_vjp(E, 1, taped, s2b, dt / 4.0, 0.25, nullptr, 0.0, nullptr, 0.0, s1b);
+  launch_vjp(E, 0, taped, s1b, dt, 1.0, lam, 1.0 / 3.0, s2b, 0.75, out);
 }
 
 }  // namespace
@@ -804,9 +881,11 @@ void launch_rk3_reverse(Engine& E, double dt, const double* lam, double* s2b, do
 // writes its stage states to s1[m], s2[m] and U^{n+1} to base(n + 1), where
 // base(n) is the checkpoint ckpt[n / K] when m == 0, mid[m] otherwise, and
 // base(N) is the context's own state.  Every step has its own device
-// descriptor with these buffers as U[0..3], so neither the forward nor the
-// recompute copies a state: the forward leaves the last segment on the tape
-// and the backward recomputes only the segments before it.
+// descriptor with these buffers as U[0..3] and the slot's own Derived planes
+// per stage state, so neither the forward nor the recompute copies a state,
+// the reverse sweep finds each stage state's ghost values and Derived planes
+// as its forward stage left them, the forward leaves the last segment on the
+// tape and the backward recomputes only the segments before it.
 struct rh3t_adjoint {
   rh3t_ctx* ctx = nullptr;
   int every = 5;
@@ -815,6 +894,7 @@ struct rh3t_adjoint {
   std::vector<double*> ckpt;     // U^{kK}, k = 0, 1, ...
   std::vector<double*> s1, s2;   // stage states of slot m
   std::vector<double*> mid;      // U^{kK + m}, m = 1 .. K-1 (mid[0] unused)
+  std::vector<double*> dpl;      // Derived planes of the three stage states of slot m: dpl[3m + b]
   std::vector<SolveDesc> h_steps;
   SolveDesc* d_steps = nullptr;
   int steps_cap = 0;
@@ -873,7 +953,7 @@ int rh3t_l_vjp(rh3t_ctx* ctx, const double* rbar, double* ubar, double* pgrad) {
   if (st) return st;
   AdjWork& W = *E.adj;
   if ((st = upload_cot(E, rbar, W.lam[0]))) return st;
-  launch_vjp(E, 0, W.lam[0], 1.0, 0.0, nullptr, 0.0, nullptr, 0.0, W.lam[1]);
+  launch_vjp(E, 0, false, W.lam[0], 1.0, 0.0, nullptr, 0.0, nullptr, 0.0, W.lam[1]);
   HIP_TRY(hipGetLastError());
   if ((st = download_cot(E, W.lam[1], ubar, pgrad))) return st;
   return check_errors(E);
@@ -893,7 +973,7 @@ int rh3t_rk3_vjp(rh3t_ctx* ctx, double dt, const double* lam, double* ubar, doub
   launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3});
   launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5});
   if ((st = upload_cot(E, lam, W.lam[0]))) return st;
-  launch_rk3_reverse(E, dt, W.lam[0], W.lam[1], W.lam[2], W.lam[3]);
+  launch_rk3_reverse(E, false, dt, W.lam[0], W.lam[1], W.lam[2], W.lam[3]);
   HIP_TRY(hipGetLastError());
   if ((st = download_cot(E, W.lam[3], ubar, pgrad))) return st;
   return check_errors(E);
@@ -916,6 +996,11 @@ int rh3t_adjoint_create(rh3t_ctx* ctx, int32_t checkpoint_every, rh3t_adjoint** 
     st = adj_alloc(A, &A->s1[m], plane6);
     if (!st) st = adj_alloc(A, &A->s2[m], plane6);
     if (!st && m > 0) st = adj_alloc(A, &A->mid[m], plane6);
+    for (int b = 0; b < 3 && !st; ++b) {
+      double* p = nullptr;
+      st = adj_alloc(A, &p, (size_t)D_N * E.h_desc[0].g.plane);
+      A->dpl.push_back(p);
+    }
     if (st) {
       rh3t_adjoint_destroy(A);
       return st;
@@ -964,6 +1049,7 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
     sd.U[1] = A->s1[n % K];
     sd.U[2] = A->s2[n % K];
     sd.U[3] = base(n + 1);
+    for (int b = 0; b < 3; ++b) sd.D[b] = A->dpl[3 * (n % K) + b];
   }
   if (steps.size() != A->h_steps.size() ||
       std::memcmp(steps.data(), A->h_steps.data(), sizeof(SolveDesc) * nsteps) != 0) {
@@ -1011,16 +1097,22 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
       for (int n = n0; n < n1; ++n) {
         k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, W.dts, n, 0);
         OnStep on(E, A, n);
-        launch_stage(E, 0, kStage1);
-        launch_stage(E, 1, kStage2);
-        if (n + 1 < n1) launch_stage(E, 2, kStage3);
+        launch_stage(E, 0, kStage1, false);
+        launch_stage(E, 1, kStage2, false);
+        if (n + 1 < n1) {
+          launch_stage(E, 2, kStage3, false);
+        } else {  // what stage 3 would have left on the tape
+          launch_ghosts(E, 2, 6u, false);
+          launch_derived(E, 2);
+        }
       }
       A->tape_seg = k;
     }
-    // reverse sweep over the segment
+    // reverse sweep over the segment: every stage state on the tape still
+    // holds the ghost values and Derived planes of its forward stage
     for (int n = n1 - 1; n >= n0; --n) {
       OnStep on(E, A, n);
-      launch_rk3_reverse(E, A->dts[n], lam, W.lam[1], W.lam[2], nxt);
+      launch_rk3_reverse(E, true, A->dts[n], lam, W.lam[1], W.lam[2], nxt);
       std::swap(lam, nxt);
     }
   }

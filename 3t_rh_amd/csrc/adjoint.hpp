@@ -359,6 +359,15 @@ __device__ __forceinline__ void roe_bar_final(const EigConst& K, const RoeV& R, 
 // interface_vjp in registers (it is re-read from L1/L2 instead).
 __device__ __forceinline__ void reload_barrier() { __asm__ volatile("" ::: "memory"); }
 
+// The same pointer, opaque to the compiler: the constants read through it
+// are loaded (scalar loads) where they are used instead of being hoisted to
+// the kernel entry and held in spilled scalar registers throughout.
+template <class T>
+__device__ __forceinline__ const T* fresh(const T* p) {
+  __asm__ volatile("" : "+s"(p));
+  return p;
+}
+
 // -------------------------------------------------- interface flux VJP
 
 // State and physical flux (flux_from_derived, src/state.rs:56-85) of the

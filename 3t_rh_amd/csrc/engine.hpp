@@ -22,7 +22,8 @@ struct SolveDesc {
   Geo g;
   double* U[4];         // U0, U1, U2; U[3]: stage-3 destination (U0 itself
                         // except where a step writes a new buffer)
-  double* D;
+  double* D[3];         // Derived planes of U0, U1, U2 (one shared buffer
+                        // except on an adjoint tape)
   double* FX;
   double* FY;
   double* RHS;          // compact SoA nx*ny (l() output)
@@ -65,6 +66,7 @@ struct StageSpec {
   int mode;        // 0: write rhs, 1..3: SSP-RK3 stage
   int base, add, dst;  // buffer indices into SolveDesc::U
   unsigned phase;
+  int src;         // buffer whose l() the stage applies (set by launch_stage)
 };
 
 enum KernelId {
@@ -191,7 +193,9 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, bool capture = true);
 // Derived::from_state on every fluid cell and ghost of buffer buf
 void launch_derived(Engine& E, int buf);
 // ghosts -> derived -> x/y fluxes -> assemble (one l() + stage) on buffer src
-void launch_stage(Engine& E, int src, const StageSpec& sa);
+// capture: record the failing point's state of a stage error (a recompute
+// of stages that already ran without error skips it)
+void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture = true);
 // one SSP-RK3 step with the dt held in the solves' scalars
 void launch_rk3(Engine& E);
 int check_errors(Engine& E);

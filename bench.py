@@ -148,13 +148,14 @@ def _gpu_ms(profile):
     return sum(avg * cnt for avg, cnt in profile.values())
 
 
-def adjoint_leg(rh3t, n=512, nsteps=20, every=5, fd_dirs=32):
+def adjoint_leg(rh3t, n=512, nsteps=20, every=5, fd_dirs=32, reps=5):
     """The differentiation workload: the smooth-mode periodic box at n^2
     with the exchange terms on, nsteps RK3 steps with the dt schedule of a
     base run frozen, J = sum(w * E_r(u^N)) with a seeded Gaussian window.
     Times the checkpointing forward (state kept every `every` steps) and the
-    backward (segment recompute + reverse sweep), by wall clock and by summed
-    per-kernel GPU time, and checks the gradient against central
+    backward (segment recompute + reverse sweep), by wall clock (medians of
+    `reps` pairs) and by summed per-kernel GPU time, and checks the gradient
+    against central
     differences (h = 1e-6) of the forward on fd_dirs seeded smooth
     directions over (u0, omega_ei, omega_er)."""
     import numpy as np
@@ -174,13 +175,16 @@ def adjoint_leg(rh3t, n=512, nsteps=20, every=5, fd_dirs=32):
     s.set_state(u)
     adj.forward(dts)
     adj.backward(lam)  # warm-up (allocations, code objects)
-    s.set_state(u)
-    t0 = time.perf_counter()
-    adj.forward(dts)
-    t_fwd = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    gu, gp = adj.backward(lam)
-    t_bwd = time.perf_counter() - t0
+    t_f, t_b = [], []
+    for _ in range(reps):  # medians of reps forward + backward pairs
+        s.set_state(u)
+        t0 = time.perf_counter()
+        adj.forward(dts)
+        t_f.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        gu, gp = adj.backward(lam)
+        t_b.append(time.perf_counter() - t0)
+    t_fwd, t_bwd = float(np.median(t_f)), float(np.median(t_b))
     # GPU-side time of the same two calls
     s.set_state(u)
     s.profile(True)
@@ -216,7 +220,7 @@ def adjoint_leg(rh3t, n=512, nsteps=20, every=5, fd_dirs=32):
     kern = {k: round(v[0] * v[1], 4) for k, v in prof_b.items() if v[1]}
     return {"workload": f"smooth-mode periodic box {n}x{n}, omega_ei=1, omega_er=0.1, {nsteps} SSP-RK3 steps, "
                         f"J = sum(gaussian window * E_r)",
-            "checkpoint_every": every, "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+            "checkpoint_every": every, "timing_reps": reps, "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
             "adjoint_over_forward": t_bwd / t_fwd, "forward_gpu_ms": g_fwd, "backward_gpu_ms": g_bwd,
             "adjoint_over_forward_gpu": g_bwd / g_fwd, "backward_kernels_ms": kern,
             "grad_params": [float(gp[0]), float(gp[1])], "fd_directions": fd_dirs, "fd_h": h,
