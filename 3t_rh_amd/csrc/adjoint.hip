@@ -782,7 +782,7 @@ void launch_vjp(Engine& E, int buf, bool taped, const double* LAM, double rscale
   A.g = g;
   A.pt = d.pt;
   A.S = d.U[buf];
-  A.D = d.D[buf];
+  A.D = d.D;
   A.LAM = LAM;
   A.XB = W.XB;
   A.YB = W.YB;
@@ -867,10 +867,15 @@ int ensure_dts(Engine& E, int n) {
 //   S2b  = 2/3 lam + (2dt/3) J(S2)^T lam
 //   S1b  = 1/4 S2b + (dt/4) J(S1)^T S2b
 //   out  = 1/3 lam + 3/4 S2b + S1b + dt J(U^n)^T S1b
+// on(b) is called before the VJP of the stage state in buffer b.
+template <class Select>
 void launch_rk3_reverse(Engine& E, bool taped, double dt, const double* lam, double* s2b, double* s1b,
-                        double* out) {
+                        double* out, Select on) {
+  on(2);
   launch_vjp(E, 2, taped, lam, 2.0 * dt / 3.0, 2.0 / 3.0, nullptr, 0.0, nullptr, 0.0, s2b);
+  on(1);
   launch_vjp(E, 1, taped, s2b, dt / 4.0, 0.25, nullptr, 0.0, nullptr, 0.0, s1b);
+  on(0);
   launch_vjp(E, 0, taped, s1b, dt, 1.0, lam, 1.0 / 3.0, s2b, 0.75, out);
 }
 
@@ -880,9 +885,10 @@ void launch_rk3_reverse(Engine& E, bool taped, double dt, const double* lam, dou
 // (segment n / K, slot m = n % K) reads its state U^n from base(n) and
 // writes its stage states to s1[m], s2[m] and U^{n+1} to base(n + 1), where
 // base(n) is the checkpoint ckpt[n / K] when m == 0, mid[m] otherwise, and
-// base(N) is the context's own state.  Every step has its own device
-// descriptor with these buffers as U[0..3] and the slot's own Derived planes
-// per stage state, so neither the forward nor the recompute copies a state,
+// base(N) is the context's own state.  Every step and stage state b has its
+// own device descriptor with these buffers as U[0..3] and the slot's Derived
+// planes of state b as D, so neither the forward nor the recompute copies a
+// state,
 // the reverse sweep finds each stage state's ghost values and Derived planes
 // as its forward stage left them, the forward leaves the last segment on the
 // tape and the backward recomputes only the segments before it.
@@ -895,7 +901,7 @@ struct rh3t_adjoint {
   std::vector<double*> s1, s2;   // stage states of slot m
   std::vector<double*> mid;      // U^{kK + m}, m = 1 .. K-1 (mid[0] unused)
   std::vector<double*> dpl;      // Derived planes of the three stage states of slot m: dpl[3m + b]
-  std::vector<SolveDesc> h_steps;
+  std::vector<SolveDesc> h_steps;  // [3 n + b]
   SolveDesc* d_steps = nullptr;
   int steps_cap = 0;
   int tape_seg = -1;             // segment whose stage states the tape holds
@@ -923,14 +929,20 @@ int adj_alloc(rh3t_adjoint* A, double** p, size_t n) {
   return RH3T_OK;
 }
 
-// Launches of step n run on that step's buffers while this is in scope.
+// While in scope, launches run on the buffers of step n; at(b) selects the
+// descriptor whose Derived planes are those of stage state b.
 struct OnStep {
   Engine& E;
+  const rh3t_adjoint* A;
+  int n;
   SolveDesc* d0;
   SolveDesc h0;
-  OnStep(Engine& e, const rh3t_adjoint* A, int n) : E(e), d0(e.d_desc), h0(e.h_desc[0]) {
-    E.d_desc = A->d_steps + n;
-    E.h_desc[0] = A->h_steps[n];
+  OnStep(Engine& e, const rh3t_adjoint* a, int step) : E(e), A(a), n(step), d0(e.d_desc), h0(e.h_desc[0]) {
+    at(0);
+  }
+  void at(int b) {
+    E.d_desc = A->d_steps + 3 * n + b;
+    E.h_desc[0] = A->h_steps[3 * n + b];
   }
   ~OnStep() {
     E.d_desc = d0;
@@ -973,7 +985,7 @@ int rh3t_rk3_vjp(rh3t_ctx* ctx, double dt, const double* lam, double* ubar, doub
   launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3});
   launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5});
   if ((st = upload_cot(E, lam, W.lam[0]))) return st;
-  launch_rk3_reverse(E, false, dt, W.lam[0], W.lam[1], W.lam[2], W.lam[3]);
+  launch_rk3_reverse(E, false, dt, W.lam[0], W.lam[1], W.lam[2], W.lam[3], [](int) {});
   HIP_TRY(hipGetLastError());
   if ((st = download_cot(E, W.lam[3], ubar, pgrad))) return st;
   return check_errors(E);
@@ -1036,25 +1048,26 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
   }
   if (A->steps_cap < nsteps) {
     void* q = nullptr;
-    HIP_TRY(hipMalloc(&q, sizeof(SolveDesc) * nsteps));
+    HIP_TRY(hipMalloc(&q, sizeof(SolveDesc) * 3 * nsteps));
     A->allocs.push_back(q);
     A->d_steps = (SolveDesc*)q;
     A->steps_cap = nsteps;
   }
   auto base = [&](int n) { return n == nsteps ? d.U[0] : (n % K == 0 ? A->ckpt[n / K] : A->mid[n % K]); };
-  std::vector<SolveDesc> steps(nsteps, d);
-  for (int n = 0; n < nsteps; ++n) {
-    SolveDesc& sd = steps[n];
-    sd.U[0] = base(n);
-    sd.U[1] = A->s1[n % K];
-    sd.U[2] = A->s2[n % K];
-    sd.U[3] = base(n + 1);
-    for (int b = 0; b < 3; ++b) sd.D[b] = A->dpl[3 * (n % K) + b];
-  }
+  std::vector<SolveDesc> steps(3 * (size_t)nsteps, d);
+  for (int n = 0; n < nsteps; ++n)
+    for (int b = 0; b < 3; ++b) {
+      SolveDesc& sd = steps[3 * n + b];
+      sd.U[0] = base(n);
+      sd.U[1] = A->s1[n % K];
+      sd.U[2] = A->s2[n % K];
+      sd.U[3] = base(n + 1);
+      sd.D = A->dpl[3 * (n % K) + b];
+    }
   if (steps.size() != A->h_steps.size() ||
-      std::memcmp(steps.data(), A->h_steps.data(), sizeof(SolveDesc) * nsteps) != 0) {
+      std::memcmp(steps.data(), A->h_steps.data(), sizeof(SolveDesc) * steps.size()) != 0) {
     HIP_TRY(hipStreamSynchronize(E.stream));
-    HIP_TRY(hipMemcpy(A->d_steps, steps.data(), sizeof(SolveDesc) * nsteps, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(A->d_steps, steps.data(), sizeof(SolveDesc) * steps.size(), hipMemcpyHostToDevice));
     A->h_steps.swap(steps);
   }
   if ((st = ensure_dts(E, nsteps))) return st;
@@ -1067,7 +1080,9 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
     k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, E.adj->dts, n, 1);
     OnStep on(E, A, n);
     launch_stage(E, 0, kStage1);
+    on.at(1);
     launch_stage(E, 1, kStage2);
+    on.at(2);
     launch_stage(E, 2, kStage3);
   }
   HIP_TRY(hipGetLastError());
@@ -1098,7 +1113,9 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
         k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, W.dts, n, 0);
         OnStep on(E, A, n);
         launch_stage(E, 0, kStage1, false);
+        on.at(1);
         launch_stage(E, 1, kStage2, false);
+        on.at(2);
         if (n + 1 < n1) {
           launch_stage(E, 2, kStage3, false);
         } else {  // what stage 3 would have left on the tape
@@ -1112,7 +1129,7 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
     // holds the ghost values and Derived planes of its forward stage
     for (int n = n1 - 1; n >= n0; --n) {
       OnStep on(E, A, n);
-      launch_rk3_reverse(E, true, A->dts[n], lam, W.lam[1], W.lam[2], nxt);
+      launch_rk3_reverse(E, true, A->dts[n], lam, W.lam[1], W.lam[2], nxt, [&](int b) { on.at(b); });
       std::swap(lam, nxt);
     }
   }

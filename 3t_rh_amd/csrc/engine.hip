@@ -475,7 +475,7 @@ __global__ __launch_bounds__(256) void k_derived(const SolveDesc* __restrict__ d
   derived_from_state(ph, st, d, nplanes > 8);
 #pragma unroll
   for (int f = 0; f < D_N; ++f)
-    if (f < nplanes) sd.D[buf][f * sd.g.plane + p] = d[f];
+    if (f < nplanes) sd.D[f * sd.g.plane + p] = d[f];
 }
 
 // ------------------------------------------------------- WENO flux
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
   const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
   if (i >= ni || j >= nj) return;
   const double* __restrict__ S = sd.U[buf];
-  const double* __restrict__ D = sd.D[buf];
+  const double* __restrict__ D = sd.D;
   double* __restrict__ F = DIR == 0 ? sd.FX : sd.FY;
   const long fpl = DIR == 0 ? (long)(g.nx + 1) * g.ny : (long)g.nx * (g.ny + 1);
   const long fidx = DIR == 0 ? (long)i * g.ny + j : (long)i * (g.ny + 1) + j;
@@ -560,7 +560,7 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
       for (int f = 0; f < 6; ++f) sd.RHS[f * ncell + lin] = 0.0;
     return INFINITY;
   }
-  const double* __restrict__ D = sd.D[sa.src];
+  const double* __restrict__ D = sd.D;
   const double* __restrict__ FX = sd.FX;
   const double* __restrict__ FY = sd.FY;
   const long fxpl = (long)(g.nx + 1) * g.ny, fypl = (long)g.nx * (g.ny + 1);
@@ -885,9 +885,7 @@ void launch_derived(Engine& E, int buf) {
   k_derived<<<total(E.t_der), 256, 0, E.stream>>>(E.d_desc, E.tab(3), E.nsolve, E.phys, buf, E.nplanes);
 }
 
-void launch_stage(Engine& E, int src, const StageSpec& spec, bool capture) {
-  StageSpec sa = spec;
-  sa.src = src;
+void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture) {
   launch_ghosts(E, src, sa.mode == 0 ? 2u : (unsigned)(2 * sa.mode), capture);
   launch_derived(E, src);
   dim3 blk(E.bx, E.by);
@@ -1174,9 +1172,8 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
       HIP_TRY(hipMemset(d.U[b], 0, 6 * (size_t)g.plane * sizeof(double)));
     }
     d.U[3] = d.U[0];
-    if ((st = dev_alloc(E, &d.D[0], (size_t)D_N * g.plane))) return st;
-    HIP_TRY(hipMemset(d.D[0], 0, (size_t)D_N * g.plane * sizeof(double)));
-    d.D[1] = d.D[2] = d.D[0];
+    if ((st = dev_alloc(E, &d.D, (size_t)D_N * g.plane))) return st;
+    HIP_TRY(hipMemset(d.D, 0, (size_t)D_N * g.plane * sizeof(double)));
     if ((st = dev_alloc(E, &d.FX, 6 * (size_t)(g.nx + 1) * g.ny))) return st;
     if ((st = dev_alloc(E, &d.FY, 6 * (size_t)g.nx * (g.ny + 1)))) return st;
     if ((st = dev_alloc(E, &d.RHS, 6 * (size_t)ncell))) return st;

@@ -22,8 +22,9 @@ struct SolveDesc {
   Geo g;
   double* U[4];         // U0, U1, U2; U[3]: stage-3 destination (U0 itself
                         // except where a step writes a new buffer)
-  double* D[3];         // Derived planes of U0, U1, U2 (one shared buffer
-                        // except on an adjoint tape)
+  double* D;            // Derived planes of the state a kernel reads (on an
+                        // adjoint tape every step and stage state has its
+                        // own descriptor and planes)
   double* FX;
   double* FY;
   double* RHS;          // compact SoA nx*ny (l() output)
@@ -66,7 +67,6 @@ struct StageSpec {
   int mode;        // 0: write rhs, 1..3: SSP-RK3 stage
   int base, add, dst;  // buffer indices into SolveDesc::U
   unsigned phase;
-  int src;         // buffer whose l() the stage applies (set by launch_stage)
 };
 
 enum KernelId {
