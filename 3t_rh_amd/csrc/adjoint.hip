@@ -881,30 +881,35 @@ void launch_rk3_reverse(Engine& E, bool taped, double dt, const double* lam, dou
 
 }  // namespace
 
-// Buffers of a checkpointed rollout with checkpoint interval K.  Step n
-// (segment n / K, slot m = n % K) reads its state U^n from base(n) and
-// writes its stage states to s1[m], s2[m] and U^{n+1} to base(n + 1), where
-// base(n) is the checkpoint ckpt[n / K] when m == 0, mid[m] otherwise, and
-// base(N) is the context's own state.  Every step and stage state b has its
-// own device descriptor with these buffers as U[0..3] and the slot's Derived
-// planes of state b as D, so neither the forward nor the recompute copies a
-// state,
-// the reverse sweep finds each stage state's ghost values and Derived planes
-// as its forward stage left them, the forward leaves the last segment on the
-// tape and the backward recomputes only the segments before it.
+// Buffers of a checkpointed rollout with checkpoint interval K.  Segment k
+// (steps kK .. kK+K-1) is taped in set k % 2 of two tape sets.  Step n
+// (segment k = n / K, slot m = n % K) reads its state U^n from base(n) and
+// writes its stage states to s1, s2 of its set and slot and U^{n+1} to
+// base(n + 1), where base(n) is the checkpoint ckpt[k] when m == 0, the
+// set's mid[m] otherwise, and base(N) is the context's own state.  Every
+// step and stage state b has its own device descriptor with these buffers as
+// U[0..3] and the slot's Derived planes of state b as D, so neither the
+// forward nor the recompute copies a state, and the reverse sweep finds each
+// stage state's ghost values and Derived planes as its forward stage left
+// them.  The forward leaves its last two segments on the tape; the backward
+// recomputes each earlier segment on a second stream while the reverse sweep
+// of the segment after it runs on the context's stream.
 struct rh3t_adjoint {
   rh3t_ctx* ctx = nullptr;
   int every = 5;
   int nsteps = 0;
   std::vector<double> dts;
   std::vector<double*> ckpt;     // U^{kK}, k = 0, 1, ...
-  std::vector<double*> s1, s2;   // stage states of slot m
-  std::vector<double*> mid;      // U^{kK + m}, m = 1 .. K-1 (mid[0] unused)
-  std::vector<double*> dpl;      // Derived planes of the three stage states of slot m: dpl[3m + b]
+  std::vector<double*> s1, s2;   // stage states, [set * K + m]
+  std::vector<double*> mid;      // U^{kK + m}, m = 1 .. K-1, [set * K + m] (m = 0 unused)
+  std::vector<double*> dpl;      // Derived planes of stage state b, [(set * K + m) * 3 + b]
   std::vector<SolveDesc> h_steps;  // [3 n + b]
   SolveDesc* d_steps = nullptr;
   int steps_cap = 0;
-  int tape_seg = -1;             // segment whose stage states the tape holds
+  int tape_of[2] = {-1, -1};     // segment whose stage states each set holds
+  hipStream_t side = nullptr;    // recompute stream (default priority)
+  hipEvent_t ev_start = nullptr, ev_free[2] = {nullptr, nullptr}, ev_ready[2] = {nullptr, nullptr};
+  hipEvent_t ev_join = nullptr;
   std::vector<void*> allocs;
 };
 
@@ -950,7 +955,36 @@ struct OnStep {
   }
 };
 
+// While in scope, launches go to stream st.
+struct OnStream {
+  Engine& E;
+  hipStream_t s0;
+  OnStream(Engine& e, hipStream_t st) : E(e), s0(e.stream) { E.stream = st; }
+  ~OnStream() { E.stream = s0; }
+};
+
 const StageSpec kStage1{1, 0, 0, 1, 3}, kStage2{2, 0, 1, 2, 5}, kStage3{3, 0, 2, 3, 7};
+
+// Recomputes segment k from its checkpoint onto tape set k % 2 (U^{n1} is
+// the next checkpoint already; stage 3 of the last step only fills what the
+// reverse sweep reads of S2).
+void recompute_segment(Engine& E, const rh3t_adjoint* A, int k) {
+  const int n0 = k * A->every, n1 = std::min(A->nsteps, n0 + A->every);
+  for (int n = n0; n < n1; ++n) {
+    k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, E.adj->dts, n, 0);
+    OnStep on(E, A, n);
+    launch_stage(E, 0, kStage1, false);
+    on.at(1);
+    launch_stage(E, 1, kStage2, false);
+    on.at(2);
+    if (n + 1 < n1) {
+      launch_stage(E, 2, kStage3, false);
+    } else {
+      launch_ghosts(E, 2, 6u, false);
+      launch_derived(E, 2);
+    }
+  }
+}
 
 }  // namespace
 
@@ -1000,23 +1034,32 @@ int rh3t_adjoint_create(rh3t_ctx* ctx, int32_t checkpoint_every, rh3t_adjoint** 
   auto* A = new rh3t_adjoint();
   A->ctx = ctx;
   A->every = checkpoint_every;
-  const size_t plane6 = 6 * (size_t)E.h_desc[0].g.plane;
-  A->s1.assign(checkpoint_every, nullptr);
-  A->s2.assign(checkpoint_every, nullptr);
-  A->mid.assign(checkpoint_every, nullptr);
-  for (int m = 0; m < checkpoint_every; ++m) {
-    st = adj_alloc(A, &A->s1[m], plane6);
-    if (!st) st = adj_alloc(A, &A->s2[m], plane6);
-    if (!st && m > 0) st = adj_alloc(A, &A->mid[m], plane6);
-    for (int b = 0; b < 3 && !st; ++b) {
-      double* p = nullptr;
-      st = adj_alloc(A, &p, (size_t)D_N * E.h_desc[0].g.plane);
-      A->dpl.push_back(p);
+  const size_t plane = (size_t)E.h_desc[0].g.plane;
+  const int K = checkpoint_every;
+  A->s1.assign(2 * K, nullptr);
+  A->s2.assign(2 * K, nullptr);
+  A->mid.assign(2 * K, nullptr);
+  A->dpl.assign(6 * K, nullptr);
+  for (int q = 0; q < 2 * K && !st; ++q) {
+    st = adj_alloc(A, &A->s1[q], 6 * plane);
+    if (!st) st = adj_alloc(A, &A->s2[q], 6 * plane);
+    if (!st && q % K > 0) st = adj_alloc(A, &A->mid[q], 6 * plane);
+    for (int b = 0; b < 3 && !st; ++b) st = adj_alloc(A, &A->dpl[3 * q + b], (size_t)D_N * plane);
+  }
+  hipError_t e = hipSuccess;
+  if (!st) {
+    e = hipStreamCreateWithFlags(&A->side, hipStreamNonBlocking);
+    hipEvent_t* evs[6] = {&A->ev_start, &A->ev_free[0], &A->ev_free[1], &A->ev_ready[0], &A->ev_ready[1],
+                          &A->ev_join};
+    for (int q = 0; q < 6 && e == hipSuccess; ++q) e = hipEventCreateWithFlags(evs[q], hipEventDisableTiming);
+    if (e != hipSuccess) {
+      std::snprintf(E.message, sizeof(E.message), "stream/event: %s", hipGetErrorString(e));
+      st = RH3T_ERR_HIP;
     }
-    if (st) {
-      rh3t_adjoint_destroy(A);
-      return st;
-    }
+  }
+  if (st) {
+    rh3t_adjoint_destroy(A);
+    return st;
   }
   *out = A;
   return RH3T_OK;
@@ -1026,6 +1069,12 @@ void rh3t_adjoint_destroy(rh3t_adjoint* A) {
   if (!A) return;
   (void)hipSetDevice(A->ctx->e.device);
   (void)hipStreamSynchronize(A->ctx->e.stream);
+  if (A->side) {
+    (void)hipStreamSynchronize(A->side);
+    (void)hipStreamDestroy(A->side);
+  }
+  for (hipEvent_t ev : {A->ev_start, A->ev_free[0], A->ev_free[1], A->ev_ready[0], A->ev_ready[1], A->ev_join})
+    if (ev) (void)hipEventDestroy(ev);
   for (void* p : A->allocs) (void)hipFree(p);
   delete A;
 }
@@ -1053,27 +1102,32 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
     A->d_steps = (SolveDesc*)q;
     A->steps_cap = nsteps;
   }
-  auto base = [&](int n) { return n == nsteps ? d.U[0] : (n % K == 0 ? A->ckpt[n / K] : A->mid[n % K]); };
+  auto slot = [&](int n) { return ((n / K) % 2) * K + n % K; };
+  auto base = [&](int n) { return n == nsteps ? d.U[0] : (n % K == 0 ? A->ckpt[n / K] : A->mid[slot(n)]); };
   std::vector<SolveDesc> steps(3 * (size_t)nsteps, d);
   for (int n = 0; n < nsteps; ++n)
     for (int b = 0; b < 3; ++b) {
       SolveDesc& sd = steps[3 * n + b];
       sd.U[0] = base(n);
-      sd.U[1] = A->s1[n % K];
-      sd.U[2] = A->s2[n % K];
+      sd.U[1] = A->s1[slot(n)];
+      sd.U[2] = A->s2[slot(n)];
       sd.U[3] = base(n + 1);
-      sd.D = A->dpl[3 * (n % K) + b];
+      sd.D = A->dpl[3 * slot(n) + b];
     }
   if (steps.size() != A->h_steps.size() ||
       std::memcmp(steps.data(), A->h_steps.data(), sizeof(SolveDesc) * steps.size()) != 0) {
     HIP_TRY(hipStreamSynchronize(E.stream));
+    HIP_TRY(hipStreamSynchronize(A->side));
     HIP_TRY(hipMemcpy(A->d_steps, steps.data(), sizeof(SolveDesc) * steps.size(), hipMemcpyHostToDevice));
     A->h_steps.swap(steps);
   }
   if ((st = ensure_dts(E, nsteps))) return st;
   A->dts.assign(dt, dt + nsteps);
   A->nsteps = nsteps;
-  A->tape_seg = -1;
+  A->tape_of[0] = A->tape_of[1] = -1;
+  // a backward still recomputing on the side stream must finish first
+  HIP_TRY(hipEventRecord(A->ev_join, A->side));
+  HIP_TRY(hipStreamWaitEvent(E.stream, A->ev_join, 0));
   HIP_TRY(hipMemcpyAsync(E.adj->dts, dt, nsteps * sizeof(double), hipMemcpyHostToDevice, E.stream));
   HIP_TRY(hipMemcpyAsync(A->ckpt[0], d.U[0], plane6 * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
   for (int n = 0; n < nsteps; ++n) {
@@ -1086,7 +1140,9 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
     launch_stage(E, 2, kStage3);
   }
   HIP_TRY(hipGetLastError());
-  A->tape_seg = nck - 1;
+  // the last two segments are on the tape
+  A->tape_of[(nck - 1) % 2] = nck - 1;
+  if (nck >= 2) A->tape_of[(nck - 2) % 2] = nck - 2;
   E.dt_partials_valid = true;
   return check_errors(E);
 }
@@ -1098,32 +1154,42 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
   int st = begin_adjoint_call(E);
   if (st) return st;
   AdjWork& W = *E.adj;
+  const hipStream_t main = E.stream;
+  HIP_TRY(hipMemcpyAsync(W.dts, A->dts.data(), A->nsteps * sizeof(double), hipMemcpyHostToDevice, main));
+  // the side stream starts after everything queued so far; both tape sets
+  // are free until a reverse sweep reads them
+  HIP_TRY(hipEventRecord(A->ev_start, main));
+  HIP_TRY(hipStreamWaitEvent(A->side, A->ev_start, 0));
+  HIP_TRY(hipEventRecord(A->ev_free[0], main));
+  HIP_TRY(hipEventRecord(A->ev_free[1], main));
+  bool pending[2] = {false, false};
+  // enqueue the recompute of segment k on the side stream unless its set
+  // holds it already
+  auto ensure = [&](int k) -> int {
+    if (k < 0 || A->tape_of[k % 2] == k) return RH3T_OK;
+    const int set = k % 2;
+    A->tape_of[set] = -1;
+    HIP_TRY(hipStreamWaitEvent(A->side, A->ev_free[set], 0));
+    {
+      OnStream os(E, A->side);
+      recompute_segment(E, A, k);
+    }
+    HIP_TRY(hipEventRecord(A->ev_ready[set], A->side));
+    A->tape_of[set] = k;
+    pending[set] = true;
+    return RH3T_OK;
+  };
+  const int nseg = (A->nsteps + A->every - 1) / A->every;
+  if ((st = ensure(nseg - 1)) || (st = ensure(nseg - 2))) return st;
   double* lam = W.lam[0];
   double* nxt = W.lam[3];
   if ((st = upload_cot(E, lam_final, lam))) return st;
-  HIP_TRY(hipMemcpyAsync(E.adj->dts, A->dts.data(), A->nsteps * sizeof(double), hipMemcpyHostToDevice, E.stream));
-  const int nseg = (A->nsteps + A->every - 1) / A->every;
   for (int k = nseg - 1; k >= 0; --k) {
     const int n0 = k * A->every, n1 = std::min(A->nsteps, n0 + A->every);
-    if (A->tape_seg != k) {
-      // recompute the segment from its checkpoint onto the tape (U^{n1} is
-      // the next checkpoint already)
-      A->tape_seg = -1;
-      for (int n = n0; n < n1; ++n) {
-        k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, W.dts, n, 0);
-        OnStep on(E, A, n);
-        launch_stage(E, 0, kStage1, false);
-        on.at(1);
-        launch_stage(E, 1, kStage2, false);
-        on.at(2);
-        if (n + 1 < n1) {
-          launch_stage(E, 2, kStage3, false);
-        } else {  // what stage 3 would have left on the tape
-          launch_ghosts(E, 2, 6u, false);
-          launch_derived(E, 2);
-        }
-      }
-      A->tape_seg = k;
+    const int set = k % 2;
+    if (pending[set]) {
+      HIP_TRY(hipStreamWaitEvent(main, A->ev_ready[set], 0));
+      pending[set] = false;
     }
     // reverse sweep over the segment: every stage state on the tape still
     // holds the ghost values and Derived planes of its forward stage
@@ -1132,8 +1198,12 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
       launch_rk3_reverse(E, true, A->dts[n], lam, W.lam[1], W.lam[2], nxt, [&](int b) { on.at(b); });
       std::swap(lam, nxt);
     }
+    HIP_TRY(hipEventRecord(A->ev_free[set], main));
+    if ((st = ensure(k - 2))) return st;
   }
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(A->ev_join, A->side));
+  HIP_TRY(hipStreamWaitEvent(main, A->ev_join, 0));
   if ((st = download_cot(E, lam, lam_initial, pgrad))) return st;
   // the recompute's stage 3 overwrote the dt minima
   E.dt_partials_valid = false;
