@@ -319,7 +319,8 @@ typedef struct rh3t_adjoint rh3t_adjoint;
 /* Adjoint of nsteps RK3 steps of ctx (which must outlive it): the forward
  * keeps the state every checkpoint_every steps; the backward recomputes
  * each segment from its checkpoint (taping the stage states) and sweeps it
- * in reverse. */
+ * in reverse.  Device memory: the checkpoints plus two segments of stage
+ * states (2 checkpoint_every steps x (3 states + their Derived planes)). */
 int rh3t_adjoint_create(rh3t_ctx* ctx, int32_t checkpoint_every,
                         rh3t_adjoint** out);
 void rh3t_adjoint_destroy(rh3t_adjoint* adj);
@@ -328,9 +329,10 @@ void rh3t_adjoint_destroy(rh3t_adjoint* adj);
 int rh3t_adjoint_forward(rh3t_adjoint* adj, int32_t nsteps, const double* dt);
 /* lam_initial = (d u^N / d u^0)^T lam_final and the parameter cotangent,
  * for the last forward.  The context's state is not touched (it stays at
- * u^N unless the caller changed it); the stage states of the forward's last
- * checkpoint segment are kept, so the backward recomputes only the segments
- * before it. */
+ * u^N unless the caller changed it).  The stage states of the forward's
+ * last two checkpoint segments are kept, so the backward recomputes only
+ * the segments before them, each on a second stream while the reverse
+ * sweep of the segment after it runs. */
 int rh3t_adjoint_backward(rh3t_adjoint* adj, const double* lam_final,
                           double* lam_initial, double* pgrad);
 
