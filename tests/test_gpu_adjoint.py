@@ -98,8 +98,9 @@ def _dts(solver, u, nsteps):
 
 def test_adjoint_checkpoint_schedule(rh3t, workloads):
     """12 steps on a 48x48 box: the gradient does not depend on the
-    checkpoint interval (bitwise), and satisfies the dot-product identity
-    against the oracle's forward mode through all 12 steps."""
+    checkpoint interval (bitwise) nor on what an earlier backward left on the
+    tape, and satisfies the dot-product identity against the oracle's
+    forward mode through all 12 steps."""
     f, u, p = _case(workloads, "periodic", 48)
     _say("adjoint checkpoint schedule periodic 48, 12 steps")
     s = f.solver(physics=rh3t.Physics(**KW_ACTIVE))
@@ -115,6 +116,10 @@ def test_adjoint_checkpoint_schedule(rh3t, workloads):
         lam0, pg = adj.backward(w)
         after, _ = s.get_state()
         assert np.array_equal(after, un), "backward leaves the context at u^N"
+        # a second backward finds other segments on the tape and recomputes
+        # the last ones too
+        lam0b, pgb = adj.backward(w)
+        assert np.array_equal(lam0b, lam0) and np.array_equal(pgb, pg)
         results.append((lam0, pg))
         adj.close()
     for lam0, pg in results[1:]:
