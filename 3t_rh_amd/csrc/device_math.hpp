@@ -71,16 +71,6 @@ __device__ __forceinline__ double weno5(double u0, double u1, double u2, double 
   return w0 * p0 + w1 * p1 + w2 * p2;
 }
 
-// Roe state and the closed-form 6x6 left/right eigenvectors of
-// Stencil6::build_l_plain / build_r_plain (src/weno.rs:28-548).  DIR 0 = X
-// (rows: u-c, electron, ion, radiation, transverse, u+c), DIR 1 = Y (rows:
-// v-c, transverse, electron, ion, radiation, v+c).
-struct Eigen {
-  double L[6][6];
-  double R[6][6];
-  double lam[6];
-};
-
 // Energy-mode row of L, src/weno.rs:110-233 (identical in X and Y).
 __device__ __forceinline__ void energy_row(int mode, double hk, double u, double v, double w2,
                                            double cs2, double gt, double b, double ge, double gi,
@@ -104,111 +94,12 @@ __device__ __forceinline__ void energy_row(int mode, double hk, double u, double
   }
 }
 
-// rho1/rho2: densities of points [3] and [2]; d3/d2 their (u, v, e_e, e_i,
-// e_r) in derived order.
-template <int DIR>
-__device__ __forceinline__ void build_eigen(const Phys& ph, double rho3, double rho2, double u3,
-                                            double v3, double ee3, double ei3, double er3,
-                                            double u2, double v2, double ee2, double ei2,
-                                            double er2, Eigen& E) {
-  double r1 = sqrt(rho3);
-  double r2 = sqrt(rho2);
-  double rs = r1 + r2;
-  double u = (u3 * r1 + u2 * r2) / rs;
-  double v = (v3 * r1 + v2 * r2) / rs;
-  double w2 = sq(u) + sq(v);
-  double ee = (ee3 * r1 + ee2 * r2) / rs;
-  double ei = (ei3 * r1 + ei2 * r2) / rs;
-  double er = (er3 * r1 + er2 * r2) / rs;
-  const double ge = ph.ge, gi = ph.gi, gr = ph.gr, gt = ph.gt;
-  double cs2 = ph.gamma_e * ge * ee + ph.gamma_i * gi * ei + ph.gamma_r * gr * er;
-  double cs = sqrt(cs2);
-  double he = 6.0 * gi * gr * (ph.gamma_i * ei - ph.gamma_r * er) + ph.gamma_e * gt * ee * w2;
-  double hi = 6.0 * ge * gr * (ph.gamma_r * er - ph.gamma_e * ee) + ph.gamma_i * gt * ei * w2;
-  double hr = 6.0 * ge * gi * (ph.gamma_e * ee - ph.gamma_i * ei) + ph.gamma_r * gt * er * w2;
-  double b = gt * (36.0 * ge * gi * gr + gt * sq(w2)) * cs2;
-  double ad = 12.0 * cs2;
-  const int ie = DIR == 0 ? 1 : 2, ii = DIR == 0 ? 2 : 3, ir = DIR == 0 ? 3 : 4;
-  const int it = DIR == 0 ? 4 : 1;
-  if (DIR == 0) {
-    E.L[0][0] = (gt * w2 + 6.0 * u * cs) / ad;
-    E.L[0][1] = (-2.0 * gt * u - 6.0 * cs) / ad;
-    E.L[0][2] = (-2.0 * gt * v) / ad;
-    E.L[5][0] = (gt * w2 - 6.0 * u * cs) / ad;
-    E.L[5][1] = (-2.0 * gt * u + 6.0 * cs) / ad;
-    E.L[5][2] = (-2.0 * gt * v) / ad;
-    E.L[it][0] = -v;
-    E.L[it][1] = 0.0;
-    E.L[it][2] = 1.0;
-  } else {
-    E.L[0][0] = (gt * w2 + 6.0 * v * cs) / ad;
-    E.L[0][1] = (-2.0 * gt * u) / ad;
-    E.L[0][2] = (-2.0 * gt * v - 6.0 * cs) / ad;
-    E.L[5][0] = (gt * w2 - 6.0 * v * cs) / ad;
-    E.L[5][1] = (-2.0 * gt * u) / ad;
-    E.L[5][2] = (-2.0 * gt * v + 6.0 * cs) / ad;
-    E.L[it][0] = -u;
-    E.L[it][1] = 1.0;
-    E.L[it][2] = 0.0;
-  }
-  E.L[it][3] = 0.0;
-  E.L[it][4] = 0.0;
-  E.L[it][5] = 0.0;
-  E.L[0][3] = 6.0 * ge / ad;
-  E.L[0][4] = 6.0 * gi / ad;
-  E.L[0][5] = 6.0 * gr / ad;
-  E.L[5][3] = E.L[0][3];
-  E.L[5][4] = E.L[0][4];
-  E.L[5][5] = E.L[0][5];
-  energy_row(0, he, u, v, w2, cs2, gt, b, ge, gi, gr, E.L[ie]);
-  energy_row(1, hi, u, v, w2, cs2, gt, b, ge, gi, gr, E.L[ii]);
-  energy_row(2, hr, u, v, w2, cs2, gt, b, ge, gi, gr, E.L[ir]);
-
-  double ge_ = ph.gamma_e * ee + w2 / 6.0;
-  double gi_ = ph.gamma_i * ei + w2 / 6.0;
-  double gr_ = ph.gamma_r * er + w2 / 6.0;
-  if (DIR == 0) {
-    double ucs = u * cs / 3.0;
-    const double r[6][6] = {{1.0, 1.0, 1.0, 1.0, 0.0, 1.0},
-                            {u - cs, u, u, u, 0.0, u + cs},
-                            {v, v, v, v, 1.0, v},
-                            {ge_ - ucs, gt * w2 / (6.0 * ge), -gr, gi, v / 3.0, ge_ + ucs},
-                            {gi_ - ucs, gr, gt * w2 / (6.0 * gi), -ge, v / 3.0, gi_ + ucs},
-                            {gr_ - ucs, -gi, ge, gt * w2 / (6.0 * gr), v / 3.0, gr_ + ucs}};
-#pragma unroll
-    for (int a = 0; a < 6; ++a)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) E.R[a][c] = r[a][c];
-    E.lam[0] = u - cs;
-    E.lam[1] = u;
-    E.lam[2] = u;
-    E.lam[3] = u;
-    E.lam[4] = u;
-    E.lam[5] = u + cs;
-  } else {
-    double vcs = v * cs / 3.0;
-    const double r[6][6] = {{1.0, 0.0, 1.0, 1.0, 1.0, 1.0},
-                            {u, 1.0, u, u, u, u},
-                            {v - cs, 0.0, v, v, v, v + cs},
-                            {ge_ - vcs, u / 3.0, gt * w2 / (6.0 * ge), -gr, gi, ge_ + vcs},
-                            {gi_ - vcs, u / 3.0, gr, gt * w2 / (6.0 * gi), -ge, gi_ + vcs},
-                            {gr_ - vcs, u / 3.0, -gi, ge, gt * w2 / (6.0 * gr), gr_ + vcs}};
-#pragma unroll
-    for (int a = 0; a < 6; ++a)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) E.R[a][c] = r[a][c];
-    E.lam[0] = v - cs;
-    E.lam[1] = v;
-    E.lam[2] = v;
-    E.lam[3] = v;
-    E.lam[4] = v;
-    E.lam[5] = v + cs;
-  }
-}
-
-// The X-direction (DIR 0) eigensystem of build_eigen entry by entry, for
-// callers that need single rows of L or entries of R rather than the whole
-// 6x6 matrices (same expressions, hence the same values).
+// Roe state and the closed-form left/right eigenvectors of the X direction
+// of Stencil6::build_l_plain / build_r_plain (src/weno.rs:28-548), entry by
+// entry, in the reference's expressions: roe_x forms the Roe state of points
+// [3] and [2] (densities rho3 / rho2, their u, v, e_e, e_i, e_r), lx_row a
+// row of L (u-c, electron, ion, radiation, transverse, u+c), rx_entry an
+// entry of R, lamx an eigenvalue.
 struct RoeX {
   double u, v, w2, ee, ei, er, cs2, cs, h[3], b, ad;
 };
