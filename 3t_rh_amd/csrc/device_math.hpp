@@ -175,6 +175,37 @@ __device__ __forceinline__ double rx_entry(const Phys& ph, const RoeX& Q, int a,
   return j == 0 ? -ph.gi : ph.ge;
 }
 
+// Row i of R and row c of L for an index that differs between lanes: one
+// compile-time case per index (a select between constant fields would
+// otherwise become an indexed load and force the constants into scratch).
+__device__ __forceinline__ void rx_row_rt(const Phys& ph, const RoeX& Q, int i, double r[6]) {
+  switch (i) {
+#define RH3T_RX_ROW(a) \
+  case a:              \
+    for (int c = 0; c < 6; ++c) r[c] = rx_entry(ph, Q, a, c); \
+    break;
+    RH3T_RX_ROW(0)
+    RH3T_RX_ROW(1)
+    RH3T_RX_ROW(2)
+    RH3T_RX_ROW(3)
+    RH3T_RX_ROW(4)
+    default:
+      for (int c = 0; c < 6; ++c) r[c] = rx_entry(ph, Q, 5, c);
+#undef RH3T_RX_ROW
+  }
+}
+
+__device__ __forceinline__ void lx_row_rt(const Phys& ph, const RoeX& Q, int c, double row[6]) {
+  switch (c) {
+    case 0: lx_row(ph, Q, 0, row); break;
+    case 1: lx_row(ph, Q, 1, row); break;
+    case 2: lx_row(ph, Q, 2, row); break;
+    case 3: lx_row(ph, Q, 3, row); break;
+    case 4: lx_row(ph, Q, 4, row); break;
+    default: lx_row(ph, Q, 5, row);
+  }
+}
+
 __device__ __forceinline__ double lamx(const RoeX& Q, int c) {
   return c == 0 ? Q.u - Q.cs : (c == 5 ? Q.u + Q.cs : Q.u);
 }

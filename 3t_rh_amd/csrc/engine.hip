@@ -335,71 +335,100 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
     s_v[(k) * 6 + c] = res;
   }
   __syncthreads();
-  if (!active || lane != 0) return;
-
   const RoeX Qs = *s_q;
   const int kind = hg.kind[h];
   const double dist = hg.dist[h];
-  // Taylor expansion to the ghost point, sum_k dist^k / k! u^(k)
-  double ug[6];
-  bool ok = true, solved = true;
   if (!WALL) {
     // FarField, src/bc1.rs:829-924: incoming characteristics of u^(0) from
     // u_inf, of u^(k >= 1) zero.  Outflow, src/bc1.rs:760-828: the u - c
     // characteristic of u^(1) from the pressure relaxation when incoming.
-    double vinf[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (kind == RH3T_BC_FARFIELD) {
-      double ui[6];
+    // u^(k) = R rhs_k one lane per (k, component), then the Taylor
+    // expansion to the ghost point, sum_k dist^k / k! u^(k), one lane per
+    // component (the LDS of vk is free again).
+    double* const s_uk = s_vk;       // [k][i]
+    double* const s_ug = s_vk + 30;  // Taylor sums
+    double* const s_vin = s_vk + 36; // L u_inf (FarField)
+    double* const s_fix = s_vk + 42; // Outflow: flag, rhs_1[0]
+    const bool far = kind == RH3T_BC_FARFIELD;
+    if (active && lane < 6) {
+      if (far) {
+        double ui[6];
 #pragma unroll
-      for (int f = 0; f < 6; ++f) ui[f] = hg.bcstate[6L * h + f];
-      double uloc[6] = {ui[0], ui[1] * nx + ui[2] * ny, -ui[1] * ny + ui[2] * nx, ui[3], ui[4], ui[5]};
-      lx_apply(ph, Qs, uloc, vinf);
+        for (int f = 0; f < 6; ++f) ui[f] = hg.bcstate[6L * h + f];
+        double uloc[6] = {ui[0], ui[1] * nx + ui[2] * ny, -ui[1] * ny + ui[2] * nx, ui[3], ui[4], ui[5]};
+        double lr[6];
+        lx_row_rt(ph, Qs, lane, lr);
+        s_vin[lane] = dot6(lr, uloc);
+      } else {
+        double r[6], x[6];
+        rx_row_rt(ph, Qs, lane, r);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) x[c] = s_v[c];
+        s_uk[lane] = dot6(r, x);
+      }
     }
-    bool fix1 = false;
-    double rhs10 = 0.0;
-    double coef = 1.0;
-    for (int k = 0; k <= 4; ++k) {
-      double rhs[6], uk[6];
+    __syncthreads();
+    if (active && lane == 0 && !far) {
+      const double* uk = s_uk;
+      double rho0 = uk[0];
+      double un0 = uk[1] / rho0, ut0 = uk[2] / rho0;
+      double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
+      double ee1 = uk[3] / rho0 - w2 / 6.0;
+      double ei1 = uk[4] / rho0 - w2 / 6.0;
+      double er1 = uk[5] / rho0 - w2 / 6.0;
+      double p0 = rho0 * (ph.ge * ee1 + ph.gi * ei1 + ph.gr * er1);
+      double a2 = ph.gamma_e * ph.ge * ee1 + ph.gamma_i * ph.gi * ei1 + ph.gamma_r * ph.gr * er1;
+      double a = sqrt(a2);
+      double lambda1 = un0 - a;
+      s_fix[0] = 0.0;
+      if (lambda1 < 0.0) {
+        const double p_inf = hg.outflow[3L * h], sigma = hg.outflow[3L * h + 1], l_dom = hg.outflow[3L * h + 2];
+        double ml = fabs(un0 / a);
+        double l1 = sigma * (1.0 - sq(ml)) * a / l_dom * (p0 - p_inf);
+        s_fix[0] = 1.0;
+        s_fix[1] = l1 / lambda1;
+      }
+    }
+    __syncthreads();
+    if (active && lane < 30 && (far || lane >= 6)) {
+      const int k = lane / 6, i = lane % 6;
+      double rhs[6], r[6];
 #pragma unroll
       for (int m = 0; m < 6; ++m) rhs[m] = s_v[(k) * 6 + m];
-      if (kind == RH3T_BC_FARFIELD) {
+      if (far) {
 #pragma unroll
         for (int m = 0; m < 6; ++m)
-          if (lamx(Qs, m) < 0.0) rhs[m] = k == 0 ? vinf[m] : 0.0;
-      } else if (k == 1 && fix1) {
-        rhs[0] = rhs10;
+          if (lamx(Qs, m) < 0.0) rhs[m] = k == 0 ? s_vin[m] : 0.0;
+      } else if (k == 1 && s_fix[0] != 0.0) {
+        rhs[0] = s_fix[1];
       }
-      rx_apply(ph, Qs, rhs, uk);
-      if (k == 0) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) ug[c] = uk[c];
-        if (kind == RH3T_BC_OUTFLOW) {
-          double rho0 = uk[0];
-          double un0 = uk[1] / rho0, ut0 = uk[2] / rho0;
-          double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
-          double ee1 = uk[3] / rho0 - w2 / 6.0;
-          double ei1 = uk[4] / rho0 - w2 / 6.0;
-          double er1 = uk[5] / rho0 - w2 / 6.0;
-          double p0 = rho0 * (ph.ge * ee1 + ph.gi * ei1 + ph.gr * er1);
-          double a2 = ph.gamma_e * ph.ge * ee1 + ph.gamma_i * ph.gi * ei1 + ph.gamma_r * ph.gr * er1;
-          double a = sqrt(a2);
-          double lambda1 = un0 - a;
-          if (lambda1 < 0.0) {
-            const double p_inf = hg.outflow[3L * h], sigma = hg.outflow[3L * h + 1],
-                         l_dom = hg.outflow[3L * h + 2];
-            double ml = fabs(un0 / a);
-            double l1 = sigma * (1.0 - sq(ml)) * a / l_dom * (p0 - p_inf);
-            fix1 = true;
-            rhs10 = l1 / lambda1;
-          }
-        }
-      } else {
-        coef *= dist / (double)k;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) ug[c] = ug[c] + coef * uk[c];
-      }
+      rx_row_rt(ph, Qs, i, r);
+      s_uk[(k) * 6 + i] = dot6(r, rhs);
     }
-  } else {  // Wall-ILW, src/bc1.rs:691-756
+    __syncthreads();
+    if (active && lane < 6) {
+      double ug = s_uk[lane];
+      double coef = 1.0;
+      for (int k = 1; k <= 4; ++k) {
+        coef *= dist / (double)k;
+        ug = ug + coef * s_uk[(k) * 6 + lane];
+      }
+      s_ug[lane] = ug;
+    }
+    __syncthreads();
+    if (active && lane < 6) {  // rotation back to (x, y)
+      const int f = lane;
+      const double out = f == 1 ? s_ug[1] * nx - s_ug[2] * ny : (f == 2 ? s_ug[1] * ny + s_ug[2] * nx : s_ug[f]);
+      S[f * plane + hg.pad[h]] = out;
+      if (!isfinite(out)) raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
+    }
+    return;
+  }
+  if (!active || lane != 0) return;
+  // Taylor expansion to the ghost point, sum_k dist^k / k! u^(k)
+  double ug[6];
+  bool ok = true, solved = true;
+  {  // Wall-ILW, src/bc1.rs:691-756
     double L[6][6], R[6][6], uk[5][6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) lx_row(ph, Qs, c, L[c]);
