@@ -104,6 +104,8 @@ SIGNATURES = {
     "rh3t_problem_init_cylinder": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.POINTER(_P)]),
     "rh3t_problem_cylinder_state": (ctypes.c_int, [_P, ctypes.c_double, _DP]),
     "rh3t_problem_clone": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
+    "rh3t_problem_init_double_mach": (ctypes.c_int, [ctypes.c_double, ctypes.POINTER(_P)]),
+    "rh3t_problem_double_mach_state": (ctypes.c_int, [_P, _DP]),
     "rh3t_problem_set_bc_state": (ctypes.c_int, [_P, ctypes.c_int32, _DP]),
     "rh3t_problem_grid": (ctypes.c_int, [_P, ctypes.POINTER(c_grid)]),
     "rh3t_problem_counts": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
@@ -291,6 +293,20 @@ class Field:
         f = cls(None, None, None, _handle=h)
         f.mach = mach
         return f
+
+    @classmethod
+    def init_double_mach(cls, h=1.0 / 320.0):
+        """init::init_double_mach (src/init.rs:26-113) with mesh width h
+        (1/320 in the reference); ghosts built."""
+        hd = ctypes.c_void_p()
+        _check(lib().rh3t_problem_init_double_mach(float(h), ctypes.byref(hd)), "init_double_mach")
+        return cls(None, None, None, _handle=hd)
+
+    def double_mach_state(self):
+        """Initial state of init_double_mach (src/init.rs:97-104)."""
+        u = np.zeros((self.grid.nx, self.grid.ny, 6))
+        _check(lib().rh3t_problem_double_mach_state(self._h, _dptr(u)), "double_mach_state")
+        return u
 
     def clone(self):
         """Copy sharing the (geometry-only) heavy-ghost tables."""

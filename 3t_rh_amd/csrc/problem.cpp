@@ -961,6 +961,96 @@ int rh3t_problem_init_cylinder(int32_t nx, double mach, rh3t_problem** out) {
   });
 }
 
+namespace {
+
+// euler_to_three_energy, src/init.rs:7-24
+void euler_to_three_energy(double rho, double ux, double uy, double p, double gamma, double s[6]) {
+  double kinetic = 0.5 * rho * (ux * ux + uy * uy);
+  double e_total = p / (gamma - 1.0) + kinetic;
+  s[0] = rho;
+  s[1] = rho * ux;
+  s[2] = rho * uy;
+  s[3] = e_total / 3.0;
+  s[4] = e_total / 3.0;
+  s[5] = e_total / 3.0;
+}
+
+// The Mach-10 states of init_double_mach (src/init.rs:45-49).
+void double_mach_states(double pre[6], double post[6]) {
+  euler_to_three_energy(1.4, 0.0, 0.0, 1.0, 1.4, pre);
+  euler_to_three_energy(8.0, 8.25, 0.0, 116.5, 1.4, post);
+}
+
+}  // namespace
+
+int rh3t_problem_init_double_mach(double h, rh3t_problem** out) {
+  // src/init.rs:26-113 (h = 1/320 in the reference): the polygon
+  // A(-1/2 - sqrt3/12, 0) B(0, 0) C(23 sqrt3/12, 23/12) D(C.x, 23/12 +
+  // sqrt3/2) E(A.x, D.y), Constant(post) on AB and EA, the 30-degree Wall on
+  // BC, Constant(pre) on CD, ZerothOrder on DE.
+  if (!out || !(h > 0.0) || h > 0.25) return RH3T_ERR_ARG;
+  *out = nullptr;
+  return guarded([&]() {
+    const double sqrt3 = std::sqrt(3.0);
+    const double ax = -0.5 - sqrt3 / 12.0, ay = 0.0;
+    const double bx = 0.0, by = 0.0;
+    const double cx = 23.0 * sqrt3 / 12.0, cy = 23.0 / 12.0;
+    const double dx = cx, dy = 23.0 / 12.0 + sqrt3 / 2.0;
+    const double ex = ax, ey = dy;
+    const double lx = cx - ax, ly = dy;
+    rh3t_grid g;
+    g.nx = (int32_t)std::ceil(lx / h) + 1;
+    g.ny = (int32_t)std::ceil(ly / h) + 1;
+    g.dx = h;
+    g.dy = h;
+    g.x0 = ax;
+    g.y0 = 0.0;
+    double pre[6], post[6];
+    double_mach_states(pre, post);
+    rh3t_bc c_post, c_pre, wall, zeroth;
+    std::memset(&c_post, 0, sizeof(c_post));
+    c_post.kind = RH3T_BC_CONSTANT;
+    std::memcpy(c_post.state, post, sizeof(post));
+    c_pre = c_post;
+    std::memcpy(c_pre.state, pre, sizeof(pre));
+    std::memset(&wall, 0, sizeof(wall));
+    wall.kind = RH3T_BC_WALL;
+    std::memset(&zeroth, 0, sizeof(zeroth));
+    zeroth.kind = RH3T_BC_ZEROTH_ORDER;
+    const double pts[10] = {ax, ay, bx, by, cx, cy, dx, dy, ex, ey};
+    const rh3t_bc bcs[5] = {c_post, wall, c_pre, zeroth, c_post};
+    const double inner[8] = {-1002.0, -1002.0, -1001.0, -1002.0, -1001.0, -1001.0, -1002.0, -1001.0};
+    const rh3t_bc bc_inner[4] = {wall, wall, wall, wall};
+    rh3t_problem* pb = nullptr;
+    int st = rh3t_problem_create(&g, pts, 5, bcs, inner, 4, bc_inner, &pb);
+    if (st != RH3T_OK) return st;
+    st = rh3t_problem_build_ghosts(pb);
+    if (st != RH3T_OK) {
+      delete pb;
+      return st;
+    }
+    *out = pb;
+    return (int)RH3T_OK;
+  });
+}
+
+int rh3t_problem_double_mach_state(const rh3t_problem* pb, double* u) {
+  // src/init.rs:97-104: post-shock where x <= 0, pre-shock elsewhere, on
+  // the fluid cells (zero elsewhere)
+  if (!pb || !u) return RH3T_ERR_ARG;
+  double pre[6], post[6];
+  double_mach_states(pre, post);
+  const rh3t_grid& g = pb->p.grid;
+  for (int i = 0; i < g.nx; ++i)
+    for (int j = 0; j < g.ny; ++j) {
+      const size_t l = (size_t)i * g.ny + j;
+      const double x = g.x0 + (double)i * g.dx;
+      const double* s = x <= 0.0 ? post : pre;
+      for (int c = 0; c < 6; ++c) u[l * 6 + c] = pb->p.fluid[l] ? s[c] : 0.0;
+    }
+  return RH3T_OK;
+}
+
 int rh3t_problem_clone(const rh3t_problem* src, rh3t_problem** out) {
   if (!src || !out) return RH3T_ERR_ARG;
   *out = nullptr;

@@ -163,6 +163,17 @@ int rh3t_problem_build_ghosts(rh3t_problem* pb);
  * reference exactly).  The returned problem has its ghosts built. */
 int rh3t_problem_init_cylinder(int32_t nx, double mach, rh3t_problem** out);
 
+/* init::init_double_mach, src/init.rs:26-113, with the mesh width h as a
+ * parameter (1/320 reproduces the reference): the Mach-10 double Mach
+ * reflection polygon with its Constant / Wall / ZerothOrder sides.  The
+ * returned problem has its ghosts built. */
+int rh3t_problem_init_double_mach(double h, rh3t_problem** out);
+
+/* Initial state of init_double_mach (src/init.rs:97-104: post-shock where
+ * x <= 0, pre-shock elsewhere, on fluid cells; zero elsewhere), AoS
+ * nx*ny*6. */
+int rh3t_problem_double_mach_state(const rh3t_problem* pb, double* u_aos);
+
 /* Copy of a problem (fluid mask, BCs, ghost list).  The heavy-ghost tables
  * depend only on geometry and grid and are shared with the source, on the
  * host and, inside one rh3t_batch, on the device. */

@@ -148,6 +148,52 @@ def test_rk3_cylinder_steps(rh3t, cyl, arith, tol):
     s.close()
 
 
+@pytest.fixture(scope="module")
+def dmr(rh3t):
+    f = rh3t.Field.init_double_mach(1.0 / 40.0)
+    return f, f.double_mach_state(), ob.OracleProblem(f)
+
+
+@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
+def test_l_double_mach(rh3t, dmr, arith):
+    """init_double_mach at h = 1/40: the Wall (ILW) ghosts of the 30-degree
+    ramp, ZerothOrder on the top and Constant in/outflow, after 3 oracle
+    steps have moved the Mach-10 shock onto the ramp."""
+    f, u, p = dmr
+    _say(f"l() double Mach h=1/40 arith={arith}")
+    for _ in range(3):
+        u, _t = ob.rk3_step(p, u, 0.05 * ob.global_dt(p, u))
+    s = f.solver(arith=arith)
+    s.set_state(u)
+    rhs, gv = s.l(with_ghosts=True)
+    ref, gref = ob.l(p, u, with_ghosts=True)
+    gerr = ob.relerr(gv, gref)
+    err = ob.relerr(rhs, ref, f.fluid)
+    print("ghost relerr", gerr, "rhs relerr", err)
+    assert gerr.max() <= 1e-12
+    assert err.max() <= (TOL_EXACT if arith == 0 else TOL)
+    s.close()
+
+
+@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
+def test_rk3_double_mach_steps(rh3t, dmr, arith):
+    """Three RK3 steps from the same input with per-step resync."""
+    f, u, p = dmr
+    _say(f"rk3 double Mach 3 steps with per-step resync arith={arith}")
+    s = f.solver(arith=arith)
+    for step in range(3):
+        dt = 0.05 * ob.global_dt(p, u)
+        s.set_state(u, 0.0)
+        s.rk3_ssp(dt)
+        got, _ = s.get_state()
+        ref, _ = ob.rk3_step(p, u, dt)
+        err = ob.relerr(got, ref, f.fluid)
+        print("step", step, "relerr", err)
+        assert err.max() <= TOL
+        u = ref
+    s.close()
+
+
 def test_l_translating_shock(rh3t, workloads):
     _say("l() translating shock 128")
     f = workloads.translating_shock(128)

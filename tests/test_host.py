@@ -105,6 +105,47 @@ def test_cylinder_fluid_and_ghost_sets(rh3t, nx, nfluid, nghost):
     assert kinds == {rh3t.BC_FARFIELD, rh3t.BC_REFLECTIVE_WALL}
 
 
+def double_mach_polygon():
+    s3 = math.sqrt(3.0)
+    a = (-0.5 - s3 / 12.0, 0.0)
+    c = (23.0 * s3 / 12.0, 23.0 / 12.0)
+    d = (c[0], 23.0 / 12.0 + s3 / 2.0)
+    return [a, (0.0, 0.0), c, d, (a[0], d[1])]
+
+
+@pytest.mark.parametrize("h", [1.0 / 40.0, 1.0 / 320.0], ids=["h40", "h320"])
+def test_double_mach_fluid_and_ghost_sets(rh3t, h):
+    """init_double_mach (src/init.rs:26-113): grid size, fluid mask and
+    ghost set against the brute-force crossing rule and radius-4 cross
+    discovery; BC kinds; the initial shock at x = 0 (src/init.rs:97-104)."""
+    f = rh3t.Field.init_double_mach(h)
+    g = f.grid
+    pts = double_mach_polygon()
+    lx, ly = pts[2][0] - pts[0][0], pts[3][1]
+    assert (g.nx, g.ny) == (math.ceil(lx / h) + 1, math.ceil(ly / h) + 1)
+    assert (g.x0, g.y0, g.dx, g.dy) == (pts[0][0], 0.0, h, h)
+    inner = [(-1002.0, -1002.0), (-1001.0, -1002.0), (-1001.0, -1001.0), (-1002.0, -1001.0)]
+
+    def fluid_fn(i, j):
+        x = g.x0 + np.asarray(i, dtype=np.float64) * g.dx
+        y = g.y0 + np.asarray(j, dtype=np.float64) * g.dy
+        return is_fluid_poly(pts, x, y) & is_fluid_poly(inner, x, y, inside=False)
+
+    fl, ghosts = brute_ghosts(fluid_fn, g.nx, g.ny)
+    assert np.array_equal(f.fluid, fl)
+    gh = f.ghosts()
+    assert list(zip(gh["i"].tolist(), gh["j"].tolist())) == ghosts
+    assert set(gh["bc_kind"].tolist()) == {rh3t.BC_CONSTANT, rh3t.BC_WALL, rh3t.BC_ZEROTH_ORDER}
+    # Wall ghosts are the heavy ones; every one of them has its tables
+    assert f.counts()[2] == int(np.sum(gh["bc_kind"] == rh3t.BC_WALL))
+    u = f.double_mach_state()
+    x = g.x0 + np.arange(g.nx, dtype=np.float64)[:, None] * g.dx + 0.0 * np.arange(g.ny)[None, :]
+    post = x <= 0.0
+    assert np.all(u[fl & post, 0] == 8.0) and np.all(u[fl & ~post, 0] == 1.4)
+    assert np.all(u[fl & post, 1] == 8.0 * 8.25)
+    assert np.all(u[~fl] == 0.0)
+
+
 @pytest.mark.parametrize("nx,nfluid,nghost", [(201, 153084, 8204), (241, 220444, 9852)])
 def test_cylinder_counts_large(rh3t, nx, nfluid, nghost):
     f = rh3t.Field.init_cylinder(nx, 2.5)
