@@ -143,6 +143,31 @@ def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
             "kernels_ms_per_step": kern}
 
 
+def double_mach_leg(rh3t, h=1.0 / 320.0, steps=50):
+    """The reference's second benchmark, init_double_mach (src/init.rs:26-113)
+    at its own h = 1/320 (1270 x 892): the Mach-10 ramp with Wall (ILW)
+    ghosts, `steps` SSP-RK3 steps with dt = 0.05 dt_cfl, hipGraph replay."""
+    f = rh3t.Field.init_double_mach(h)
+    n_fluid = int(f.fluid.sum())
+    s = f.solver()
+    s.set_state(f.double_mach_state(), 0.0)
+    s.advance(2, 0.05)
+    s.profile(True)
+    s.advance(3, 0.05)
+    kern = {k: round(v[0] * v[1] / 3, 4) for k, v in s.profile_read().items() if v[1]}
+    s.profile(False)
+    s.advance(1, 0.05)  # graph captured again outside the timed region
+    t0 = time.perf_counter()
+    s.advance(steps, 0.05)
+    wall = time.perf_counter() - t0
+    counts = f.counts()
+    s.close()
+    return {"workload": f"init_double_mach h=1/{round(1 / h)}", "grid": [f.grid.nx, f.grid.ny],
+            "fluid_cells": n_fluid, "ghosts": counts[1], "wall_ghosts": counts[2], "steps": steps,
+            "ms_per_step": wall / steps * 1e3, "cell_stage_per_s": n_fluid * 3 * steps / wall,
+            "kernels_ms_per_step": kern}
+
+
 def _gpu_ms(profile):
     """Summed kernel time (ms) of a solver's per-kernel profile."""
     return sum(avg * cnt for avg, cnt in profile.values())
@@ -243,6 +268,7 @@ def main():
     ap.add_argument("--batched-solves", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
+    ap.add_argument("--no-double-mach", action="store_true", help="skip the init_double_mach leg")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the differentiation leg (for profiling it); prints its JSON object")
     ap.add_argument("--fd-dirs", type=int, default=32, help="central-difference directions of the adjoint leg")
@@ -335,6 +361,9 @@ def main():
     if not args.no_batched:
         log("batched leg")
         line["batched"] = batched_leg(rh3t, args.batched_solves, args.batched_steps)
+    if not args.no_double_mach:
+        log("double Mach leg")
+        line["double_mach"] = double_mach_leg(rh3t)
     if not args.no_adjoint:
         log("adjoint leg")
         line["adjoint"] = adjoint_leg(rh3t, fd_dirs=args.fd_dirs)
