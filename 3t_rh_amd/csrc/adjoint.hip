@@ -662,6 +662,18 @@ struct AdjWork {
   double* ent_n = nullptr;
   double* dts = nullptr;   // frozen dt schedule (device)
   int dts_cap = 0;
+  // the ghosts outside the fluid box are gathered on a second stream, beside
+  // the tile gather (default priority; ordered by the two events)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_iface = nullptr, ev_ghosts = nullptr;
+  ~AdjWork() {
+    if (side) {
+      (void)hipStreamSynchronize(side);
+      (void)hipStreamDestroy(side);
+    }
+    if (ev_iface) (void)hipEventDestroy(ev_iface);
+    if (ev_ghosts) (void)hipEventDestroy(ev_ghosts);
+  }
 };
 
 void free_adjoint(Engine& E) {
@@ -752,6 +764,11 @@ int init_adjoint(Engine& E) {
   if ((st = upload(E, &W->ent_pad, ent_pad))) return st;
   if ((st = upload(E, &W->ent_kind, ent_kind))) return st;
   if ((st = upload(E, &W->ent_n, ent_n))) return st;
+  if (W->n_out > 0) {
+    HIP_TRY(hipStreamCreateWithFlags(&W->side, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&W->ev_iface, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&W->ev_ghosts, hipEventDisableTiming));
+  }
   guard.w = nullptr;
   E.adj = W;
   // the gather needs the temperatures of every stage state
@@ -801,12 +818,25 @@ void launch_vjp(Engine& E, int buf, bool taped, const double* LAM, double rscale
     const int nby = (int)(((long)g.nx * (g.ny + 1) + kIfBx - 1) / kIfBx);
     k_adj_iface<<<nbx + nby, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys, nbx);
   }
+  if (W.n_out > 0) {
+    // ghost gather on the side stream, concurrently with the tile gather
+    // (it needs few registers and no LDS, so it fits beside it)
+    (void)hipEventRecord(W.ev_iface, E.stream);
+    (void)hipStreamWaitEvent(W.side, W.ev_iface, 0);
+    const hipStream_t main = E.stream;
+    E.stream = W.side;
+    {
+      Launcher L(E, K_ADJ_GATHER);
+      k_adj_gather_ghosts<<<(W.n_out + 63) / 64, 64, 0, E.stream>>>(A, E.phys, W.n_out, W.out_pads);
+    }
+    E.stream = main;
+    (void)hipEventRecord(W.ev_ghosts, W.side);
+  }
   {
     Launcher L(E, K_ADJ_GATHER);
     k_adj_gather<<<W.pg_blocks, dim3(kGx, kGy), 0, E.stream>>>(A, E.phys);
-    if (W.n_out > 0)
-      k_adj_gather_ghosts<<<(W.n_out + 63) / 64, 64, 0, E.stream>>>(A, E.phys, W.n_out, W.out_pads);
   }
+  if (W.n_out > 0) (void)hipStreamWaitEvent(E.stream, W.ev_ghosts, 0);
   if (W.nrows > 0) {
     Launcher L(E, K_ADJ_GHOST);
     k_adj_ghost_T<<<(W.nrows + 255) / 256, 256, 0, E.stream>>>(W.nrows, W.row_cell, W.row_ptr, W.ent_pad,
