@@ -71,4 +71,21 @@ def test_driver_on_device(rh3t, workloads, tmp_path):
     err = ob.relerr(un, u).max()
     print("final state vs oracle", err)
     assert err <= 1e-11
+    # the readers on the run's own output: the energy split of the last
+    # snapshot is that of the device state, and on a periodic box the
+    # totals of rho, the momenta and ee + ei + er stay put (the flux
+    # differences telescope, the noncon and exchange terms sum to zero)
+    pp = import_module("3t_rh_amd.postprocess")
+    paths = pp.list_snapshots(out)
+    rows = pp.energy_split_history(paths, str(tmp_path / "split.csv"))
+    assert [r[0] for r in rows] == [pp.read_snapshot(p).time for p in paths]
+    d = np.abs(un[..., 3] - un[..., 5])
+    assert rows[-1][2] == d.max()
+    assert abs(rows[-1][5] - d.mean()) <= 1e-14 * d.mean()
+    tots = np.array([pp.totals(p) for p in paths])
+    scale = np.abs(un).sum(axis=(0, 1))
+    cons = np.column_stack([tots[:, :3], tots[:, 3:].sum(axis=1)])
+    drift = np.abs(cons - cons[0]).max(axis=0) / np.array([*scale[:3], scale[3:].sum()])
+    print("conservation drift (rho, mx, my, E)", drift)
+    assert np.all(drift <= 1e-13)
     s.close()
