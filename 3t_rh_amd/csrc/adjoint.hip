@@ -47,8 +47,8 @@ struct AdjDesc {
   const double* S;     // stage state with its ghosts
   const double* D;     // its Derived planes (D_N)
   const double* LAM;   // cotangent of the stage output (fluid cells, 0 elsewhere)
-  double* XB;          // 36 planes over x interfaces
-  double* YB;          // 36 planes over y interfaces
+  double* XB;          // x interfaces: [q][interface][f], q the stencil point
+  double* YB;          // y interfaces: [q][interface][f]
   double* SB;          // 6 planes: state cotangents of ghost points
   double* OUT;         // 6 planes: result on fluid cells
   const double* X1;
@@ -107,7 +107,9 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
   const long step = DIR == 0 ? g.PY : 1;
   if (A.pt[c0] != 1 && A.pt[c0 - step] != 1) {
 #pragma unroll
-    for (int k = 0; k < 36; ++k) out[k * fpl + fidx] = 0.0;
+    for (int q = 0; q < 6; ++q)
+#pragma unroll
+      for (int f = 0; f < 6; ++f) out[(q * fpl + fidx) * 6 + f] = 0.0;
     return;
   }
   const double* S = A.S;
@@ -216,7 +218,7 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
     }
     derived_vjp(ph, st, db, sb);
 #pragma unroll
-    for (int f = 0; f < 6; ++f) out[(q * 6 + f) * fpl + fidx] = sb[f];
+    for (int f = 0; f < 6; ++f) out[(q * fpl + fidx) * 6 + f] = sb[f];
   }
 }
 
@@ -432,7 +434,7 @@ __device__ __forceinline__ void gather_point(const AdjDesc& A, const Phys& ph, i
       if (ix < 0 || ix > g.nx) continue;
       const long fi = (long)ix * g.ny + j;
 #pragma unroll
-      for (int f = 0; f < 6; ++f) sb[f] += A.XB[(q * 6 + f) * fxpl + fi];
+      for (int f = 0; f < 6; ++f) sb[f] += A.XB[(q * fxpl + fi) * 6 + f];
     }
   }
   if (i >= 0 && i < g.nx) {
@@ -442,7 +444,7 @@ __device__ __forceinline__ void gather_point(const AdjDesc& A, const Phys& ph, i
       if (jy < 0 || jy > g.ny) continue;
       const long fi = (long)i * (g.ny + 1) + jy;
 #pragma unroll
-      for (int f = 0; f < 6; ++f) sb[f] += A.YB[(q * 6 + f) * fypl + fi];
+      for (int f = 0; f < 6; ++f) sb[f] += A.YB[(q * fypl + fi) * 6 + f];
     }
   }
   double db[D_N];

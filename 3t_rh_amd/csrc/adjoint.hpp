@@ -364,8 +364,12 @@ __device__ __forceinline__ void reload_barrier() { __asm__ volatile("" ::: "memo
 // the kernel entry and held in spilled scalar registers throughout.
 template <class T>
 __device__ __forceinline__ const T* fresh(const T* p) {
-  __asm__ volatile("" : "+s"(p));
-  return p;
+  // laundered as a constant-address-space pointer, so that the loads
+  // through the result are inferred as constant (scalar) loads, not flat
+  using CT = const __attribute__((address_space(4))) T;
+  CT* q = (CT*)p;
+  __asm__ volatile("" : "+s"(q));
+  return (const T*)q;
 }
 
 // -------------------------------------------------- interface flux VJP

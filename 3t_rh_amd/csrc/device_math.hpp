@@ -12,6 +12,17 @@
 
 namespace rh3t_dev {
 
+// A pointer read from a descriptor in device memory is generic to the
+// compiler, and loads through it are flat loads (counted on both the vector
+// and the scalar/LDS wait counters).  The planes all live in global memory:
+// re-qualifying the pointer lets the loads and stores through it be global
+// ones.
+template <class T>
+__device__ __forceinline__ T* glb(T* p) {
+  return (T*)(__attribute__((address_space(1))) T*)p;
+}
+
+
 struct Phys {
   double gamma_e, gamma_i, gamma_r;
   double ge, gi, gr;  // gamma_k - 1.0

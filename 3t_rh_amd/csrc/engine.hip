@@ -496,15 +496,16 @@ __global__ __launch_bounds__(256) void k_derived(const SolveDesc* __restrict__ d
   const int s = find_solve(tiles, nsolve, blockIdx.x);
   const SolveDesc& sd = descs[s];
   const long p = (long)(blockIdx.x - tiles[s]) * blockDim.x + threadIdx.x;
-  if (p >= sd.g.plane || sd.pt[p] == 0) return;
-  const double* S = sd.U[buf];
+  if (p >= sd.g.plane || glb(sd.pt)[p] == 0) return;
+  const double* S = glb(sd.U[buf]);
+  double* __restrict__ Dout = glb(sd.D);
   double st[6], d[D_N];
 #pragma unroll
   for (int f = 0; f < 6; ++f) st[f] = S[f * sd.g.plane + p];
   derived_from_state(ph, st, d, nplanes > 8);
 #pragma unroll
   for (int f = 0; f < D_N; ++f)
-    if (f < nplanes) sd.D[f * sd.g.plane + p] = d[f];
+    if (f < nplanes) Dout[f * sd.g.plane + p] = d[f];
 }
 
 // ------------------------------------------------------- WENO flux
@@ -531,14 +532,15 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
   const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
   const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
   if (i >= ni || j >= nj) return;
-  const double* __restrict__ S = sd.U[buf];
-  const double* __restrict__ D = sd.D;
-  double* __restrict__ F = DIR == 0 ? sd.FX : sd.FY;
+  const double* __restrict__ S = glb(sd.U[buf]);
+  const double* __restrict__ D = glb(sd.D);
+  double* __restrict__ F = glb(DIR == 0 ? sd.FX : sd.FY);
+  const uint8_t* __restrict__ pt = glb(sd.pt);
   const long fpl = DIR == 0 ? (long)(g.nx + 1) * g.ny : (long)g.nx * (g.ny + 1);
   const long fidx = DIR == 0 ? (long)i * g.ny + j : (long)i * (g.ny + 1) + j;
   const long c0 = pad_index(g, i, j);
   const long step = DIR == 0 ? g.PY : 1;
-  if (sd.pt[c0] != 1 && sd.pt[c0 - step] != 1) {
+  if (pt[c0] != 1 && pt[c0 - step] != 1) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) F[f * fpl + fidx] = 0.0;
     return;
@@ -583,15 +585,16 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
   const long c0 = pad_index(g, i, j);
   const long lin = (long)i * g.ny + j;
   const long ncell = (long)g.nx * g.ny;
-  if (sd.pt[c0] != 1) {
+  double* __restrict__ RHS = glb(sd.RHS);
+  if (glb(sd.pt)[c0] != 1) {
     if (sa.mode == 0)
 #pragma unroll
-      for (int f = 0; f < 6; ++f) sd.RHS[f * ncell + lin] = 0.0;
+      for (int f = 0; f < 6; ++f) RHS[f * ncell + lin] = 0.0;
     return INFINITY;
   }
-  const double* __restrict__ D = sd.D;
-  const double* __restrict__ FX = sd.FX;
-  const double* __restrict__ FY = sd.FY;
+  const double* __restrict__ D = glb(sd.D);
+  const double* __restrict__ FX = glb(sd.FX);
+  const double* __restrict__ FY = glb(sd.FY);
   const long fxpl = (long)(g.nx + 1) * g.ny, fypl = (long)g.nx * (g.ny + 1);
   const long ixl = (long)i * g.ny + j, ixr = ixl + g.ny;
   const long iyb = (long)i * (g.ny + 1) + j, iyt = iyb + 1;
@@ -654,14 +657,14 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
   }
   if (sa.mode == 0) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) sd.RHS[f * ncell + lin] = r[f];
+    for (int f = 0; f < 6; ++f) RHS[f * ncell + lin] = r[f];
     return INFINITY;
   }
   // stage_update_rhs / stage_update_comb, src/main.rs:307-349
   const double dt = sd.sc->dt;
-  const double* __restrict__ base = sd.U[sa.base];
-  const double* __restrict__ add = sd.U[sa.add];
-  double* dst = sd.U[sa.dst];
+  const double* __restrict__ base = glb(sd.U[sa.base]);
+  const double* __restrict__ add = glb(sd.U[sa.add]);
+  double* dst = glb(sd.U[sa.dst]);
   double v[6];
   if (sa.mode == 1) {
 #pragma unroll
