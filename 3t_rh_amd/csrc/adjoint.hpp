@@ -359,19 +359,6 @@ __device__ __forceinline__ void roe_bar_final(const EigConst& K, const RoeV& R, 
 // interface_vjp in registers (it is re-read from L1/L2 instead).
 __device__ __forceinline__ void reload_barrier() { __asm__ volatile("" ::: "memory"); }
 
-// The same pointer, opaque to the compiler: the constants read through it
-// are loaded (scalar loads) where they are used instead of being hoisted to
-// the kernel entry and held in spilled scalar registers throughout.
-template <class T>
-__device__ __forceinline__ const T* fresh(const T* p) {
-  // laundered as a constant-address-space pointer, so that the loads
-  // through the result are inferred as constant (scalar) loads, not flat
-  using CT = const __attribute__((address_space(4))) T;
-  CT* q = (CT*)p;
-  __asm__ volatile("" : "+s"(q));
-  return (const T*)q;
-}
-
 // -------------------------------------------------- interface flux VJP
 
 // State and physical flux (flux_from_derived, src/state.rs:56-85) of the

@@ -520,7 +520,8 @@ __global__ __launch_bounds__(256) void k_derived(const SolveDesc* __restrict__ d
 template <int DIR, bool FAST>
 __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* __restrict__ descs,
                                                                const int* __restrict__ tiles, int nsolve,
-                                                               EigConst K, double eps, int buf) {
+                                                               const EigConst* __restrict__ Kp, double eps,
+                                                               int buf) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int s = find_solve(tiles, nsolve, t);
   const SolveDesc& sd = descs[s];
@@ -568,7 +569,7 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
   const double e3[3] = {D[D_EE * g.plane + c0], D[D_EI * g.plane + c0], D[D_ER * g.plane + c0]};
   const double e2[3] = {D[D_EE * g.plane + p2], D[D_EI * g.plane + p2], D[D_ER * g.plane + p2]};
   double out[6];
-  interface_flux<DIR, FAST>(K, eps, U, Fl, D[D_U * g.plane + c0], D[D_V * g.plane + c0], e3,
+  interface_flux<DIR, FAST>(Kp, eps, U, Fl, D[D_U * g.plane + c0], D[D_V * g.plane + c0], e3,
                             D[D_U * g.plane + p2], D[D_V * g.plane + p2], e2, out);
 #pragma unroll
   for (int f = 0; f < 6; ++f) F[f * fpl + fidx] = out[f];
@@ -924,16 +925,16 @@ void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture) {
   {
     Launcher L(E, K_FLUX_X);
     if (E.fast)
-      k_flux<0, true><<<total(E.t_fx), blk, 0, E.stream>>>(E.d_desc, E.tab(0), E.nsolve, E.eig, E.phys.eps, src);
+      k_flux<0, true><<<total(E.t_fx), blk, 0, E.stream>>>(E.d_desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
     else
-      k_flux<0, false><<<total(E.t_fx), blk, 0, E.stream>>>(E.d_desc, E.tab(0), E.nsolve, E.eig, E.phys.eps, src);
+      k_flux<0, false><<<total(E.t_fx), blk, 0, E.stream>>>(E.d_desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
   }
   {
     Launcher L(E, K_FLUX_Y);
     if (E.fast)
-      k_flux<1, true><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.eig, E.phys.eps, src);
+      k_flux<1, true><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps, src);
     else
-      k_flux<1, false><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.eig, E.phys.eps, src);
+      k_flux<1, false><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps, src);
   }
   {
     Launcher L(E, K_ASSEMBLE);
