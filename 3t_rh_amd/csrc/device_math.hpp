@@ -13,13 +13,19 @@
 namespace rh3t_dev {
 
 // A pointer read from a descriptor in device memory is generic to the
-// compiler, and loads through it are flat loads (counted on both the vector
-// and the scalar/LDS wait counters).  The planes all live in global memory:
-// re-qualifying the pointer lets the loads and stores through it be global
-// ones.
+// compiler, and loads through it are flat loads (a 64-bit VGPR address per
+// lane, counted on both the vector and the scalar/LDS wait counters).  The
+// planes all live in global memory: passing the pointer through an opaque
+// step as a global-address-space pointer (a plain cast round trip would be
+// folded away) lets the loads and stores through it be global ones, with
+// the base in SGPRs.  p must be uniform over the wavefront (it is held in
+// SGPRs).
 template <class T>
 __device__ __forceinline__ T* glb(T* p) {
-  return (T*)(__attribute__((address_space(1))) T*)p;
+  using GT = __attribute__((address_space(1))) T;
+  GT* q = (GT*)p;
+  __asm__("" : "+s"(q));
+  return (T*)q;
 }
 
 

@@ -215,7 +215,7 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
   const long row = hg.tab[h];
   const double d5r = lane < 30 ? hg.d5[5L * row + lane / 6] : 0.0;
   const SolveDesc& sd = descs[hg.sid[h]];
-  double* S = sd.U[buf];
+  double* S = sd.U[buf];  // flat: measured faster here than global
   const long plane = sd.g.plane;
   const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
   // build_local_plain, src/bc1.rs:294-312: rotate the nearest state to the
@@ -755,8 +755,8 @@ __global__ __launch_bounds__(256) void k_dt_partial(const SolveDesc* __restrict_
   double m = INFINITY;
   if (i < sd.g.nx && j < sd.g.ny) {
     const long p = pad_index(sd.g, i, j);
-    if (sd.pt[p] == 1) {
-      const double* S = sd.U[0];
+    if (glb(sd.pt)[p] == 1) {
+      const double* S = glb(sd.U[0]);
       double st[6];
 #pragma unroll
       for (int f = 0; f < 6; ++f) st[f] = S[f * sd.g.plane + p];
