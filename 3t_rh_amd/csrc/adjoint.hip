@@ -251,7 +251,7 @@ constexpr double kLap[7] = {2.0, -27.0, 270.0, -490.0, 270.0, -27.0, 2.0};
 // velocities, the scaled energy cotangents rb = rscale * LAM[3..6) and the
 // point types of the tile plus a kGh-point halo in LDS; the noncon and
 // diffusion line pullbacks read their 11-point lines from there.
-constexpr int kGx = 64, kGy = 4, kGh = 5;
+constexpr int kGx = 32, kGy = 8, kGh = 5;
 constexpr int kGr = kGy + 2 * kGh, kGc = kGx + 2 * kGh;
 enum { T_PE = 0, T_PI, T_PR, T_U, T_V, T_R3, T_R4, T_R5, T_N };
 struct GatherTile {
