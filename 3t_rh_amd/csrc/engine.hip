@@ -233,11 +233,19 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
     roe_x(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI],
           d1[D_ER], Q);
   }
-  if (active && lane < 55) {
-    long q = hg.stencil[55L * row + lane];
+  // lanes 0..54: L . u at the E_r points; lane 55 of a FarField ghost:
+  // L . u_inf (src/bc1.rs:829-924) into the free slot 55 of each row
+  const bool far = !WALL && hg.kind[h] == RH3T_BC_FARFIELD;
+  if (active && (lane < 55 || (lane == 55 && far))) {
     double s[6];
+    if (lane < 55) {
+      const long q = hg.stencil[55L * row + lane];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) s[f] = S[f * plane + q];
+      for (int f = 0; f < 6; ++f) s[f] = S[f * plane + q];
+    } else {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) s[f] = hg.bcstate[6L * h + f];
+    }
     double u6[6] = {s[0], s[1] * nx + s[2] * ny, -s[1] * ny + s[2] * nx, s[3], s[4], s[5]};
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -347,25 +355,15 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
     // component (the LDS of vk is free again).
     double* const s_uk = s_vk;       // [k][i]
     double* const s_ug = s_vk + 30;  // Taylor sums
-    double* const s_vin = s_vk + 36; // L u_inf (FarField)
     double* const s_fix = s_vk + 42; // Outflow: flag, rhs_1[0]
-    const bool far = kind == RH3T_BC_FARFIELD;
-    if (active && lane < 6) {
-      if (far) {
-        double ui[6];
+    // FarField: L u_inf is slot 55 of the data rows (formed with the E_r
+    // data); Outflow: u^(0) = R V^(0), one lane per component
+    if (active && lane < 6 && !far) {
+      double r[6], x[6];
+      rx_row_rt(ph, Qs, lane, r);
 #pragma unroll
-        for (int f = 0; f < 6; ++f) ui[f] = hg.bcstate[6L * h + f];
-        double uloc[6] = {ui[0], ui[1] * nx + ui[2] * ny, -ui[1] * ny + ui[2] * nx, ui[3], ui[4], ui[5]};
-        double lr[6];
-        lx_row_rt(ph, Qs, lane, lr);
-        s_vin[lane] = dot6(lr, uloc);
-      } else {
-        double r[6], x[6];
-        rx_row_rt(ph, Qs, lane, r);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) x[c] = s_v[c];
-        s_uk[lane] = dot6(r, x);
-      }
+      for (int c = 0; c < 6; ++c) x[c] = s_v[c];
+      s_uk[lane] = dot6(r, x);
     }
     __syncthreads();
     if (active && lane == 0 && !far) {
@@ -398,7 +396,7 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
       if (far) {
 #pragma unroll
         for (int m = 0; m < 6; ++m)
-          if (lamx(Qs, m) < 0.0) rhs[m] = k == 0 ? s_vin[m] : 0.0;
+          if (lamx(Qs, m) < 0.0) rhs[m] = k == 0 ? s_data[(m) * 56 + 55] : 0.0;
       } else if (k == 1 && s_fix[0] != 0.0) {
         rhs[0] = s_fix[1];
       }
