@@ -38,10 +38,11 @@ FLOP_PER_CELL_STAGE = 5250
 BYTES_PER_CELL_STAGE = 128  # stage 1: read U^k, write U (96 B); stages 2-3: + U^n (144 B)
 # HBM traffic of one k_flux<0> launch on the 2048^2 box, from two rocprofv3
 # --pmc passes over this bench (FETCH_SIZE and WRITE_SIZE, KiB per dispatch,
-# mean of 15 dispatches), each scaled by what the same counter reads for a
-# plain 1 GiB device-to-device copy (`--copy-calibration`: FETCH_SIZE reads
-# 524,304 KiB -> x2.0, WRITE_SIZE 1,048,576 KiB -> x1.0).
-FLUX_X_FETCH_KIB, FLUX_X_WRITE_KIB = 239769, 229192
+# mean of 15 dispatches, four significant digits), each scaled by what the
+# same counter reads for a plain 1 GiB device-to-device copy
+# (`--copy-calibration`: FETCH_SIZE reads 524,304 KiB -> x2.0, WRITE_SIZE
+# 1,048,576 KiB -> x1.0).
+FLUX_X_FETCH_KIB, FLUX_X_WRITE_KIB = 237200, 196700
 COPY_FETCH_SCALE, COPY_WRITE_SCALE = 2.0, 1.0
 FLUX_X_TRAFFIC_BYTES = (FLUX_X_FETCH_KIB * COPY_FETCH_SCALE + FLUX_X_WRITE_KIB * COPY_WRITE_SCALE) * 1024
 
