@@ -111,46 +111,54 @@ __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict
   if (!ok) raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)cg.gid[k]);
 }
 
-__device__ __forceinline__ void matvec6(const double M[6][6], const double x[6], double y[6]) {
-#pragma unroll
-  for (int i = 0; i < 6; ++i) y[i] = dot6(M[i], x);
-}
-
 // solve6 (LAPACK dgesv): LU with partial pivoting, first maximal pivot.
-__device__ bool solve6(const double A0[6][6], const double b0[6], double x[6]) {
+// Fully unrolled: the row interchange is a predicated exchange with
+// compile-time row indices, so the matrix stays in registers.
+__device__ __forceinline__ bool solve6(const double A0[6][6], const double b0[6], double x[6]) {
   double A[6][6], b[6];
+#pragma unroll
   for (int i = 0; i < 6; ++i) {
     b[i] = b0[i];
+#pragma unroll
     for (int j = 0; j < 6; ++j) A[i][j] = A0[i][j];
   }
+#pragma unroll
   for (int c = 0; c < 6; ++c) {
     int p = c;
     double best = fabs(A[c][c]);
+#pragma unroll
     for (int r = c + 1; r < 6; ++r)
       if (fabs(A[r][c]) > best) {
         best = fabs(A[r][c]);
         p = r;
       }
     if (best == 0.0) return false;
-    if (p != c) {
-      for (int q = 0; q < 6; ++q) {
-        double t = A[c][q];
-        A[c][q] = A[p][q];
-        A[p][q] = t;
+#pragma unroll
+    for (int r = c + 1; r < 6; ++r)
+      if (r == p) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          double t = A[c][q];
+          A[c][q] = A[r][q];
+          A[r][q] = t;
+        }
+        double t = b[c];
+        b[c] = b[r];
+        b[r] = t;
       }
-      double t = b[c];
-      b[c] = b[p];
-      b[p] = t;
-    }
+#pragma unroll
     for (int r = c + 1; r < 6; ++r) {
       double f = A[r][c] / A[c][c];
       A[r][c] = f;
+#pragma unroll
       for (int q = c + 1; q < 6; ++q) A[r][q] -= f * A[c][q];
       b[r] -= f * b[c];
     }
   }
+#pragma unroll
   for (int r = 5; r >= 0; --r) {
     double s = b[r];
+#pragma unroll
     for (int q = r + 1; q < 6; ++q) s -= A[r][q] * x[q];
     x[r] = s / A[r][r];
   }
@@ -158,7 +166,7 @@ __device__ bool solve6(const double A0[6][6], const double b0[6], double x[6]) {
 }
 
 // y = R x and y = L x of the X eigensystem with the entries formed on the fly
-// (matvec6's order).
+// (one dot6 per row, rows in order).
 __device__ __forceinline__ void rx_apply(const Phys& ph, const RoeX& Q, const double x[6], double y[6]) {
 #pragma unroll
   for (int a = 0; a < 6; ++a) {
@@ -427,17 +435,13 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
   double ug[6];
   bool ok = true, solved = true;
   {  // Wall-ILW, src/bc1.rs:691-756
-    double L[6][6], R[6][6], uk[5][6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) lx_row(ph, Qs, c, L[c]);
-#pragma unroll
-    for (int a = 0; a < 6; ++a)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) R[a][c] = rx_entry(ph, Qs, a, c);
+    double uk[5][6];
     double A[6][6], rhs[6];
-    for (int i = 0; i < 6; ++i)
-      for (int j = 0; j < 6; ++j) A[i][j] = L[i][j];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) lx_row(ph, Qs, c, A[c]);
+#pragma unroll
     for (int c = 0; c < 6; ++c) A[0][c] = c == 1 ? 1.0 : 0.0;
+#pragma unroll
     for (int m = 0; m < 6; ++m) rhs[m] = s_v[(0) * 6 + m];
     rhs[0] = 0.0;
     solved = solve6(A, rhs, uk[0]);
@@ -448,13 +452,17 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
     A[0][3] = -ph.ge;
     A[0][4] = -ph.gi;
     A[0][5] = -ph.gr;
+#pragma unroll
     for (int m = 0; m < 6; ++m) rhs[m] = s_v[(1) * 6 + m];
     rhs[0] = 0.0;
     solved = solved && solve6(A, rhs, uk[1]);
-    for (int k = 2; k <= 4; ++k) matvec6(R, (s_v + (k) * 6), uk[k]);
+    // u^(k) = R V^(k), k >= 2
+#pragma unroll
+    for (int k = 2; k <= 4; ++k) rx_apply(ph, Qs, s_v + (k) * 6, uk[k]);
 #pragma unroll
     for (int c = 0; c < 6; ++c) ug[c] = uk[0][c];
     double coef = 1.0;
+#pragma unroll
     for (int k = 1; k <= 4; ++k) {
       coef *= dist / (double)k;
 #pragma unroll
