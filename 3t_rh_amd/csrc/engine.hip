@@ -265,62 +265,57 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
   if (active && lane == 0) *s_q = Q;
   __syncthreads();
   if (active) {
-    // One lane per row a of S_r (the 55 E_r points, r = 0..4) applies the row
-    // to all six characteristic data vectors, and one lane per (k, r), k <= r,
-    // applies row k of w_r: every table entry is read once per ghost.  Sums
-    // run over b / a in increasing order, as in weno_extrapolation_pre.
+    // One lane per table row applies it to all six characteristic data
+    // vectors: lanes 0..54 row a of S_r (the 55 E_r points), lanes 55..63
+    // row k of w_r for r = 4 (k = 0..4) and r = 3 (k = 0..3), all in one
+    // loop; then lanes 0..5 the six rows of w_r, r <= 2.  Every table entry
+    // is read once per ghost.  Sums run over b / a in increasing order, as
+    // in weno_extrapolation_pre.
     const double* Sh = hg.S + 979L * row;
     const double* Wh = hg.W + 225L * row;
-    if (lane < 55) {
-      int r = 0;
-      while (lane >= kStOff[r + 1]) ++r;
-      const int m = (r + 1) * (r + 1), a = lane - kStOff[r];
-      const double* srow = Sh + kSOffD[r] + a * m;
-      const double* dat = s_data + kStOff[r];
-      double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      // chunks of 5 loads in flight; the sum still runs over b in order
+    auto apply_row = [&](const double* trow, int m, const double* dat, double acc[6]) {
+      // chunks of 5 loads in flight; the sum still runs in order
       for (int b0 = 0; b0 < m; b0 += 5) {
-        double sv[5];
+        double tv[5];
 #pragma unroll
-        for (int t = 0; t < 5; ++t) sv[t] = b0 + t < m ? srow[b0 + t] : 0.0;
+        for (int t = 0; t < 5; ++t) tv[t] = b0 + t < m ? trow[b0 + t] : 0.0;
 #pragma unroll
         for (int t = 0; t < 5; ++t)
           if (b0 + t < m)
 #pragma unroll
-            for (int c = 0; c < 6; ++c) acc[c] += sv[t] * dat[c * 56 + b0 + t];
+            for (int c = 0; c < 6; ++c) acc[c] += tv[t] * dat[c * 56 + b0 + t];
       }
-#pragma unroll
-      for (int c = 0; c < 6; ++c) s_part[(c) * 56 + lane] = s_data[(c) * 56 + lane] * acc[c];
-    }
-    // (k, r) pairs: lanes 55..63 take r = 4 (k = 0..4) and r = 3 (k = 0..3),
-    // lanes 0..5 the six pairs of r <= 2 (after their light S rows)
-    int k = -1, r = 0;
-    if (lane >= 55) {
-      const int q = lane - 55;
-      r = q < 5 ? 4 : 3;
-      k = q < 5 ? q : q - 5;
-    } else if (lane < 6) {
-      const int rr[6] = {2, 2, 2, 1, 1, 0}, kk[6] = {0, 1, 2, 0, 1, 0};
-      r = rr[lane];
-      k = kk[lane];
-    }
-    if (k >= 0) {
-      const int m = (r + 1) * (r + 1);
-      const double* wv = Wh + kWOffD[r] + k * m;
-      const double* dat = s_data + kStOff[r];
-      double vk[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      for (int a0 = 0; a0 < m; a0 += 5) {
-        double wa[5];
-#pragma unroll
-        for (int t = 0; t < 5; ++t) wa[t] = a0 + t < m ? wv[a0 + t] : 0.0;
-#pragma unroll
-        for (int t = 0; t < 5; ++t)
-          if (a0 + t < m)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) vk[c] += wa[t] * dat[c * 56 + a0 + t];
+    };
+    {
+      int r, k = 0;
+      const double* trow;
+      if (lane < 55) {
+        r = 0;
+        while (lane >= kStOff[r + 1]) ++r;
+        trow = Sh + kSOffD[r] + (lane - kStOff[r]) * (r + 1) * (r + 1);
+      } else {
+        const int q = lane - 55;
+        r = q < 5 ? 4 : 3;
+        k = q < 5 ? q : q - 5;
+        trow = Wh + kWOffD[r] + k * (r + 1) * (r + 1);
       }
+      double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      apply_row(trow, (r + 1) * (r + 1), s_data + kStOff[r], acc);
+      if (lane < 55) {
 #pragma unroll
-      for (int c = 0; c < 6; ++c) s_vk[((k) * 6 + c) * 5 + r] = vk[c];
+        for (int c = 0; c < 6; ++c) s_part[(c) * 56 + lane] = s_data[(c) * 56 + lane] * acc[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s_vk[((k) * 6 + c) * 5 + r] = acc[c];
+      }
+    }
+    if (lane < 6) {  // (r, k) = (2,0) (2,1) (2,2) (1,0) (1,1) (0,0)
+      const int r = lane < 3 ? 2 : (lane < 5 ? 1 : 0);
+      const int k = lane < 3 ? lane : (lane < 5 ? lane - 3 : 0);
+      double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      apply_row(Wh + kWOffD[r] + k * (r + 1) * (r + 1), (r + 1) * (r + 1), s_data + kStOff[r], acc);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s_vk[((k) * 6 + c) * 5 + r] = acc[c];
     }
     // v_k = 0 for k > r (weno_extrapolation_pre)
     if (lane >= 6 && lane < 16) {
