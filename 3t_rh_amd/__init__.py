@@ -144,6 +144,7 @@ SIGNATURES = {
     "rh3t_adjoint_destroy": (None, [_P]),
     "rh3t_adjoint_forward": (ctypes.c_int, [_P, ctypes.c_int32, _DP]),
     "rh3t_adjoint_backward": (ctypes.c_int, [_P, _DP, _DP, _DP]),
+    "rh3t_adjoint_device_ms": (ctypes.c_int, [_P, _DP, _DP]),
     "rh3t_last_error": (ctypes.c_int, [_P, ctypes.POINTER(c_error)]),
     "rh3t_profile": (ctypes.c_int, [_P, ctypes.c_int32]),
     "rh3t_profile_read": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, _DP,
@@ -534,6 +535,15 @@ class Adjoint:
         if st != OK:
             self.solver._raise(st, "adjoint backward")
         return out, pg
+
+    def device_ms(self):
+        """(forward, backward) device time in ms of the last forward and
+        backward, without the backward's host transfers."""
+        f, b = ctypes.c_double(), ctypes.c_double()
+        st = lib().rh3t_adjoint_device_ms(self._h, ctypes.byref(f), ctypes.byref(b))
+        if st != OK:
+            self.solver._raise(st, "adjoint device_ms")
+        return f.value, b.value
 
 
 def _profile_read(fn, h):

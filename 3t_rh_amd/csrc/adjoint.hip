@@ -942,6 +942,10 @@ struct rh3t_adjoint {
   hipStream_t side = nullptr;    // recompute stream (default priority)
   hipEvent_t ev_start = nullptr, ev_free[2] = {nullptr, nullptr}, ev_ready[2] = {nullptr, nullptr};
   hipEvent_t ev_join = nullptr;
+  // device time of the last forward / backward, from after the host inputs
+  // are on the device to before the results leave it
+  hipEvent_t t_fwd[2] = {nullptr, nullptr}, t_bwd[2] = {nullptr, nullptr};
+  bool timed_fwd = false, timed_bwd = false;
   std::vector<void*> allocs;
 };
 
@@ -1084,6 +1088,8 @@ int rh3t_adjoint_create(rh3t_ctx* ctx, int32_t checkpoint_every, rh3t_adjoint** 
     hipEvent_t* evs[6] = {&A->ev_start, &A->ev_free[0], &A->ev_free[1], &A->ev_ready[0], &A->ev_ready[1],
                           &A->ev_join};
     for (int q = 0; q < 6 && e == hipSuccess; ++q) e = hipEventCreateWithFlags(evs[q], hipEventDisableTiming);
+    hipEvent_t* tev[4] = {&A->t_fwd[0], &A->t_fwd[1], &A->t_bwd[0], &A->t_bwd[1]};
+    for (int q = 0; q < 4 && e == hipSuccess; ++q) e = hipEventCreate(tev[q]);
     if (e != hipSuccess) {
       std::snprintf(E.message, sizeof(E.message), "stream/event: %s", hipGetErrorString(e));
       st = RH3T_ERR_HIP;
@@ -1105,7 +1111,8 @@ void rh3t_adjoint_destroy(rh3t_adjoint* A) {
     (void)hipStreamSynchronize(A->side);
     (void)hipStreamDestroy(A->side);
   }
-  for (hipEvent_t ev : {A->ev_start, A->ev_free[0], A->ev_free[1], A->ev_ready[0], A->ev_ready[1], A->ev_join})
+  for (hipEvent_t ev : {A->ev_start, A->ev_free[0], A->ev_free[1], A->ev_ready[0], A->ev_ready[1], A->ev_join,
+                        A->t_fwd[0], A->t_fwd[1], A->t_bwd[0], A->t_bwd[1]})
     if (ev) (void)hipEventDestroy(ev);
   for (void* p : A->allocs) (void)hipFree(p);
   delete A;
@@ -1161,6 +1168,7 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
   HIP_TRY(hipEventRecord(A->ev_join, A->side));
   HIP_TRY(hipStreamWaitEvent(E.stream, A->ev_join, 0));
   HIP_TRY(hipMemcpyAsync(E.adj->dts, dt, nsteps * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  HIP_TRY(hipEventRecord(A->t_fwd[0], E.stream));
   HIP_TRY(hipMemcpyAsync(A->ckpt[0], d.U[0], plane6 * sizeof(double), hipMemcpyDeviceToDevice, E.stream));
   for (int n = 0; n < nsteps; ++n) {
     k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, E.adj->dts, n, 1);
@@ -1172,6 +1180,8 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
     launch_stage(E, 2, kStage3);
   }
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(A->t_fwd[1], E.stream));
+  A->timed_fwd = true;
   // the last two segments are on the tape
   A->tape_of[(nck - 1) % 2] = nck - 1;
   if (nck >= 2) A->tape_of[(nck - 2) % 2] = nck - 2;
@@ -1216,6 +1226,7 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
   double* lam = W.lam[0];
   double* nxt = W.lam[3];
   if ((st = upload_cot(E, lam_final, lam))) return st;
+  HIP_TRY(hipEventRecord(A->t_bwd[0], main));
   for (int k = nseg - 1; k >= 0; --k) {
     const int n0 = k * A->every, n1 = std::min(A->nsteps, n0 + A->every);
     const int set = k % 2;
@@ -1236,10 +1247,26 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(A->ev_join, A->side));
   HIP_TRY(hipStreamWaitEvent(main, A->ev_join, 0));
+  HIP_TRY(hipEventRecord(A->t_bwd[1], main));
+  A->timed_bwd = true;
   if ((st = download_cot(E, lam, lam_initial, pgrad))) return st;
   // the recompute's stage 3 overwrote the dt minima
   E.dt_partials_valid = false;
   return check_errors(E);
+}
+
+int rh3t_adjoint_device_ms(const rh3t_adjoint* A, double* forward_ms, double* backward_ms) {
+  if (!A || !forward_ms || !backward_ms || !A->timed_fwd || !A->timed_bwd) return RH3T_ERR_ARG;
+  Engine& E = A->ctx->e;
+  HIP_TRY(hipSetDevice(E.device));
+  float f = 0.0f, b = 0.0f;
+  HIP_TRY(hipEventSynchronize(A->t_fwd[1]));
+  HIP_TRY(hipEventSynchronize(A->t_bwd[1]));
+  HIP_TRY(hipEventElapsedTime(&f, A->t_fwd[0], A->t_fwd[1]));
+  HIP_TRY(hipEventElapsedTime(&b, A->t_bwd[0], A->t_bwd[1]));
+  *forward_ms = f;
+  *backward_ms = b;
+  return RH3T_OK;
 }
 
 }  // extern "C"

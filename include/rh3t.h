@@ -346,6 +346,15 @@ int rh3t_adjoint_forward(rh3t_adjoint* adj, int32_t nsteps, const double* dt);
  * sweep of the segment after it runs. */
 int rh3t_adjoint_backward(rh3t_adjoint* adj, const double* lam_final,
                           double* lam_initial, double* pgrad);
+/* Device time in ms of the last forward and the last backward (HIP events
+ * on the context's stream, both streams joined): from after the host inputs
+ * are on the device to before the results are copied back, so the
+ * backward's cotangent upload and gradient download are not included.  No
+ * counterpart in the reference (which has no reverse mode); for measuring
+ * the adjoint's cost against the forward's with device-resident data.
+ * RH3T_ERR_ARG before one forward and one backward have run. */
+int rh3t_adjoint_device_ms(const rh3t_adjoint* adj, double* forward_ms,
+                           double* backward_ms);
 
 #ifdef __cplusplus
 }

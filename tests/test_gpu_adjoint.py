@@ -147,6 +147,36 @@ def test_heavy_ghosts_are_rejected(rh3t, workloads):
     s.close()
 
 
+def test_adjoint_device_timing(rh3t, workloads):
+    """rh3t_adjoint_device_ms is refused until a forward and a backward have
+    run, then gives positive device times within the wall clock around each
+    call (the backward's host transfers fall outside its events)."""
+    import time
+
+    _say("adjoint device timing, box 64, 6 steps")
+    f, u, _p = _case(workloads, "periodic", 64)
+    s = f.solver(physics=rh3t.Physics(**KW_ACTIVE))
+    dts = _dts(s, u, 6)
+    adj = s.adjoint(3)
+    with pytest.raises(rh3t.RH3TError) as ei:
+        adj.device_ms()
+    assert ei.value.status == rh3t.ERR_ARG
+    s.set_state(u)
+    t0 = time.perf_counter()
+    adj.forward(dts)
+    wall_f = (time.perf_counter() - t0) * 1e3
+    lam = np.random.default_rng(4).standard_normal(u.shape)
+    lam[~f.fluid] = 0.0
+    t0 = time.perf_counter()
+    adj.backward(lam)
+    wall_b = (time.perf_counter() - t0) * 1e3
+    dev_f, dev_b = adj.device_ms()
+    print(f"forward device {dev_f:.3f} ms (wall {wall_f:.3f}), backward device {dev_b:.3f} ms (wall {wall_b:.3f})")
+    assert 0.0 < dev_f <= wall_f and 0.0 < dev_b <= wall_b
+    adj.close()
+    s.close()
+
+
 def test_autograd_matches_central_differences(rh3t, workloads):
     """The differentiation workload through torch.autograd, against central
     differences of the GPU forward on 32 seeded directions (seeds
