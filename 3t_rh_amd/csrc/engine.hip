@@ -478,9 +478,10 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
     raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
 }
 
-// FarField / Outflow: four wavefronts per SIMD (128 VGPRs) to hide the
-// latency of the table reads
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ghost_open(
+// FarField / Outflow: five wavefronts per SIMD (96 VGPRs; a few values go
+// to scratch) to hide the latency of the table reads; measured faster than
+// four (110 VGPRs, no scratch) and than six
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_ghost_open(
     const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
   ghost_heavy<false>(descs, hg, ph, buf, phase, h0, count);
 }
