@@ -288,6 +288,8 @@ def main():
     ap.add_argument("--fd-dirs", type=int, default=32, help="central-difference directions of the adjoint leg")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-vs-plain leg (for profiling it); prints its JSON object")
+    ap.add_argument("--double-mach-only", action="store_true",
+                    help="run only the init_double_mach leg (for profiling it); prints its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (three launches), the reference for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -313,6 +315,9 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(rh3t, fd_dirs=args.fd_dirs)}), flush=True)
+        return
+    if args.double_mach_only:
+        print(json.dumps({"double_mach": double_mach_leg(rh3t)}), flush=True)
         return
     n = args.size
     log(f"building {n}x{n} periodic box")
