@@ -595,7 +595,7 @@ __global__ __launch_bounds__(256) void k_adj_param(const double* __restrict__ PG
   for (int k = 0; k < kNParam; ++k) red[k][threadIdx.x] = s[k];
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
+    if ((int)threadIdx.x < w)
 #pragma unroll
       for (int k = 0; k < kNParam; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
     __syncthreads();

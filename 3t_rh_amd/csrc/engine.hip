@@ -165,8 +165,8 @@ __device__ __forceinline__ bool solve6(const double A0[6][6], const double b0[6]
   return true;
 }
 
-// y = R x and y = L x of the X eigensystem with the entries formed on the fly
-// (one dot6 per row, rows in order).
+// y = R x of the X eigensystem with the entries formed on the fly (one dot6
+// per row, rows in order).
 __device__ __forceinline__ void rx_apply(const Phys& ph, const RoeX& Q, const double x[6], double y[6]) {
 #pragma unroll
   for (int a = 0; a < 6; ++a) {
@@ -174,15 +174,6 @@ __device__ __forceinline__ void rx_apply(const Phys& ph, const RoeX& Q, const do
 #pragma unroll
     for (int c = 0; c < 6; ++c) r[c] = rx_entry(ph, Q, a, c);
     y[a] = dot6(r, x);
-  }
-}
-
-__device__ __forceinline__ void lx_apply(const Phys& ph, const RoeX& Q, const double x[6], double y[6]) {
-#pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    double r[6];
-    lx_row(ph, Q, c, r);
-    y[c] = dot6(r, x);
   }
 }
 
@@ -347,7 +338,6 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
   }
   __syncthreads();
   const RoeX Qs = *s_q;
-  const int kind = hg.kind[h];
   const double dist = hg.dist[h];
   if (!WALL) {
     // FarField, src/bc1.rs:829-924: incoming characteristics of u^(0) from
