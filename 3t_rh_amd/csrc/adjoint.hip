@@ -6,9 +6,10 @@
 //   OUT = a LAM + b1 X1 + b2 X2 + c (dl/dS)^T LAM
 // in gather form (every output written by exactly one lane, every sum in a
 // fixed order: results are bitwise reproducible):
-//   k_adj_iface      per interface (x and y): the pullback of the characteristic WENO5
-//                    flux (src/weno.rs:685-802) through the eigensystem,
-//                    the physical fluxes and the Derived values of its 6
+//   k_adj_iface      per interface (x and y; a lane pair each): the
+//                    pullback of the characteristic WENO5 flux
+//                    (src/weno.rs:685-802) through the eigensystem, the
+//                    physical fluxes and the Derived values of its 6
 //                    stencil points (36 state cotangents per interface,
 //                    stored)
 //   k_adj_gather     per fluid cell / ghost: sums its 12 interface
@@ -76,50 +77,59 @@ __device__ __forceinline__ void roe_average(const double* S, const double* D, lo
   }
 }
 
-// Interface flux pullback, one lane per interface, in two phases.
-// Characteristic phase: the WENO5 / local Lax-Friedrichs / eigenvector
-// pullback of every characteristic (src/weno.rs:685-802) with the 6-point
-// stencil in registers; the split-flux cotangents (10 per characteristic)
-// go to LDS, the eigensystem cotangents to the Roe state.  Point phase:
-// those pulled back through L (F and U projections), the physical fluxes
-// (flux_from_derived, src/state.rs:56-85), the Roe average and
-// Derived::from_state to the 6 stencil states (36 values per interface,
-// stored for the gather).  The register footprint allows one wavefront per
-// SIMD, so a workgroup is one wavefront with 60 x 64 doubles of LDS.
+// Interface flux pullback in two phases.  Characteristic phase: the WENO5 /
+// local Lax-Friedrichs / eigenvector pullback of every characteristic
+// (src/weno.rs:685-802), two lanes per interface (32 interfaces per 64-lane workgroup): lane `half` of a pair
+// holds stencil points 3 half .. 3 half + 2 in registers and evaluates one
+// of the two WENO5 pullbacks of each characteristic (characteristic_step_pair);
+// the split-flux cotangents of its points (6 per characteristic) go to LDS,
+// the eigensystem cotangents to the Roe state, formed by both lanes.  Point
+// phase: each lane pulls its three points' split-flux cotangents back
+// through L (F and U projections), the physical fluxes (flux_from_derived,
+// src/state.rs:56-85), the Roe average and Derived::from_state to their
+// states (36 values per interface, stored for the gather).  Half the
+// stencil per lane fits two wavefronts per SIMD (256 VGPRs, a few spilled);
+// one lane per interface needs ~450 registers and runs one wavefront per
+// SIMD, 4% slower end to end on the reverse mode.
 constexpr int kIfBx = 64;
-constexpr int kCb = 60;
+constexpr int kIfPair = 32;
+constexpr int kCbPair = 36;
 
 // Interfaces of a direction are numbered i-major, t = i * nj + j, and a
-// workgroup takes 64 consecutive ones (a workgroup may span two lines).
+// workgroup takes 32 consecutive ones (a workgroup may span two lines).
 template <int DIR>
 __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __restrict__ Kp, const Phys& ph,
                                           int tile, double (*s_cb)[kIfBx]) {
   const Geo g = A.g;
   const int ni = DIR == 0 ? g.nx + 1 : g.nx;
   const int nj = DIR == 0 ? g.ny : g.ny + 1;
-  const long t = (long)tile * kIfBx + threadIdx.x;
-  if (t >= (long)ni * nj) return;
+  const int half = threadIdx.x & 1;
+  const long t = (long)tile * kIfPair + (threadIdx.x >> 1);
+  if (t >= (long)ni * nj) return;  // both lanes of a pair
   const int i = (int)(t / nj), j = (int)(t - (long)i * nj);
   const long fpl = DIR == 0 ? (long)(g.nx + 1) * g.ny : (long)g.nx * (g.ny + 1);
   const long fidx = DIR == 0 ? (long)i * g.ny + j : (long)i * (g.ny + 1) + j;
   double* __restrict__ out = DIR == 0 ? A.XB : A.YB;
   const long c0 = pad_index(g, i, j);
   const long step = DIR == 0 ? g.PY : 1;
+  const int q0 = 3 * half;
   if (A.pt[c0] != 1 && A.pt[c0 - step] != 1) {
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
+    for (int k = 0; k < 3; ++k)
 #pragma unroll
-      for (int f = 0; f < 6; ++f) out[(q * fpl + fidx) * 6 + f] = 0.0;
+      for (int f = 0; f < 6; ++f) out[((q0 + k) * fpl + fidx) * 6 + f] = 0.0;
     return;
   }
   const double* S = A.S;
   const double* D = A.D;
   const long P = g.plane;
   double* cbs = &s_cb[0][threadIdx.x];  // this lane's column, stride kIfBx
-  RoeAvg ra;
-  roe_average(S, D, P, c0, c0 - step, ra);
   RoeV R;
-  roe_values<DIR>(*fresh(Kp), ra.qr[0], ra.qr[1], ra.qr + 2, R);
+  {
+    RoeAvg ra;
+    roe_average(S, D, P, c0, c0 - step, ra);
+    roe_values<DIR>(*fresh(Kp), ra.qr[0], ra.qr[1], ra.qr + 2, R);
+  }
   double roeb[5];
   {
     // flux F[i] enters r(i) with +1/ds and r(i-1) with -1/ds (src/main.rs:266-270)
@@ -127,23 +137,23 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
     double ob[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) ob[f] = A.rscale * inv * (A.LAM[f * P + c0] - A.LAM[f * P + c0 - step]);
-    double U[6][6], F[6][6];
+    double U[3][6], F[3][6];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) load_point<DIR>(S, D, P, (unsigned)((c0 + (q - 3) * step) * 8), U[q], F[q]);
+    for (int k = 0; k < 3; ++k) load_point<DIR>(S, D, P, (unsigned)((c0 + (q0 + k - 3) * step) * 8), U[k], F[k]);
     RoeBar B;
     roe_bar_zero(B);
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-      double cb[10];
-      characteristic_step<DIR>(c, *fresh(Kp), ph.eps, R, U, F, ob, cb, B);
+      double cb6[6];
+      characteristic_step_pair<DIR>(c, half, *fresh(Kp), ph.eps, R, U, F, ob, cb6, B);
 #pragma unroll
-      for (int k = 0; k < 10; ++k) cbs[(c * 10 + k) * kIfBx] = cb[k];
+      for (int k = 0; k < 6; ++k) cbs[(c * 6 + k) * kIfBx] = cb6[k];
     }
     roe_bar_final<DIR>(*fresh(Kp), R, B, roeb);
   }
   // the split-flux cotangents are read back from LDS, not kept in registers
   reload_barrier();
-  // ---- point phase
+  // ---- point phase, this lane's points q0 .. q0 + 2
   double L[6][6], a[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -151,23 +161,27 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
     row_col<DIR>(*fresh(Kp), R, c, L[c], col, &lam);
     a[c] = fabs(lam);
   }
-  double b3[5], b2[5], r1b = 0.0, r2b = 0.0;
+  // the Roe-average pullback enters point 2 (half 0) or point 3 (half 1)
+  double bb[5], rb = 0.0, rr;
+  {
+    RoeAvg ra;
+    roe_average(S, D, P, c0, c0 - step, ra);
+    rr = half ? ra.r1 : ra.r2;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    b3[k] = roeb[k] * ra.r1 * ra.irs;
-    b2[k] = roeb[k] * ra.r2 * ra.irs;
-    r1b += roeb[k] * (ra.q3[k] - ra.qr[k]) * ra.irs;
-    r2b += roeb[k] * (ra.q2[k] - ra.qr[k]) * ra.irs;
+    for (int k = 0; k < 5; ++k) {
+      bb[k] = roeb[k] * rr * ra.irs;
+      rb += roeb[k] * ((half ? ra.q3[k] : ra.q2[k]) - ra.qr[k]) * ra.irs;
+    }
   }
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
+  for (int k = 0; k < 3; ++k) {
+    const int q = q0 + k;
     const long p = c0 + (q - 3) * step;
     // cotangents of this point's physical flux (fb) and state (sb) through L
     double fb[6] = {0, 0, 0, 0, 0, 0}, sb[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-      const double fpb = q < 5 ? cbs[(c * 10 + q) * kIfBx] : 0.0;
-      const double fmb = q > 0 ? cbs[(c * 10 + 4 + q) * kIfBx] : 0.0;
+      const double fpb = cbs[(c * 6 + k) * kIfBx], fmb = cbs[(c * 6 + 3 + k) * kIfBx];
       const double cfb = 0.5 * (fpb + fmb), cub = a[c] * 0.5 * (fpb - fmb);
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
@@ -179,7 +193,7 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
 #pragma unroll
     for (int f = 0; f < 6; ++f) st[f] = S[f * P + p];
 #pragma unroll
-    for (int k = 0; k < D_N; ++k) db[k] = 0.0;
+    for (int m = 0; m < D_N; ++m) db[m] = 0.0;
     const double u = D[D_U * P + p], v = D[D_V * P + p];
     const double pk[3] = {D[D_PE * P + p], D[D_PI * P + p], D[D_PR * P + p]};
     const double rho = st[0], w = DIR == 0 ? u : v;
@@ -196,10 +210,10 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
     }
     const double Pb = DIR == 0 ? fb[1] : fb[2];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      sb[3 + k] += fb[3 + k] * w;
-      wb += fb[3 + k] * (st[3 + k] + pk[k]);
-      db[D_PE + k] = Pb + fb[3 + k] * w;
+    for (int m = 0; m < 3; ++m) {
+      sb[3 + m] += fb[3 + m] * w;
+      wb += fb[3 + m] * (st[3 + m] + pk[m]);
+      db[D_PE + m] = Pb + fb[3 + m] * w;
     }
     if (DIR == 0)
       ub += wb;
@@ -207,14 +221,13 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
       vb += wb;
     db[D_U] = ub;
     db[D_V] = vb;
-    if (q == 3 || q == 2) {
-      const double* bb = q == 3 ? b3 : b2;
+    if (half ? k == 0 : k == 2) {
       db[D_U] += bb[0];
       db[D_V] += bb[1];
       db[D_EE] = bb[2];
       db[D_EI] = bb[3];
       db[D_ER] = bb[4];
-      sb[0] += (q == 3 ? r1b * 0.5 / ra.r1 : r2b * 0.5 / ra.r2);
+      sb[0] += rb * 0.5 / rr;
     }
     derived_vjp(ph, st, db, sb);
 #pragma unroll
@@ -225,9 +238,9 @@ __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __re
 // x interfaces in workgroups [0, nbx), y interfaces after them: one launch,
 // so that the two directions share a single partial last round of
 // workgroups
-__global__ __launch_bounds__(kIfBx) void k_adj_iface(AdjDesc A, const EigConst* __restrict__ Kp, Phys ph,
-                                                     int nbx) {
-  __shared__ double s_cb[kCb][kIfBx];
+__global__ __launch_bounds__(kIfBx) __attribute__((amdgpu_waves_per_eu(2))) void k_adj_iface(
+    AdjDesc A, const EigConst* __restrict__ Kp, Phys ph, int nbx) {
+  __shared__ double s_cb[kCbPair][kIfBx];
   if ((int)blockIdx.x < nbx)
     adj_iface<0>(A, Kp, ph, blockIdx.x, s_cb);
   else
@@ -816,9 +829,15 @@ void launch_vjp(Engine& E, int buf, bool taped, const double* LAM, double rscale
   A.PG = W.PG;
   {
     Launcher L(E, K_ADJ_IFACE);
+#if RH3T_IFACE_PAIR
+    const int nbx = (int)(((long)(g.nx + 1) * g.ny + kIfPair - 1) / kIfPair);
+    const int nby = (int)(((long)g.nx * (g.ny + 1) + kIfPair - 1) / kIfPair);
+    k_adj_iface<<<nbx + nby, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys, nbx);
+#else
     const int nbx = (int)(((long)(g.nx + 1) * g.ny + kIfBx - 1) / kIfBx);
     const int nby = (int)(((long)g.nx * (g.ny + 1) + kIfBx - 1) / kIfBx);
     k_adj_iface<<<nbx + nby, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys, nbx);
+#endif
   }
   if (W.n_out > 0) {
     // ghost gather on the side stream, concurrently with the tile gather

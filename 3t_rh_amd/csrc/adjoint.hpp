@@ -380,61 +380,84 @@ __device__ __forceinline__ void load_point(const double* S, const double* D, lon
   F[5] = (U[5] + pr) * w;
 }
 
+// The value of the partner lane (lane ^ 1) of a lane pair: DPP quad
+// permutation [1, 0, 3, 2] of each 32-bit half.
+__device__ __forceinline__ double pair_swap(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0xB1, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 // Characteristic c of the pullback of one interface flux (interface between
-// stencil points 2 and 3).  U, F: states and physical fluxes of the 6
-// points; ob: flux cotangent.  The cotangents of the split fluxes fp[0..5)
-// and fm[1..6) that enter the two WENO5 reconstructions are written to
-// cb[0..10) (their pullback to the stencil states is linear and is formed
-// by the point phase), and the cotangents of row c of L, column c of R and
-// lambda_c are accumulated into B.
+// stencil points 2 and 3), split over a lane pair: lane `half` holds points
+// 3 half .. 3 half + 2 (U, F: their states and physical fluxes) and
+// evaluates one of the two WENO5 pullbacks (half 0: the reconstruction from
+// fp[0..5), half 1: the one from fm[5], fm[4], .., fm[1]); the values each
+// needs from the other half cross by pair_swap.  ob: flux cotangent.  cb6:
+// the cotangents of the split fluxes fp then fm at this lane's three points
+// (their pullback to the stencil states is linear and is formed by the
+// point phase).  Both lanes accumulate the same cotangents of row c of L,
+// column c of R and lambda_c into B (partial sums over the points are
+// combined lane 0 first).
 template <int DIR>
-__device__ __forceinline__ void characteristic_step(int c, const EigConst& K, double eps, const RoeV& R,
-                                                    const double U[6][6], const double F[6][6], const double ob[6],
-                                                    double cb[10], RoeBar& B) {
+__device__ __forceinline__ void characteristic_step_pair(int c, int half, const EigConst& K, double eps,
+                                                         const RoeV& R, const double U[3][6], const double F[3][6],
+                                                         const double ob[6], double cb6[6], RoeBar& B) {
   double row[6], col[6], lam;
   row_col<DIR>(K, R, c, row, col, &lam);
   const double a = fabs(lam);
-  double cu[6], fp[6], fm[6];
+  double cu[3], fp[3], fm[3];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) {
+  for (int k = 0; k < 3; ++k) {
     double cf = 0.0, u_ = 0.0;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
-      cf += row[f] * F[q][f];
-      u_ += row[f] * U[q][f];
+      cf += row[f] * F[k][f];
+      u_ += row[f] * U[k][f];
     }
-    cu[q] = u_;
-    fp[q] = 0.5 * (cf + a * u_);
-    fm[q] = 0.5 * (cf - a * u_);
+    cu[k] = u_;
+    fp[k] = 0.5 * (cf + a * u_);
+    fm[k] = 0.5 * (cf - a * u_);
   }
   double fcb = 0.0;
 #pragma unroll
   for (int r = 0; r < 6; ++r) fcb += col[r] * ob[r];
-  double gp[5], gm[5];
-  const double sp[5] = {fp[0], fp[1], fp[2], fp[3], fp[4]};
-  const double sm[5] = {fm[5], fm[4], fm[3], fm[2], fm[1]};
-  const double fc = weno5_vjp(sp, eps, fcb, gp) + weno5_vjp(sm, eps, fcb, gm);
-  // fpb[q] = gp[q] (q < 5), fmb[q] = gm[5 - q] (q > 0)
+  // half 0 reconstructs from fp[0..5), half 1 from fm[5], fm[4], .., fm[1]
+  const double y0 = pair_swap(half ? fp[0] : fm[1]);
+  const double y1 = pair_swap(half ? fp[1] : fm[2]);
+  const double in[5] = {half ? fm[2] : fp[0], half ? fm[1] : fp[1], half ? fm[0] : fp[2], half ? y1 : y0,
+                        half ? y0 : y1};
+  double gw[5];
+  const double W = weno5_vjp(in, eps, fcb, gw);
+  const double Wo = pair_swap(W);
+  const double fc = half ? Wo + W : W + Wo;
+  // fpb[q] = gp[q] (q < 5), fmb[q] = gm[5 - q] (q > 0), for this lane's q
+  const double w0 = pair_swap(half ? gw[4] : gw[3]);
+  const double w1 = pair_swap(half ? gw[3] : gw[4]);
+  const double fpb[3] = {half ? w0 : gw[0], half ? w1 : gw[1], half ? 0.0 : gw[2]};
+  const double fmb[3] = {half ? gw[2] : 0.0, half ? gw[1] : w0, half ? gw[0] : w1};
+  double abp = 0.0;
+  double Lbp[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    cb[k] = gp[k];
-    cb[5 + k] = gm[4 - k];
-  }
-  double ab = 0.0;
-  double Lb[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const double fpb = q < 5 ? gp[q] : 0.0, fmb = q > 0 ? gm[5 - q] : 0.0;
-    const double cfb = 0.5 * (fpb + fmb);
-    const double d = 0.5 * (fpb - fmb);
+  for (int k = 0; k < 3; ++k) {
+    cb6[k] = fpb[k];
+    cb6[3 + k] = fmb[k];
+    const double cfb = 0.5 * (fpb[k] + fmb[k]);
+    const double d = 0.5 * (fpb[k] - fmb[k]);
     const double cub = a * d;
-    ab += cu[q] * d;
+    abp += cu[k] * d;
 #pragma unroll
-    for (int f = 0; f < 6; ++f) Lb[f] += cfb * F[q][f] + cub * U[q][f];
+    for (int f = 0; f < 6; ++f) Lbp[f] += cfb * F[k][f] + cub * U[k][f];
   }
-  double Rb[6];
+  const double abo = pair_swap(abp);
+  const double ab = half ? abo + abp : abp + abo;
+  double Lb[6], Rb[6];
 #pragma unroll
-  for (int r = 0; r < 6; ++r) Rb[r] = fc * ob[r];
+  for (int f = 0; f < 6; ++f) {
+    const double o = pair_swap(Lbp[f]);
+    Lb[f] = half ? o + Lbp[f] : Lbp[f] + o;
+    Rb[f] = fc * ob[f];
+  }
   const double lamb = lam >= 0.0 ? ab : -ab;
   row_col_bar<DIR>(K, R, c, Lb, Rb, lamb, B);
 }
