@@ -829,15 +829,10 @@ void launch_vjp(Engine& E, int buf, bool taped, const double* LAM, double rscale
   A.PG = W.PG;
   {
     Launcher L(E, K_ADJ_IFACE);
-#if RH3T_IFACE_PAIR
+    // a workgroup of kIfBx lanes covers kIfPair interfaces
     const int nbx = (int)(((long)(g.nx + 1) * g.ny + kIfPair - 1) / kIfPair);
     const int nby = (int)(((long)g.nx * (g.ny + 1) + kIfPair - 1) / kIfPair);
     k_adj_iface<<<nbx + nby, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys, nbx);
-#else
-    const int nbx = (int)(((long)(g.nx + 1) * g.ny + kIfBx - 1) / kIfBx);
-    const int nby = (int)(((long)g.nx * (g.ny + 1) + kIfBx - 1) / kIfBx);
-    k_adj_iface<<<nbx + nby, kIfBx, 0, E.stream>>>(A, E.d_eig, E.phys, nbx);
-#endif
   }
   if (W.n_out > 0) {
     // ghost gather on the side stream, concurrently with the tile gather
