@@ -380,14 +380,6 @@ __device__ __forceinline__ void load_point(const double* S, const double* D, lon
   F[5] = (U[5] + pr) * w;
 }
 
-// The value of the partner lane (lane ^ 1) of a lane pair: DPP quad
-// permutation [1, 0, 3, 2] of each 32-bit half.
-__device__ __forceinline__ double pair_swap(double x) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0xB1, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0xB1, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
 // Characteristic c of the pullback of one interface flux (interface between
 // stencil points 2 and 3), split over a lane pair: lane `half` holds points
 // 3 half .. 3 half + 2 (U, F: their states and physical fluxes) and

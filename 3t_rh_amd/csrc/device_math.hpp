@@ -344,4 +344,12 @@ __device__ __forceinline__ bool admissible(const double s[6]) {
   return !(s[3] / s[0] - k <= 0.0 || s[4] / s[0] - k <= 0.0 || s[5] / s[0] - k <= 0.0);
 }
 
+// The value of the partner lane (lane ^ 1) of a lane pair: DPP quad
+// permutation [1, 0, 3, 2] of each 32-bit half.
+__device__ __forceinline__ double pair_swap(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0xB1, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 }  // namespace rh3t_dev
