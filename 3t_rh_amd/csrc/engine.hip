@@ -540,15 +540,23 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
     return;
   }
   // 6-point stencils of U and of the physical flux (flux_from_derived,
-  // src/state.rs:56-85)
+  // src/state.rs:56-85).  In x, each value is read as a workgroup-uniform
+  // base (plane and stencil row, formed in scalar registers) plus this
+  // lane's 32-bit byte offset: one per-lane address for all the loads.  In
+  // y, plain indexing lets the compiler merge neighbouring points into
+  // 16-byte loads.
+  const unsigned boff = (unsigned)(c0 * (long)sizeof(double));
+  auto at = [&](const double* plane, long q_off) {
+    return DIR == 0 ? plane_at(glb(plane + q_off), boff) : plane[c0 + q_off];
+  };
   double U[6][6], Fl[6][6];
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    const long p = c0 + (q - 3) * step;
+    const long qo = (q - 3) * step;
 #pragma unroll
-    for (int f = 0; f < 6; ++f) U[q][f] = S[f * g.plane + p];
-    const double u = D[D_U * g.plane + p], v = D[D_V * g.plane + p];
-    const double pe = D[D_PE * g.plane + p], pi = D[D_PI * g.plane + p], pr = D[D_PR * g.plane + p];
+    for (int f = 0; f < 6; ++f) U[q][f] = at(S + f * g.plane, qo);
+    const double u = at(D + D_U * g.plane, qo), v = at(D + D_V * g.plane, qo);
+    const double pe = at(D + D_PE * g.plane, qo), pi = at(D + D_PI * g.plane, qo), pr = at(D + D_PR * g.plane, qo);
     const double ptot = pe + pi + pr;
     const double w = DIR == 0 ? u : v;
     Fl[q][0] = U[q][0] * w;
@@ -558,12 +566,11 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
     Fl[q][4] = (U[q][4] + pi) * w;
     Fl[q][5] = (U[q][5] + pr) * w;
   }
-  const long p2 = c0 - step;
-  const double e3[3] = {D[D_EE * g.plane + c0], D[D_EI * g.plane + c0], D[D_ER * g.plane + c0]};
-  const double e2[3] = {D[D_EE * g.plane + p2], D[D_EI * g.plane + p2], D[D_ER * g.plane + p2]};
+  const double e3[3] = {at(D + D_EE * g.plane, 0), at(D + D_EI * g.plane, 0), at(D + D_ER * g.plane, 0)};
+  const double e2[3] = {at(D + D_EE * g.plane, -step), at(D + D_EI * g.plane, -step), at(D + D_ER * g.plane, -step)};
   double out[6];
-  interface_flux<DIR, FAST>(Kp, eps, U, Fl, D[D_U * g.plane + c0], D[D_V * g.plane + c0], e3,
-                            D[D_U * g.plane + p2], D[D_V * g.plane + p2], e2, out);
+  interface_flux<DIR, FAST>(Kp, eps, U, Fl, at(D + D_U * g.plane, 0), at(D + D_V * g.plane, 0), e3,
+                            at(D + D_U * g.plane, -step), at(D + D_V * g.plane, -step), e2, out);
 #pragma unroll
   for (int f = 0; f < 6; ++f) F[f * fpl + fidx] = out[f];
 }
