@@ -64,6 +64,25 @@ def test_batch_launch_shape_invariance(rh3t, members, shape):
     b.close()
 
 
+def test_batch_launch_shape_limits(rh3t, members):
+    """Shapes outside the kernels' limits are refused with ERR_ARG and leave
+    the previous shape in place: more than 256 lanes, a width that is not a
+    multiple of 16, a workgroup that is not whole wavefronts, and more heavy
+    ghosts per workgroup than the ghost kernels' LDS holds."""
+    fields, states = members
+    _say("batch launch shape limits")
+    b = rh3t.Batch(fields)
+    for shape in [(256, 2, 1), (24, 8, 1), (16, 2, 1), (64, 4, 64)]:
+        with pytest.raises(rh3t.RH3TError) as ei:
+            b.set_launch(*shape)
+        assert ei.value.status == rh3t.ERR_ARG, shape
+    for k, u in enumerate(states):
+        b.set_state(k, u, 0.0)
+    b.advance(1)
+    assert all(np.isfinite(b.get_state(k)[0]).all() for k in range(len(fields)))
+    b.close()
+
+
 def test_batch_member_against_oracle(rh3t, members):
     fields, states = members
     _say("batch member (Mach 2.3 clone) against the oracle, 1 step")
