@@ -77,13 +77,14 @@ __device__ __forceinline__ void roe_average(const double* S, const double* D, lo
   }
 }
 
-// Interface flux pullback in two phases.  Characteristic phase: the WENO5 /
+// Interface flux pullback in two phases, two lanes per interface (32
+// interfaces per 64-lane workgroup).  Characteristic phase: the WENO5 /
 // local Lax-Friedrichs / eigenvector pullback of every characteristic
-// (src/weno.rs:685-802), two lanes per interface (32 interfaces per 64-lane workgroup): lane `half` of a pair
-// holds stencil points 3 half .. 3 half + 2 in registers and evaluates one
-// of the two WENO5 pullbacks of each characteristic (characteristic_step_pair);
-// the split-flux cotangents of its points (6 per characteristic) go to LDS,
-// the eigensystem cotangents to the Roe state, formed by both lanes.  Point
+// (src/weno.rs:685-802); lane `half` of a pair holds stencil points
+// 3 half .. 3 half + 2 in registers and evaluates one of the two WENO5
+// pullbacks of each characteristic (characteristic_step_pair).  The
+// split-flux cotangents of its points (6 per characteristic) go to LDS, the
+// eigensystem cotangents to the Roe state, formed by both lanes.  Point
 // phase: each lane pulls its three points' split-flux cotangents back
 // through L (F and U projections), the physical fluxes (flux_from_derived,
 // src/state.rs:56-85), the Roe average and Derived::from_state to their
