@@ -253,9 +253,10 @@ int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode);
 int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys);
 
 /* Workgroup shape of the WENO flux / assembly kernels (bx lanes along y,
- * by rows along x; bx a multiple of 16, bx*by <= 256) and heavy ghosts per
- * workgroup (1..4, one wavefront each).  Chosen by the autotuner in
- * 3t_rh_amd/autotune.py; no counterpart in the reference. */
+ * by rows along x; bx a multiple of 16, bx*by <= 256 and a multiple of 64)
+ * and heavy ghosts per workgroup (1..4, one wavefront each).  Other values
+ * return RH3T_ERR_ARG and keep the current shape.  Chosen by the autotuner
+ * in 3t_rh_amd/autotune.py; no counterpart in the reference. */
 int rh3t_ctx_set_launch(rh3t_ctx* ctx, int32_t bx, int32_t by,
                         int32_t ghosts_per_block);
 
