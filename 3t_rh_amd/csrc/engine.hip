@@ -580,6 +580,9 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
 // One cell (i, j) of solve sd: l()'s right-hand side and, for sa.mode >= 1,
 // the stage update.  Returns get_local_dt of the updated state after stage 3
 // (the dt rule of the next step), +inf otherwise.
+// FAST: the noncon divisions as products (noncon_fast); EXACT: the
+// reference's divisions.
+template <bool FAST>
 __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys& ph, const StageSpec& sa, int i,
                                                 int j) {
   const Geo g = sd.g;
@@ -619,7 +622,10 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
       pi[k] = D[D_PI * g.plane + p];
       pr[k] = D[D_PR * g.plane + p];
     }
-    noncon(vel, pe, pi, pr, g.dx, ncx);
+    if (FAST)
+      noncon_fast(vel, pe, pi, pr, inv_dx, ncx);
+    else
+      noncon(vel, pe, pi, pr, g.dx, ncx);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const long p = c0 + (k - 4);
@@ -628,7 +634,10 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
       pi[k] = D[D_PI * g.plane + p];
       pr[k] = D[D_PR * g.plane + p];
     }
-    noncon(vel, pe, pi, pr, g.dy, ncy);
+    if (FAST)
+      noncon_fast(vel, pe, pi, pr, inv_dy, ncy);
+    else
+      noncon(vel, pe, pi, pr, g.dy, ncy);
   }
 #pragma unroll
   for (int e = 0; e < 3; ++e) r[3 + e] = r[3 + e] + ncx[e] + ncy[e];
@@ -708,6 +717,7 @@ __device__ __forceinline__ void put_group_min(double m, double* __restrict__ par
 // Assembly + stage update of every solve; the last RK stage also leaves the
 // per-workgroup minima of get_local_dt of the new state (src/dt.rs:5-34)
 // for the next step's dt.
+template <bool FAST>
 __global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ descs, const int* __restrict__ tiles,
                                                   int nsolve, Phys ph, StageSpec sa, double* __restrict__ partial) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
@@ -717,7 +727,7 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ 
   const int lt = t - tiles[s];
   const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
   const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
-  const double m = (i < sd.g.nx && j < sd.g.ny) ? assemble_cell(sd, ph, sa, i, j) : INFINITY;
+  const double m = (i < sd.g.nx && j < sd.g.ny) ? assemble_cell<FAST>(sd, ph, sa, i, j) : INFINITY;
   if (sa.mode == 3) put_group_min(m, partial, t);
 }
 
@@ -938,7 +948,10 @@ void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture) {
   }
   {
     Launcher L(E, K_ASSEMBLE);
-    k_assemble<<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
+    if (E.fast)
+      k_assemble<true><<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
+    else
+      k_assemble<false><<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
   }
   if (sa.mode > 0 && capture) {
     Launcher L(E, K_CAPTURE);
