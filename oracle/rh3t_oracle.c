@@ -511,13 +511,13 @@ static int solve6(const double* A0, const double* b0, double* x) {
 
 /* Taylor expansion to the ghost point and rotation back, shared tail of the
  * Wall / Outflow / FarField branches (src/bc1.rs:728-755, 809-827, 896-923). */
-static void taylor_rotate(double uk[5][6], double dist, const double n[2], double out[6]) {
+static void taylor_rotate(const double* uk, double dist, const double n[2], double out[6]) {
   double ug[6];
-  memcpy(ug, uk[0], sizeof(ug));
+  memcpy(ug, uk, sizeof(ug));
   double coef = 1.0;
   for (int k = 1; k <= 4; ++k) {
     coef *= dist / (double)k;
-    for (int c = 0; c < 6; ++c) ug[c] = ug[c] + coef * uk[k][c];
+    for (int c = 0; c < 6; ++c) ug[c] = ug[c] + coef * uk[6 * k + c];
   }
   double mom_n = ug[1], mom_t = ug[2];
   out[0] = ug[0];
@@ -526,6 +526,86 @@ static void taylor_rotate(double uk[5][6], double dist, const double n[2], doubl
   out[3] = ug[3];
   out[4] = ug[4];
   out[5] = ug[5];
+}
+
+/* The heavy branches of set_ghost_point_value (src/bc1.rs:663-928):
+ * L, R, lambda of the nearest state in the (n, t) frame, V^(k) of the
+ * characteristic WENO extrapolation and the normal derivatives u^(k) of the
+ * conservative state at the boundary point, before the Taylor expansion. */
+int ro_heavy_terms(const ro_problem* pb, const ro_physics* ph, const double* u, int g, double L[36],
+                   double R[36], double lam[6], double v[30], double uk[30]) {
+  int kind = pb->g_kind[g];
+  int h = pb->g_heavy[g];
+  if (h < 0 || (kind != 1 && kind != 5 && kind != 7)) return 1;
+  const double* n = pb->g_normal + 2 * g;
+  build_local_plain(ph, n, u + 6 * (size_t)pb->h_nearest[h], L, R, lam);
+  for (int k = 0; k < 5; ++k)
+    weno_extrapolation_pre(ph, L, n, pb->h_d + 5 * (size_t)h, pb->h_stencil + 55 * (size_t)h,
+                           pb->h_s + 979 * (size_t)h, pb->h_w + 225 * (size_t)h, u, k, v + 6 * k);
+  if (kind == 7) { /* FarField, src/bc1.rs:829-924 */
+    const double* ui = pb->g_bcstate + 6 * g;
+    double mn = ui[1] * n[0] + ui[2] * n[1];
+    double mt = -ui[1] * n[1] + ui[2] * n[0];
+    double uloc[6] = {ui[0], mn, mt, ui[3], ui[4], ui[5]};
+    double vinf[6];
+    matvec6(L, uloc, vinf);
+    double rhs[6];
+    for (int m = 0; m < 6; ++m) rhs[m] = lam[m] < 0.0 ? vinf[m] : v[m];
+    matvec6(R, rhs, uk);
+    for (int k = 1; k <= 4; ++k) {
+      for (int m = 0; m < 6; ++m) rhs[m] = lam[m] < 0.0 ? 0.0 : v[6 * k + m];
+      matvec6(R, rhs, uk + 6 * k);
+    }
+  } else if (kind == 5) { /* Outflow-LODI, src/bc1.rs:760-828 */
+    const double* of = pb->g_outflow + 3 * g;
+    double p_inf = of[0], sigma = of[1], l_domain = of[2];
+    matvec6(R, v, uk);
+    double rho0 = uk[0];
+    double un0 = uk[1] / rho0, ut0 = uk[2] / rho0;
+    /* local_pressure_and_a2, src/bc1.rs:424-440 */
+    double w2 = un0 * un0 + ut0 * ut0;
+    double ee1 = uk[3] / rho0 - w2 / 6.0;
+    double ei1 = uk[4] / rho0 - w2 / 6.0;
+    double er1 = uk[5] / rho0 - w2 / 6.0;
+    double ge = ph->gamma_e - 1.0, gi = ph->gamma_i - 1.0, gr = ph->gamma_r - 1.0;
+    double p0 = rho0 * (ge * ee1 + gi * ei1 + gr * er1);
+    double a2 = ph->gamma_e * ge * ee1 + ph->gamma_i * gi * ei1 + ph->gamma_r * gr * er1;
+    double a = sqrt(a2);
+    double lambda1 = un0 - a;
+    double rhs1[6];
+    memcpy(rhs1, v + 6, sizeof(rhs1));
+    if (lambda1 < 0.0) {
+      double m_local = fabs(un0 / a);
+      double l1 = sigma * (1.0 - powi(m_local, 2)) * a / l_domain * (p0 - p_inf);
+      rhs1[0] = l1 / lambda1;
+    }
+    matvec6(R, rhs1, uk + 6);
+    for (int k = 2; k <= 4; ++k) matvec6(R, v + 6 * k, uk + 6 * k);
+  } else { /* Wall-ILW, src/bc1.rs:691-756 */
+    double L0[36], rhs0[6];
+    memcpy(L0, L, sizeof(L0));
+    for (int c = 0; c < 6; ++c) L0[c] = c == 1 ? 1.0 : 0.0;
+    memcpy(rhs0, v, sizeof(rhs0));
+    rhs0[0] = 0.0;
+    if (solve6(L0, rhs0, uk)) return 8;
+    /* build_ilw_row1, src/bc1.rs:448-465 */
+    double u1 = uk[0], u3 = uk[2];
+    double gi = ph->gamma_i - 1.0, ge = ph->gamma_e - 1.0, gr = ph->gamma_r - 1.0;
+    double gt = gi + ge + gr;
+    double L1[36], rhs1[6];
+    memcpy(L1, L, sizeof(L1));
+    L1[0] = -gt * powi(u3, 2) / (6.0 * powi(u1, 2));
+    L1[1] = 0.0;
+    L1[2] = gt * u3 / (6.0 * u1);
+    L1[3] = -ge;
+    L1[4] = -gi;
+    L1[5] = -gr;
+    memcpy(rhs1, v + 6, sizeof(rhs1));
+    rhs1[0] = 0.0;
+    if (solve6(L1, rhs1, uk + 6)) return 8;
+    for (int k = 2; k <= 4; ++k) matvec6(R, v + 6 * k, uk + 6 * k);
+  }
+  return 0;
 }
 
 /* set_ghost_point_value, src/bc1.rs:609-928 (precomputed fast path) */
@@ -552,82 +632,14 @@ static int ghost_value(const ro_problem* pb, const ro_physics* ph, const double*
     return 0;
   }
   if (kind == 4) return 6; /* TimeDependent: not in this path */
-  int h = pb->g_heavy[g];
-  double L[36], R[36], lam[6];
-  build_local_plain(ph, n, u + 6 * (size_t)pb->h_nearest[h], L, R, lam);
-  double v[5][6];
-  for (int k = 0; k < 5; ++k)
-    weno_extrapolation_pre(ph, L, n, pb->h_d + 5 * (size_t)h, pb->h_stencil + 55 * (size_t)h,
-                           pb->h_s + 979 * (size_t)h, pb->h_w + 225 * (size_t)h, u, k, v[k]);
-  double uk[5][6];
-  double dist = pb->g_dist[g];
-  if (kind == 7) { /* FarField, src/bc1.rs:829-924 */
-    const double* ui = pb->g_bcstate + 6 * g;
-    double mn = ui[1] * n[0] + ui[2] * n[1];
-    double mt = -ui[1] * n[1] + ui[2] * n[0];
-    double uloc[6] = {ui[0], mn, mt, ui[3], ui[4], ui[5]};
-    double vinf[6];
-    matvec6(L, uloc, vinf);
-    double rhs[6];
-    for (int m = 0; m < 6; ++m) rhs[m] = lam[m] < 0.0 ? vinf[m] : v[0][m];
-    matvec6(R, rhs, uk[0]);
-    for (int k = 1; k <= 4; ++k) {
-      for (int m = 0; m < 6; ++m) rhs[m] = lam[m] < 0.0 ? 0.0 : v[k][m];
-      matvec6(R, rhs, uk[k]);
-    }
-  } else if (kind == 5) { /* Outflow-LODI, src/bc1.rs:760-828 */
-    const double* of = pb->g_outflow + 3 * g;
-    double p_inf = of[0], sigma = of[1], l_domain = of[2];
-    matvec6(R, v[0], uk[0]);
-    double rho0 = uk[0][0];
-    double un0 = uk[0][1] / rho0, ut0 = uk[0][2] / rho0;
-    /* local_pressure_and_a2, src/bc1.rs:424-440 */
-    double w2 = un0 * un0 + ut0 * ut0;
-    double ee1 = uk[0][3] / rho0 - w2 / 6.0;
-    double ei1 = uk[0][4] / rho0 - w2 / 6.0;
-    double er1 = uk[0][5] / rho0 - w2 / 6.0;
-    double ge = ph->gamma_e - 1.0, gi = ph->gamma_i - 1.0, gr = ph->gamma_r - 1.0;
-    double p0 = rho0 * (ge * ee1 + gi * ei1 + gr * er1);
-    double a2 = ph->gamma_e * ge * ee1 + ph->gamma_i * gi * ei1 + ph->gamma_r * gr * er1;
-    double a = sqrt(a2);
-    double lambda1 = un0 - a;
-    double rhs1[6];
-    memcpy(rhs1, v[1], sizeof(rhs1));
-    if (lambda1 < 0.0) {
-      double m_local = fabs(un0 / a);
-      double l1 = sigma * (1.0 - powi(m_local, 2)) * a / l_domain * (p0 - p_inf);
-      rhs1[0] = l1 / lambda1;
-    }
-    matvec6(R, rhs1, uk[1]);
-    for (int k = 2; k <= 4; ++k) matvec6(R, v[k], uk[k]);
-  } else if (kind == 1) { /* Wall-ILW, src/bc1.rs:691-756 */
-    double L0[36], rhs0[6];
-    memcpy(L0, L, sizeof(L0));
-    for (int c = 0; c < 6; ++c) L0[c] = c == 1 ? 1.0 : 0.0;
-    memcpy(rhs0, v[0], sizeof(rhs0));
-    rhs0[0] = 0.0;
-    if (solve6(L0, rhs0, uk[0])) return 8;
-    /* build_ilw_row1, src/bc1.rs:448-465 */
-    double u1 = uk[0][0], u3 = uk[0][2];
-    double gi = ph->gamma_i - 1.0, ge = ph->gamma_e - 1.0, gr = ph->gamma_r - 1.0;
-    double gt = gi + ge + gr;
-    double L1[36], rhs1[6];
-    memcpy(L1, L, sizeof(L1));
-    L1[0] = -gt * powi(u3, 2) / (6.0 * powi(u1, 2));
-    L1[1] = 0.0;
-    L1[2] = gt * u3 / (6.0 * u1);
-    L1[3] = -ge;
-    L1[4] = -gi;
-    L1[5] = -gr;
-    memcpy(rhs1, v[1], sizeof(rhs1));
-    rhs1[0] = 0.0;
-    if (solve6(L1, rhs1, uk[1])) return 8;
-    for (int k = 2; k <= 4; ++k) matvec6(R, v[k], uk[k]);
-  } else {
+  if (kind != 1 && kind != 5 && kind != 7) {
     memset(out, 0, sizeof(double) * 6);
     return 0;
   }
-  taylor_rotate(uk, dist, n, out);
+  double L[36], R[36], lam[6], v[30], uk[30];
+  int st = ro_heavy_terms(pb, ph, u, g, L, R, lam, v, uk);
+  if (st) return st;
+  taylor_rotate(uk, pb->g_dist[g], n, out);
   return 0;
 }
 

@@ -79,6 +79,13 @@ void ro_diffusion(const ro_physics* ph, const ro_derived d[6], double out[6]);
 void ro_source(const ro_physics* ph, const double s[6], double out[6]);
 double ro_local_dt(const ro_physics* ph, const double s[6], double dx, double dy);
 
+/* Heavy ghost g (Wall / Outflow / FarField): L, R, lambda (6x6 row-major)
+ * of the nearest state in the (n, t) frame, V^(k) (v[k*6+c]) and the
+ * normal derivatives u^(k) (uk[k*6+f]) before the Taylor expansion.
+ * Returns 0, 1 (not a heavy ghost) or 8 (singular ILW system). */
+int ro_heavy_terms(const ro_problem* pb, const ro_physics* ph, const double* u, int g, double L[36],
+                   double R[36], double lam[6], double v[30], double uk[30]);
+
 /* --- operator level -------------------------------------------------- */
 /* ghosts.update_values_parallel: ghost_vals (n_ghost*6). Returns 0 or 4/8. */
 int ro_update_ghosts(const ro_problem* pb, const ro_physics* ph, const double* u,

@@ -82,8 +82,47 @@ def lib():
                                   ctypes.c_double, ctypes.c_int, ctypes.POINTER(ro_error)]
         L.ro_global_dt.argtypes = [ctypes.POINTER(ro_problem), ctypes.POINTER(ro_physics), _DP]
         L.ro_global_dt.restype = ctypes.c_double
+        _declare_heavy(L)
         _lib = L
     return _lib
+
+
+def _declare_heavy(L):
+    L.ro_heavy_terms.argtypes = [ctypes.POINTER(ro_problem), ctypes.POINTER(ro_physics), _DP, ctypes.c_int, _DP,
+                                 _DP, _DP, _DP, _DP]
+    L.ro_heavy_terms.restype = ctypes.c_int
+
+
+def load_variant(path):
+    """A separately compiled build of rh3t_oracle.c (same exports), e.g. a
+    planted-mistake copy for the mutation tests; the default library stays
+    in place (use :class:`using` to route calls through the variant)."""
+    global ORACLE_LIB, _lib
+    saved = (ORACLE_LIB, _lib)
+    ORACLE_LIB, _lib = path, None
+    try:
+        return lib()
+    finally:
+        ORACLE_LIB, _lib = saved
+
+
+class using:
+    """Context manager: route every oracle call of this module through the
+    given loaded library (restores the default afterwards)."""
+
+    def __init__(self, library):
+        self.library = library
+
+    def __enter__(self):
+        global _lib
+        self.saved = _lib
+        _lib = self.library
+        return self.library
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
 
 
 def physics(**kw):
@@ -235,6 +274,33 @@ def source(s, phys):
     out = np.zeros(6)
     lib().ro_source(ctypes.byref(phys), _dp(s), _dp(out))
     return out
+
+
+def heavy_terms(prob, u, g, phys=None):
+    """Internals of heavy ghost g: L, R (6x6), lambda, V^(k) (5x6) and the
+    normal derivatives u^(k) (5x6) of the conservative state at the boundary
+    point in the (n, t) frame."""
+    phys = phys or physics()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    L, R, lam, v, uk = np.zeros(36), np.zeros(36), np.zeros(6), np.zeros(30), np.zeros(30)
+    st = lib().ro_heavy_terms(ctypes.byref(prob.c), ctypes.byref(phys), _dp(u), int(g), _dp(L), _dp(R), _dp(lam),
+                              _dp(v), _dp(uk))
+    if st:
+        raise OracleError(st, ro_error())
+    return {"L": L.reshape(6, 6), "R": R.reshape(6, 6), "lam": lam, "v": v.reshape(5, 6), "uk": uk.reshape(5, 6)}
+
+
+def update_ghosts(prob, u, phys=None, nthreads=8):
+    """GhostGrid::update_values_parallel: ghost values (n_ghost, 6)."""
+    phys = phys or physics()
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    gv = np.zeros((max(prob.n_ghost, 1), 6))
+    err = ro_error()
+    st = lib().ro_update_ghosts(ctypes.byref(prob.c), ctypes.byref(phys), _dp(u), _dp(gv), nthreads,
+                                ctypes.byref(err))
+    if st:
+        raise OracleError(st, err)
+    return gv[: prob.n_ghost]
 
 
 def relerr(a, b, fluid=None):
