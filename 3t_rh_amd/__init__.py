@@ -133,6 +133,7 @@ SIGNATURES = {
     "rh3t_batch_get_state": (ctypes.c_int, [_P, ctypes.c_int32, _DP, _DP]),
     "rh3t_batch_advance": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, _DP]),
     "rh3t_batch_set_launch": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "rh3t_batch_set_arith": (ctypes.c_int, [_P, ctypes.c_int32]),
     "rh3t_batch_last_error": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(c_error)]),
     "rh3t_batch_profile": (ctypes.c_int, [_P, ctypes.c_int32]),
     "rh3t_batch_profile_read": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32, _DP,
@@ -371,7 +372,7 @@ class Field:
                "access table")
         return a
 
-    def solver(self, physics=None, device=0, arith=ARITH_FAST):
+    def solver(self, physics=None, device=0, arith=ARITH_EXACT):
         return Solver(self, physics, device, arith)
 
 
@@ -384,7 +385,7 @@ def device_count():
 class Solver:
     """Device context: the l() / rk3_ssp() path on one GPU."""
 
-    def __init__(self, field, physics=None, device=0, arith=ARITH_FAST):
+    def __init__(self, field, physics=None, device=0, arith=ARITH_EXACT):
         self.field = field
         self.physics = physics if physics is not None else Physics()
         self._h = ctypes.c_void_p()
@@ -570,7 +571,7 @@ class Batch:
     behaves as a Solver on its own Field; all members share one set of
     batched kernels and one hipGraph per step."""
 
-    def __init__(self, fields, physics=None, device=0, arith=ARITH_FAST):
+    def __init__(self, fields, physics=None, device=0, arith=ARITH_EXACT):
         self.fields = list(fields)
         self.physics = physics if physics is not None else Physics()
         arr = (ctypes.c_void_p * len(self.fields))(*[f._h for f in self.fields])
@@ -579,8 +580,7 @@ class Batch:
                                      ctypes.byref(self._h))
         if st != OK:
             raise RH3TError(st, "rh3t_batch_create")
-        if arith != ARITH_FAST:
-            raise RH3TError(ERR_ARG, "batches run the FAST flux arithmetic")
+        _check(lib().rh3t_batch_set_arith(self._h, arith), "batch set_arith")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -623,6 +623,9 @@ class Batch:
 
     def set_launch(self, bx, by, ghosts_per_block=1):
         _check(lib().rh3t_batch_set_launch(self._h, bx, by, ghosts_per_block), "batch set_launch")
+
+    def set_arith(self, mode):
+        _check(lib().rh3t_batch_set_arith(self._h, mode), "batch set_arith")
 
     def profile(self, enable=True):
         _check(lib().rh3t_batch_profile(self._h, 1 if enable else 0))

@@ -946,6 +946,8 @@ struct rh3t_adjoint {
   int every = 5;
   int nsteps = 0;
   std::vector<double> dts;
+  rh3t_physics phys{};           // constants of the last forward (the backward
+                                 // differentiates the forward it follows)
   std::vector<double*> ckpt;     // U^{kK}, k = 0, 1, ...
   std::vector<double*> s1, s2;   // stage states, [set * K + m]
   std::vector<double*> mid;      // U^{kK + m}, m = 1 .. K-1, [set * K + m] (m = 0 unused)
@@ -1178,6 +1180,7 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
   if ((st = ensure_dts(E, nsteps))) return st;
   A->dts.assign(dt, dt + nsteps);
   A->nsteps = nsteps;
+  A->phys = E.src_phys;
   A->tape_of[0] = A->tape_of[1] = -1;
   // a backward still recomputing on the side stream must finish first
   HIP_TRY(hipEventRecord(A->ev_join, A->side));
@@ -1204,9 +1207,38 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
   return check_errors(E);
 }
 
+namespace {
+
+bool same_physics(const rh3t_physics& a, const rh3t_physics& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
+int adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_initial, double* pgrad);
+
+}  // namespace
+
+// The backward differentiates the last forward with the constants that
+// forward ran with, even if the context's constants were replaced since
+// (rh3t_ctx_set_physics); the context's own constants are restored after.
 int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_initial, double* pgrad) {
   if (!A || !lam_final || !lam_initial) return RH3T_ERR_ARG;
   if (A->nsteps < 1) return RH3T_ERR_ARG;
+  Engine& E = A->ctx->e;
+  if (same_physics(A->phys, E.src_phys)) return adjoint_backward(A, lam_final, lam_initial, pgrad);
+  const rh3t_physics own = E.src_phys;
+  HIP_TRY(hipSetDevice(E.device));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  HIP_TRY(hipStreamSynchronize(A->side));
+  int st = apply_physics(E, A->phys);
+  if (st == RH3T_OK) st = adjoint_backward(A, lam_final, lam_initial, pgrad);
+  // the backward's kernels read the device constants: let them finish
+  (void)hipStreamSynchronize(A->side);
+  (void)hipStreamSynchronize(E.stream);
+  int st2 = apply_physics(E, own);
+  return st != RH3T_OK ? st : st2;
+}
+
+namespace {
+
+int adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_initial, double* pgrad) {
   Engine& E = A->ctx->e;
   int st = begin_adjoint_call(E);
   if (st) return st;
@@ -1269,6 +1301,8 @@ int rh3t_adjoint_backward(rh3t_adjoint* A, const double* lam_final, double* lam_
   E.dt_partials_valid = false;
   return check_errors(E);
 }
+
+}  // namespace
 
 int rh3t_adjoint_device_ms(const rh3t_adjoint* A, double* forward_ms, double* backward_ms) {
   if (!A || !forward_ms || !backward_ms || !A->timed_fwd || !A->timed_bwd) return RH3T_ERR_ARG;

@@ -996,7 +996,9 @@ int check_errors(Engine& E) {
     unsigned idx = (unsigned)(k & 0xFFFFFFFFULL);
     rh3t_error& e = E.last[s];
     e.kind = (int)kind;
-    e.stage = phase >= 2 ? (int)((phase - 2) / 2 + 1) : 0;
+    // ghost-update failures (even phases) report stage 0, as the reference's
+    // GhostGrid panic names no RK stage; state failures report RK1..RK3
+    e.stage = (phase >= 3 && phase % 2 == 1) ? (int)((phase - 3) / 2 + 1) : 0;
     e.ghost_id = -1;
     e.i = e.j = -1;
     std::memcpy(e.state, sc[s].err_state, sizeof(e.state));
@@ -1020,6 +1022,14 @@ int reset_errors(Engine& E) {
   k_err_clean<<<(E.nsolve + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.nsolve);
   HIP_TRY(hipGetLastError());
   return RH3T_OK;
+}
+
+void set_arith(Engine& E, bool fast) {
+  if (fast != E.fast && E.graph) {
+    (void)hipGraphExecDestroy(E.graph);
+    E.graph = nullptr;
+  }
+  E.fast = fast;
 }
 
 }  // namespace rh3t_eng
@@ -1084,8 +1094,33 @@ EigConst make_eig(const Phys& ph) {
   return K;
 }
 
+}  // namespace
+
+namespace rh3t_eng {
+
+int apply_physics(Engine& E, const rh3t_physics& p) {
+  E.src_phys = p;
+  E.phys = make_phys(p);
+  E.eig = make_eig(E.phys);
+  HIP_TRY(hipMemcpy(E.d_eig, &E.eig, sizeof(EigConst), hipMemcpyHostToDevice));
+  E.nplanes = (E.adj || E.phys.diffusion || E.phys.source) ? D_N : 8;
+  // the stored get_local_dt minima were formed with the old constants
+  E.dt_partials_valid = false;
+  if (E.graph) {
+    (void)hipGraphExecDestroy(E.graph);
+    E.graph = nullptr;
+  }
+  return RH3T_OK;
+}
+
+}  // namespace rh3t_eng
+
+namespace {
+
 void free_engine(Engine& E) {
   (void)hipSetDevice(E.device);
+  // nothing queued on the stream may still use the graph or the buffers
+  if (E.stream) (void)hipStreamSynchronize(E.stream);
   if (E.graph) (void)hipGraphExecDestroy(E.graph);
   for (auto& r : E.prof) {
     (void)hipEventDestroy(r.a);
@@ -1099,6 +1134,7 @@ void free_engine(Engine& E) {
 
 int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_physics& phys, int device) {
   E.device = device;
+  E.src_phys = phys;
   E.phys = make_phys(phys);
   E.eig = make_eig(E.phys);
   E.nsolve = n;
@@ -1367,16 +1403,20 @@ int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
     HIP_TRY(hipGetLastError());
   } else {
     if (!E.graph) {
-      hipGraph_t gr;
+      hipGraph_t gr = nullptr;
       HIP_TRY(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
       launch_dt(E, 1, false);
       launch_rk3(E);
-      HIP_TRY(hipStreamEndCapture(E.stream, &gr));
-      hipError_t e = hipGraphInstantiate(&E.graph, gr, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(gr);
+      // end the capture whatever the launches reported, so that the stream
+      // never stays in capture mode
+      hipError_t le = hipGetLastError();
+      hipError_t ce = hipStreamEndCapture(E.stream, &gr);
+      hipError_t e = le != hipSuccess ? le : ce;
+      if (e == hipSuccess) e = hipGraphInstantiate(&E.graph, gr, nullptr, nullptr, 0);
+      if (gr) (void)hipGraphDestroy(gr);
       if (e != hipSuccess) {
         E.graph = nullptr;
-        std::snprintf(E.message, sizeof(E.message), "graph instantiate: %s", hipGetErrorString(e));
+        std::snprintf(E.message, sizeof(E.message), "step graph capture: %s", hipGetErrorString(e));
         return RH3T_ERR_HIP;
       }
     }
@@ -1569,12 +1609,7 @@ int rh3t_advance(rh3t_ctx* ctx, int32_t nsteps, double cfl_factor, double dt_fix
 
 int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode) {
   if (!ctx || (mode != RH3T_ARITH_EXACT && mode != RH3T_ARITH_FAST)) return RH3T_ERR_ARG;
-  bool f = mode == RH3T_ARITH_FAST;
-  if (f != ctx->e.fast && ctx->e.graph) {
-    (void)hipGraphExecDestroy(ctx->e.graph);
-    ctx->e.graph = nullptr;
-  }
-  ctx->e.fast = f;
+  set_arith(ctx->e, mode == RH3T_ARITH_FAST);
   return RH3T_OK;
 }
 
@@ -1583,15 +1618,7 @@ int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys) {
   Engine& E = ctx->e;
   HIP_TRY(hipSetDevice(E.device));
   HIP_TRY(hipStreamSynchronize(E.stream));
-  E.phys = make_phys(*phys);
-  E.eig = make_eig(E.phys);
-  HIP_TRY(hipMemcpy(E.d_eig, &E.eig, sizeof(EigConst), hipMemcpyHostToDevice));
-  E.nplanes = (E.adj || E.phys.diffusion || E.phys.source) ? D_N : 8;
-  if (E.graph) {
-    (void)hipGraphExecDestroy(E.graph);
-    E.graph = nullptr;
-  }
-  return RH3T_OK;
+  return apply_physics(E, *phys);
 }
 
 int rh3t_ctx_set_launch(rh3t_ctx* ctx, int32_t bx, int32_t by, int32_t ghosts_per_block) {
@@ -1666,6 +1693,12 @@ int rh3t_batch_advance(rh3t_batch* b, int32_t nsteps, double cfl_factor, double*
     for (int s = 0; s < E.nsolve; ++s) times[s] = sc[s].time;
   }
   return st;
+}
+
+int rh3t_batch_set_arith(rh3t_batch* b, int32_t mode) {
+  if (!b || (mode != RH3T_ARITH_EXACT && mode != RH3T_ARITH_FAST)) return RH3T_ERR_ARG;
+  set_arith(b->e, mode == RH3T_ARITH_FAST);
+  return RH3T_OK;
 }
 
 int rh3t_batch_set_launch(rh3t_batch* b, int32_t bx, int32_t by, int32_t ghosts_per_block) {

@@ -92,10 +92,11 @@ struct AdjWork;  // adjoint workspace (adjoint.hip)
 struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
+  rh3t_physics src_phys{};  // the constants as the caller gave them
   Phys phys{};
   EigConst eig{};
   EigConst* d_eig = nullptr;  // device copy for kernels that read it through a pointer
-  bool fast = true;
+  bool fast = false;  // flux arithmetic: EXACT unless rh3t_*_set_arith selects FAST
   int nplanes = 8;
   int nsolve = 0;
   std::vector<SolveHost> solves;
@@ -202,6 +203,12 @@ int check_errors(Engine& E);
 int reset_errors(Engine& E);
 // releases the adjoint workspace (host side; device memory is in allocs)
 void free_adjoint(Engine& E);
+// replaces the physics constants (host structs, the device copy of the
+// eigensystem constants, the Derived plane count); drops the step graph and
+// the dt minima of the old constants.  The stream must be idle.
+int apply_physics(Engine& E, const rh3t_physics& p);
+// selects the flux / assembly arithmetic (drops the step graph on change)
+void set_arith(Engine& E, bool fast);
 
 }  // namespace rh3t_eng
 

@@ -139,6 +139,36 @@ def cylinder(nx=121, mach=3.0):
     return Field.init_cylinder(nx, mach)
 
 
+CYLINDER_CLASSES = (121, 161, 201, 241)
+
+
+def batched_draw(n_solves=64, seed=4):
+    """The batched workload's draw: per solve a grid class uniform over
+    {121x481, 161x641, 201x801, 241x961} (h = 3/(nx-1)) and an inflow Mach
+    number ~ U[2, 4], from the LCG."""
+    rng = LCG(seed)
+    draws = []
+    for _ in range(n_solves):
+        nx = CYLINDER_CLASSES[min(3, int(rng() * 4))]
+        draws.append((nx, 2.0 + 2.0 * rng()))
+    return draws
+
+
+def batched_cylinders(draws):
+    """Fields and initial states of a batched draw: one init_cylinder per
+    grid class, cloned per solve (the heavy-ghost tables are shared) with
+    the solve's freestream on the FarField sides."""
+    from . import BC_FARFIELD, cylinder_freestream
+    base = {nx: Field.init_cylinder(nx, 3.0) for nx in sorted(set(d[0] for d in draws))}
+    fields, states = [], []
+    for nx, mach in draws:
+        f = base[nx].clone()
+        f.set_bc_state(BC_FARFIELD, cylinder_freestream(mach))
+        fields.append(f)
+        states.append(f.cylinder_state(mach))
+    return fields, states
+
+
 # ----------------------------------------------- differentiation problem
 
 # Exchange coefficients of the differentiated workload: switched on (the

@@ -45,60 +45,86 @@ BYTES_PER_CELL_STAGE = 128  # stage 1: read U^k, write U (96 B); stages 2-3: + U
 FLUX_X_FETCH_KIB, FLUX_X_WRITE_KIB = 237200, 196700
 COPY_FETCH_SCALE, COPY_WRITE_SCALE = 2.0, 1.0
 FLUX_X_TRAFFIC_BYTES = (FLUX_X_FETCH_KIB * COPY_FETCH_SCALE + FLUX_X_WRITE_KIB * COPY_WRITE_SCALE) * 1024
+TRAFFIC_SOURCE = ("not measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_flux<0> at 2048^2, "
+                  "copy-calibrated (profiles/flux_x_counters_2048.md)")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(field, u, workloads, max_threads=16, min_seconds=10.0):
+def _host_facts():
+    """nproc, the affinity mask, OMP_NUM_THREADS and the CPU model."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return {"nproc": os.cpu_count(), "affinity_cores": affinity, "omp_num_threads": omp, "cpu_model": model}
+
+
+def cpu_baseline(field, u, workloads, min_seconds=10.0):
+    """The oracle (C restatement of the reference algorithm, OpenMP at the
+    reference's Rayon sites) on whole RK3 steps of the same box: on the
+    host's CPU share -- every core of the affinity mask, capped by
+    OMP_NUM_THREADS where the host sets it (the share of one GPU on a shared
+    node) -- for about min_seconds, then one step on one core."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bridge as ob
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, max_threads))
+    facts = _host_facts()
+    cores = facts["affinity_cores"]
+    if facts["omp_num_threads"] and facts["omp_num_threads"].isdigit():
+        cores = min(cores, int(facts["omp_num_threads"]))
+    cores = max(1, cores)
     prob = ob.OracleProblem(field)
     n_fluid = int(field.fluid.sum())
-    # whole RK3 steps of the same box until about min_seconds of CPU work
-    steps, wall = 0, 0.0
-    while wall < min_seconds and steps < 64:
-        dt = 0.05 * ob.global_dt(prob, u)
-        t0 = time.perf_counter()
-        u, _ = ob.rk3_step(prob, u, dt, nthreads=cores)
-        wall += time.perf_counter() - t0
-        steps += 1
+
+    def run(nthreads, min_s, max_steps):
+        uu, steps, wall = u, 0, 0.0
+        while steps == 0 or (wall < min_s and steps < max_steps):
+            dt = 0.05 * ob.global_dt(prob, uu)
+            t0 = time.perf_counter()
+            uu, _ = ob.rk3_step(prob, uu, dt, nthreads=nthreads)
+            wall += time.perf_counter() - t0
+            steps += 1
+        return steps, wall
+
+    steps, wall = run(cores, min_seconds, 64)
+    steps1, wall1 = run(1, 0.0, 1)
+    box = f"{field.grid.nx}x{field.grid.ny}"
     return {"value": n_fluid * 3 * steps / wall, "unit": "cell-stage/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} SSP-RK3 steps (3 l() + stages each) of the same {field.grid.nx}x{field.grid.ny} "
-                      f"box, C oracle with OpenMP on {cores} threads, {wall:.2f} s"}
+            "sample": f"{steps} SSP-RK3 steps (3 l() + stages each) of the same {box} box, C oracle with OpenMP "
+                      f"on {cores} threads, {wall:.2f} s",
+            "ms_per_step": wall / steps * 1e3,
+            "one_core": {"value": n_fluid * 3 * steps1 / wall1, "ms_per_step": wall1 / steps1 * 1e3,
+                         "sample": f"{steps1} SSP-RK3 step of the {box} box on 1 thread, {wall1:.2f} s"},
+            "host": facts}
 
 
-def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
+def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True, arith=0):
     """Config of the batched form: n_solves independent half-cylinder solves
     with grids drawn from {121x481, 161x641, 201x801, 241x961} and inflow Mach
     ~ U[2, 4] (seeded LCG), `steps` SSP-RK3 steps each with per-solve dt =
-    0.05 dt_cfl; one batched launch sequence per step (hipGraph) against the
-    same solves run back to back with the plain solver."""
+    0.05 dt_cfl; one batched launch sequence per step (a hipGraph of the
+    step's ~20 batched kernels, replayed from the host once per step)
+    against the same solves run back to back with the plain solver (one
+    step graph per solve), both in the same arithmetic."""
     workloads = importlib.import_module("3t_rh_amd.workloads")
     autotune = importlib.import_module("3t_rh_amd.autotune")
-    rng = workloads.LCG(seed)
-    classes = [121, 161, 201, 241]
-    draws = []
-    for _ in range(n_solves):
-        draws.append((classes[min(3, int(rng() * 4))], 2.0 + 2.0 * rng()))
-    base = {}
-    for nx in sorted(set(d[0] for d in draws)):
-        log(f"batched: building cylinder {nx}")
-        base[nx] = rh3t.Field.init_cylinder(nx, 3.0)
-    fields, states = [], []
-    for nx, mach in draws:
-        f = base[nx].clone()
-        f.set_bc_state(rh3t.BC_FARFIELD, rh3t.cylinder_freestream(mach))
-        fields.append(f)
-        states.append(f.cylinder_state(mach))
+    draws = workloads.batched_draw(n_solves, seed)
+    log(f"batched (arith={arith}): building {n_solves} cylinders")
+    fields, states = workloads.batched_cylinders(draws)
     cells = sum(int(f.fluid.sum()) for f in fields)
-    batch = rh3t.Batch(fields)
+    batch = rh3t.Batch(fields, arith=arith)
 
     def reset():
         for k, u in enumerate(states):
@@ -107,7 +133,7 @@ def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
     tuned = None
     if tune:
         reset()
-        key = autotune.signature([(f.grid.nx, f.grid.ny) for f in fields], "fast")
+        key = autotune.signature([(f.grid.nx, f.grid.ny) for f in fields], arith)
         tuned = autotune.tune(batch, steps=3, log=log)
         tuned["key"] = key
     reset()
@@ -122,8 +148,9 @@ def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
     batch.advance(3)
     kern = {k: round(v[0] * v[1] / 3, 4) for k, v in batch.profile_read().items() if v[1]}
     batch.profile(False)
+    batch.close()
     # the same solves back to back with the plain solver (graph per solve)
-    solvers = [f.solver() for f in fields]
+    solvers = [f.solver(arith=arith) for f in fields]
     for s, u in zip(solvers, states):
         s.set_state(u, 0.0)
         s.advance(1)
@@ -134,10 +161,10 @@ def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True):
     t_plain = time.perf_counter() - t0
     for s in solvers:
         s.close()
-    batch.close()
-    return {"solves": n_solves, "steps": steps, "fluid_cells": cells,
-            "grids": {str(nx): sum(1 for d in draws if d[0] == nx) for nx in classes},
+    return {"arith": "exact" if arith == 0 else "fast", "solves": n_solves, "steps": steps, "fluid_cells": cells,
+            "grids": {str(nx): sum(1 for d in draws if d[0] == nx) for nx in workloads.CYLINDER_CLASSES},
             "mach_range": [min(d[1] for d in draws), max(d[1] for d in draws)],
+            "launch": "per-step hipGraph of the batched kernels, replayed from the host",
             "batched_s": t_batch, "plain_back_to_back_s": t_plain, "batched_vs_plain": t_plain / t_batch,
             "batched_cell_stage_per_s": cells * 3 * steps / t_batch,
             "plain_cell_stage_per_s": cells * 3 * steps / t_plain,
@@ -169,6 +196,33 @@ def double_mach_leg(rh3t, h=1.0 / 320.0, steps=50):
             "fluid_cells": n_fluid, "ghosts": counts[1], "wall_ghosts": counts[2], "steps": steps,
             "ms_per_step": wall / steps * 1e3, "cell_stage_per_s": n_fluid * 3 * steps / wall,
             "kernels_ms_per_step": kern}
+
+
+def plain_leg(rh3t, field, u0, arith, steps, warmup):
+    """The 2048^2 box in one arithmetic: warmup steps (which capture the
+    step's hipGraph), the timed region of `steps` steps of the production
+    loop (rh3t_advance replaying the graph; it returns after a stream sync),
+    then the same steps launched eagerly with hipEvents around every kernel
+    on the solver's stream (the per-kernel breakdown and the roofline)."""
+    log(f"plain 2048 box, arith={arith}")
+    solver = field.solver(arith=rh3t.ARITH_FAST if arith == "fast" else rh3t.ARITH_EXACT)
+    solver.set_state(u0, 0.0)
+    solver.advance(warmup, 0.05)
+    t0 = time.perf_counter()
+    solver.advance(steps, 0.05)
+    wall = time.perf_counter() - t0
+    log(f"timed {steps} steps in {wall:.3f} s")
+    solver.profile(True)
+    t0 = time.perf_counter()
+    solver.advance(steps, 0.05)
+    wall_profiled = time.perf_counter() - t0
+    prof = solver.profile_read()
+    solver.profile(False)
+    u_end, _ = solver.get_state()
+    solver.close()
+    n_fluid = int(field.fluid.sum())
+    return {"rate": n_fluid * 3 * steps / wall, "wall": wall, "wall_profiled": wall_profiled, "prof": prof,
+            "finite": bool(abs(u_end[field.fluid]).max() < 1e300)}
 
 
 def _gpu_ms(profile):
@@ -311,7 +365,8 @@ def main():
     rh3t = importlib.import_module("3t_rh_amd")
     workloads = importlib.import_module("3t_rh_amd.workloads")
     if args.batched_only:
-        print(json.dumps({"batched": batched_leg(rh3t, args.batched_solves, args.batched_steps)}), flush=True)
+        print(json.dumps({"batched": batched_leg(rh3t, args.batched_solves, args.batched_steps,
+                                                 arith=0 if args.arith == "exact" else 1)}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_leg(rh3t, fd_dirs=args.fd_dirs)}), flush=True)
@@ -324,30 +379,13 @@ def main():
     field = workloads.periodic_box(n)
     u0 = workloads.smooth_modes_state(field, seed=3)
     n_fluid = int(field.fluid.sum())
-    solver = field.solver(arith=rh3t.ARITH_FAST if args.arith == "fast" else rh3t.ARITH_EXACT)
-    solver.set_state(u0, 0.0)
-    log("warmup")
-    solver.advance(args.warmup, 0.05)  # also captures the step's hipGraph
-
-    # timed region: K steps of the production loop (rh3t_advance replays the
-    # captured step graph; it returns after a stream sync)
-    t0 = time.perf_counter()
-    solver.advance(args.steps, 0.05)
-    wall = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {wall:.3f} s")
-
-    # the same K steps launched eagerly with hipEvents around every kernel on
-    # the solver's stream: the per-kernel breakdown and the roofline entry
-    solver.profile(True)
-    t0 = time.perf_counter()
-    solver.advance(args.steps, 0.05)
-    wall_profiled = time.perf_counter() - t0
-    prof = solver.profile_read()
-    solver.profile(False)
-    u_end, t_end = solver.get_state()
-    finite = bool(abs(u_end[field.fluid]).max() < 1e300)
-
-    rate = n_fluid * 3 * args.steps / wall
+    head = args.arith
+    other = "exact" if head == "fast" else "fast"
+    legs = {a: plain_leg(rh3t, field, u0, a, args.steps, args.warmup) for a in (head, other)}
+    h = legs[head]
+    rate = h["rate"]
+    log(f"plain: {head} {rate:.4e}, {other} {legs[other]['rate']:.4e} cell-stage/s")
+    prof = h["prof"]
     fx_ms, fx_n = prof["flux_x"]
     interfaces = (n + 1) * n
     achieved = FLOP_PER_INTERFACE * interfaces / (fx_ms * 1e-3)
@@ -358,36 +396,52 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": h["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"smooth-mode periodic box {n}x{n}, 6 FP64 fields, SSP-RK3, dt=0.05*dt_cfl",
-                   "grid": [n, n], "fluid_cells": n_fluid, "arith": args.arith},
+                   "grid": [n, n], "fluid_cells": n_fluid, "arith": head},
+        # agreement with the oracle measured by the GPU tests at this
+        # workload (tests/test_gpu_parity.py, 2048^2 l() and RK3 step) and
+        # where FAST misses the 1e-12 bar
+        "arith_parity": {"bar": 1e-12, "fast_2048_l": 6.9e-13, "exact_2048": "<= 1e-13",
+                         "fast_misses": "cylinder (121x481) full RK3 step: 2.3e-12; every other config <= 1e-12",
+                         "exact_misses": "none"},
         "fp64_frac_of_peak": FLOP_PER_CELL_STAGE * rate / PEAK_FP64,
         "hbm_frac_of_peak": BYTES_PER_CELL_STAGE * rate / PEAK_HBM,
-        "ms_per_step_profiled": wall_profiled / args.steps * 1e3,
+        "ms_per_step_profiled": h["wall_profiled"] / args.steps * 1e3,
         "kernels_ms": {k: round(v[0], 4) for k, v in prof.items() if v[1]},
-        "state_finite": finite,
-        # compute-bound: priced against the dense FP64 peak (78.6 TFLOP/s, the
-        # same for MFMA and vector FP64 on MI355X); the kernel uses VALU FP64.
-        "roofline": {"bound": "mfma", "pipe": "fp64 valu", "kernel": "flux_x", "achieved": achieved / 1e12,
+        "state_finite": h["finite"],
+        other: {"value": legs[other]["rate"], "ms_per_step": legs[other]["wall"] / args.steps * 1e3,
+                "kernels_ms": {k: round(v[0], 4) for k, v in legs[other]["prof"].items() if v[1]},
+                "state_finite": legs[other]["finite"]},
+        # the dominant kernel is bound by FP64 vector (VALU) issue, priced
+        # against the dense FP64 peak (78.6 TFLOP/s: MI355X's vector and
+        # matrix FP64 peaks are equal)
+        "roofline": {"bound": "valu", "pipe": "fp64 valu", "kernel": "flux_x", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
                      "traffic": FLUX_X_TRAFFIC_BYTES if n == 2048 else None,
+                     "traffic_source": TRAFFIC_SOURCE if n == 2048 else None,
                      "flop_per_interface": FLOP_PER_INTERFACE, "interfaces": interfaces,
-                     "avg_launch_ms": fx_ms},
+                     "avg_launch_ms": fx_ms, "launches_timed": fx_n},
     }
     if not args.no_batched:
         log("batched leg")
-        line["batched"] = batched_leg(rh3t, args.batched_solves, args.batched_steps)
+        line["batched"] = batched_leg(rh3t, args.batched_solves, args.batched_steps, arith=0)
+        log(f"batched exact: {json.dumps(line['batched'])}")
+        line["batched_fast"] = batched_leg(rh3t, args.batched_solves, args.batched_steps, arith=1)
+        log(f"batched fast: {json.dumps(line['batched_fast'])}")
     if not args.no_double_mach:
         log("double Mach leg")
         line["double_mach"] = double_mach_leg(rh3t)
+        log(f"double Mach: {json.dumps(line['double_mach'])}")
     if not args.no_adjoint:
         log("adjoint leg")
         line["adjoint"] = adjoint_leg(rh3t, fd_dirs=args.fd_dirs)
+        log(f"adjoint: {json.dumps(line['adjoint'])}")
     if not args.no_cpu:
         log("cpu baseline (oracle, ~10 s of RK3 steps)")
         line["cpu_baseline"] = cpu_baseline(field, u0, workloads)

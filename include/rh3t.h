@@ -239,17 +239,24 @@ int rh3t_calc_global_dt(rh3t_ctx* ctx, double* dt_cfl);
 int rh3t_advance(rh3t_ctx* ctx, int32_t nsteps, double cfl_factor,
                  double dt_fixed, double* time);
 
-/* Arithmetic of the WENO interface flux kernels.  EXACT keeps the
- * reference's rounding sequence (literal divisions, left-to-right sums, no
- * FMA); FAST (the default) uses one reciprocal per shared denominator and
- * FMA chains -- same algebra, agreement with EXACT to roundoff. */
+/* Arithmetic of the WENO interface flux and assembly kernels.  EXACT (the
+ * default of every context and batch) keeps the reference's rounding
+ * sequence (literal divisions, left-to-right sums, no FMA) and agrees with
+ * the reference's l() and rk3_ssp() to 1e-12 on every problem.  FAST uses
+ * one reciprocal per shared denominator and FMA chains: the same algebra,
+ * agreement with EXACT to roundoff, within 1e-12 on smooth periodic boxes
+ * (6.9e-13 at 2048^2) but not on the cylinder's full RK3 step (2.3e-12),
+ * where the boundary WENO weights (eps + beta)^-10 (src/bc1.rs:241) amplify
+ * last-bit differences.  No counterpart in the reference (which has one
+ * arithmetic). */
 #define RH3T_ARITH_EXACT 0
 #define RH3T_ARITH_FAST 1
 int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode);
 
 /* Replaces the physics constants (src/constant.rs:1-25 as a runtime
  * struct) of an existing context, e.g. to evaluate the operator at other
- * exchange or diffusion coefficients.  Problem tables are kept. */
+ * exchange or diffusion coefficients.  Problem tables are kept; the next
+ * rh3t_advance recomputes dt from the state with the new constants. */
 int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys);
 
 /* Workgroup shape of the WENO flux / assembly kernels (bx lanes along y,
@@ -294,6 +301,8 @@ int rh3t_batch_get_state(rh3t_batch* b, int32_t solve, double* u_aos,
  * NULL) receives each solve's Field.time. */
 int rh3t_batch_advance(rh3t_batch* b, int32_t nsteps, double cfl_factor,
                        double* times);
+/* rh3t_ctx_set_arith for every solve of the batch (EXACT by default). */
+int rh3t_batch_set_arith(rh3t_batch* b, int32_t mode);
 int rh3t_batch_set_launch(rh3t_batch* b, int32_t bx, int32_t by,
                           int32_t ghosts_per_block);
 int rh3t_batch_last_error(const rh3t_batch* b, int32_t solve,

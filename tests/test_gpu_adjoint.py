@@ -134,6 +134,45 @@ def test_adjoint_checkpoint_schedule(rh3t, workloads):
     s.close()
 
 
+def test_interleaved_rollouts_keep_their_physics(rh3t, workloads):
+    """Two torch rollouts with different exchange coefficients on one
+    solver, the second's forward before the first's backward: the first
+    gradient equals the gradient of the first rollout run alone (bitwise),
+    i.e. each backward differentiates the forward it belongs to."""
+    import torch
+
+    from importlib import import_module
+
+    ta = import_module("3t_rh_amd.torch_adjoint")
+    _say("interleaved torch rollouts, box 32, 6 steps")
+    f, u, _p = _case(workloads, "periodic", 32)
+    phys = rh3t.Physics(**KW_ACTIVE)
+    s = f.solver(physics=phys)
+    dts = _dts(s, u, 6)
+    w = torch.from_numpy(workloads.gaussian_window(f, seed=5))
+    pa = phys.params()
+    pb = pa * np.array([3.0, 0.2, 2.0, 0.5, 1.5])
+
+    def run(pv):
+        u0 = torch.tensor(u, requires_grad=True)
+        params = torch.tensor(pv, requires_grad=True)
+        un = ta.rollout(s, u0, params, dts, checkpoint_every=2)
+        return u0, params, (w * un[..., 5]).sum()
+
+    u0a, qa, la = run(pa)
+    la.backward()
+    solo = (u0a.grad.numpy().copy(), qa.grad.numpy().copy())
+    u0a, qa, la = run(pa)
+    u0b, qb, lb = run(pb)
+    la.backward()
+    lb.backward()
+    assert np.array_equal(u0a.grad.numpy(), solo[0]) and np.array_equal(qa.grad.numpy(), solo[1])
+    assert not np.array_equal(u0b.grad.numpy(), solo[0])
+    # the solver keeps the constants of its last set_physics
+    assert s.physics.params().tolist() == pb.tolist()
+    s.close()
+
+
 def test_heavy_ghosts_are_rejected(rh3t, workloads):
     """The reverse mode covers the cheap ghost kinds; FarField / Outflow /
     Wall problems must fail loudly rather than return a partial gradient."""

@@ -1,6 +1,8 @@
 """Batched solves on the GPU: every member of a ragged batch must evolve
 exactly as the plain solver does on its own problem (same kernels, same
-arithmetic: bitwise equality), whatever the launch shape."""
+arithmetic: bitwise equality), whatever the launch shape and in both
+arithmetic variants; members against the oracle at 1e-12 (EXACT, the
+default), including the full 64-solve workload of the benchmark."""
 import numpy as np
 import pytest
 
@@ -26,15 +28,16 @@ def members(rh3t, workloads):
     return fields, states
 
 
-def test_batch_matches_plain_bitwise(rh3t, members):
+@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
+def test_batch_matches_plain_bitwise(rh3t, members, arith):
     fields, states = members
-    _say("batch of 4 (box 48, cylinders 121 x2 Mach 3 / 2.3, 161 Mach 3.7), 3 steps")
-    b = rh3t.Batch(fields)
+    _say(f"batch of 4 (box 48, cylinders 121 x2 Mach 3 / 2.3, 161 Mach 3.7), 3 steps, arith={arith}")
+    b = rh3t.Batch(fields, arith=arith)
     for k, u in enumerate(states):
         b.set_state(k, u, 0.0)
     times = b.advance(3, 0.05)
     for k, (f, u) in enumerate(zip(fields, states)):
-        s = f.solver()
+        s = f.solver(arith=arith)
         s.set_state(u, 0.0)
         t = s.advance(3, 0.05)
         ref, _ = s.get_state()
@@ -98,5 +101,49 @@ def test_batch_member_against_oracle(rh3t, members):
     assert abs(t - tref) <= 1e-15 * tref
     err = ob.relerr(got, ref, f.fluid)
     print("relerr", err)
-    assert err.max() <= 1e-11
+    assert err.max() <= 1e-12
+    b.close()
+
+
+def test_batched_workload_64(rh3t, workloads):
+    """The benchmark's batched workload itself: 64 half-cylinder solves drawn
+    with seed 4 over all four grid classes (121x481 ... 241x961) and Mach
+    ~ U[2, 4], EXACT arithmetic.  Three batched steps: every member bitwise
+    equal to its plain solve; then one member per grid class, resynced
+    every step, against the oracle's RK3 step at 1e-12 (the batch advances
+    every member with its own dt = 0.05 dt_cfl)."""
+    draws = workloads.batched_draw(64, seed=4)
+    fields, states = workloads.batched_cylinders(draws)
+    assert {nx for nx, _ in draws} == set(workloads.CYLINDER_CLASSES)
+    _say("batched workload: 64 solves, 3 steps, bitwise against plain solves")
+    b = rh3t.Batch(fields)
+    for k, u in enumerate(states):
+        b.set_state(k, u, 0.0)
+    times = b.advance(3, 0.05)
+    for k, (f, u) in enumerate(zip(fields, states)):
+        s = f.solver()
+        s.set_state(u, 0.0)
+        t = s.advance(3, 0.05)
+        got, tb = b.get_state(k)
+        ref, _ = s.get_state()
+        s.close()
+        assert tb == t == times[k], k
+        assert np.array_equal(got, ref), (k, draws[k], np.abs(got - ref).max())
+    picks = [next(k for k, d in enumerate(draws) if d[0] == nx) for nx in workloads.CYLINDER_CLASSES]
+    _say(f"batched workload: members {picks} resynced against the oracle, 3 steps")
+    us = [states[k] for k in picks]
+    probs = [ob.OracleProblem(fields[k]) for k in picks]
+    for step in range(3):
+        for k, u in zip(picks, us):
+            b.set_state(k, u, 0.0)
+        b.advance(1, 0.05)
+        for q, k in enumerate(picks):
+            dt = 0.05 * ob.global_dt(probs[q], us[q])
+            ref, tref = ob.rk3_step(probs[q], us[q], dt, nthreads=16)
+            got, t = b.get_state(k)
+            assert abs(t - tref) <= 1e-15 * tref
+            err = ob.relerr(got, ref, fields[k].fluid)
+            print("step", step, "member", k, draws[k], "relerr", err.max(), flush=True)
+            assert err.max() <= 1e-12, (step, k, err)
+            us[q] = ref
     b.close()

@@ -306,20 +306,135 @@ def test_translating_shock_1024(rh3t, workloads):
     front1 = (rho > 1.5).sum(axis=0).mean()
     print("post-shock cells per row", front0, "->", front1, "time", t)
     assert front1 > front0
+    # y-periodicity: the run from the initial state shifted by half the
+    # period in y is the shifted run (the Periodic sides wrap; the Outflow
+    # ghosts' E_r stencils near the corners are not shift-invariant, so
+    # roundoff, not bitwise)
+    _say("translating shock 1024: 100 steps from the y-shifted state")
+    half = f.grid.ny // 2
+    s.set_state(np.roll(u, half, axis=1), 0.0)
+    t2 = s.advance(100, 0.05)
+    got2, _ = s.get_state()
+    shifted = np.roll(got, half, axis=1)
+    err = ob.relerr(got2, shifted, f.fluid)
+    same = np.mean(got2[f.fluid] == shifted[f.fluid])
+    print("y-shift relerr", err, "bitwise-equal fraction", same, "time", t2)
+    assert abs(t2 - t) <= 1e-14 * t
+    assert err.max() <= 1e-12
     s.close()
 
 
-def test_l_periodic_box_2048_oracle(rh3t, workloads):
-    """The roofline workload itself: l() of the 2048^2 smooth-mode box
-    against the oracle (FAST, 1e-12)."""
+@pytest.fixture(scope="module")
+def box2048(rh3t, workloads):
     f = workloads.periodic_box(2048)
-    u = workloads.smooth_modes_state(f, seed=3)
-    _say("l() periodic box 2048 vs oracle")
-    s = f.solver()
+    return f, workloads.smooth_modes_state(f, seed=3), ob.OracleProblem(f)
+
+
+@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
+def test_l_periodic_box_2048_oracle(rh3t, box2048, arith):
+    """The roofline workload itself: l() of the 2048^2 smooth-mode box
+    against the oracle (1e-12; EXACT 1e-13)."""
+    f, u, p = box2048
+    _say(f"l() periodic box 2048 vs oracle arith={arith}")
+    s = f.solver(arith=arith)
     s.set_state(u)
     rhs = s.l()
-    ref = ob.l(ob.OracleProblem(f), u, nthreads=16)
+    ref = ob.l(p, u, nthreads=16)
     err = ob.relerr(rhs, ref, f.fluid)
     print("relerr", err)
+    assert err.max() <= (TOL_EXACT if arith == 0 else TOL)
+    s.close()
+
+
+@pytest.mark.parametrize("arith", [0, 1], ids=["exact", "fast"])
+def test_rk3_periodic_box_2048_oracle(rh3t, box2048, arith):
+    """One full RK3 step of the 2048^2 box (the benchmarked step, dt =
+    0.05 dt_cfl from the device's CFL reduction) against the oracle."""
+    f, u, p = box2048
+    _say(f"rk3 step periodic box 2048 vs oracle arith={arith}")
+    s = f.solver(arith=arith)
+    s.set_state(u, 0.0)
+    dt_o = ob.global_dt(p, u)
+    assert abs(s.calc_global_dt() - dt_o) <= 1e-15 * dt_o
+    t = s.advance(1, 0.05)
+    got, _ = s.get_state()
+    ref, tref = ob.rk3_step(p, u, 0.05 * dt_o, nthreads=16)
+    assert t == tref
+    err = ob.relerr(got, ref, f.fluid)
+    print("relerr", err)
     assert err.max() <= TOL
+    s.close()
+
+
+# ------------------------------------------------ failure paths
+
+def test_nonfinite_ghost_error(rh3t, cyl):
+    """A NaN in one fluid cell inside FarField E_r stencils: l() fails with
+    the reference's non-finite ghost error (src/ghost.rs:229-263) at the same
+    first ghost id as the oracle (the smallest failing id), and every kernel
+    runs to completion (a data-driven error word)."""
+    f, u, p = cyl
+    gh = f.ghosts()
+    # a FarField ghost above the top side: the flow runs along it, so it has
+    # outgoing characteristics that carry the extrapolated interior (on the
+    # inflow side every characteristic takes u_inf and a NaN never arrives)
+    top = np.nonzero((gh["bc_kind"] == rh3t.BC_FARFIELD) & (gh["j"] >= f.grid.ny))[0]
+    k = int(top[len(top) // 2])
+    lam = ob.heavy_terms(p, u, k)["lam"]
+    assert (lam >= 0.0).any()
+    table = f.ghost_bc(int(gh["heavy_id"][k]))
+    cell = int(table.stencil[30])  # an r = 4 point of that ghost's E_r
+    bad = u.copy()
+    bad.reshape(-1, 6)[cell, 3] = np.nan
+    _say(f"error path: NaN at linear cell {cell} (FarField ghost {k}'s stencil)")
+    s = f.solver()
+    s.set_state(bad)
+    with pytest.raises(rh3t.RH3TError) as ei:
+        s.l()
+    with pytest.raises(ob.OracleError) as eo:
+        ob.l(p, bad)
+    g, o = ei.value.error, eo.value.info
+    print("gpu", g, "oracle", o)
+    assert ei.value.status == rh3t.ERR_GHOST and eo.value.status == 4
+    assert g["ghost_id"] == o["ghost_id"]
+    assert g["stage"] == 0
+    # the context stays usable
+    s.set_state(u)
+    assert np.isfinite(s.l()).all()
+    s.close()
+
+
+def test_nonpositive_dt_error(rh3t, box64):
+    """dt <= 0 is the reference's "non-positive dt" panic (src/main.rs:464):
+    from the host (rk3_ssp with dt = 0) and from the device's CFL rule
+    (advance with a zero CFL factor: dt = 0 * dt_cfl)."""
+    f, u, p = box64
+    _say("error path: non-positive dt")
+    s = f.solver()
+    s.set_state(u, 0.0)
+    with pytest.raises(rh3t.RH3TError) as ei:
+        s.rk3_ssp(0.0)
+    assert ei.value.status == rh3t.ERR_DT
+    with pytest.raises(rh3t.RH3TError) as ei:
+        s.advance(1, 0.0)
+    assert ei.value.status == rh3t.ERR_DT
+    s.close()
+
+
+def test_set_physics_recomputes_dt(rh3t, box64):
+    """After a step, new physics constants (a stiff exchange) must take part
+    in the next dt: advance, set_physics, advance one step; the step's dt is
+    0.05 dt_cfl of the new constants at the state it starts from."""
+    f, u, p = box64
+    _say("set_physics then advance: dt of the new constants")
+    s = f.solver()
+    s.set_state(u, 0.0)
+    s.advance(2, 0.05)
+    u2, t2 = s.get_state()
+    kw = dict(omega_ei=400.0, omega_er=50.0)
+    s.set_physics(rh3t.Physics(**kw))
+    t3 = s.advance(1, 0.05)
+    dt_o = ob.global_dt(p, u2, phys=ob.physics(**kw))
+    assert dt_o < 0.5 * ob.global_dt(p, u2)
+    assert abs((t3 - t2) - 0.05 * dt_o) <= 1e-13 * dt_o
     s.close()

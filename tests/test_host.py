@@ -35,6 +35,45 @@ def test_library_exports_every_header_symbol(rh3t):
     assert set(names) == set(rh3t.SIGNATURES), set(names) ^ set(rh3t.SIGNATURES)
 
 
+def _rust_block():
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```rust\n(.*?)```", text, flags=re.S)
+    return next(b for b in blocks if 'extern "C"' in b)
+
+
+def header_structs():
+    """typedef struct { ... } name; of include/rh3t.h: name -> field names."""
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "rh3t.h")).read(), flags=re.S)
+    out = {}
+    for body, name in re.findall(r"typedef struct\s*\{(.*?)\}\s*(rh3t_[a-z_]+)\s*;", text, flags=re.S):
+        fields = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            names = decl.split(None, 1)[1]
+            fields += [re.sub(r"\[.*?\]", "", n).strip().lstrip("*") for n in names.split(",")]
+        out[name] = fields
+    return out
+
+
+def test_rust_binding_matches_header():
+    """The extern "C" block a maintainer would add to the reference crate
+     binds exactly the header's functions, and its
+    #[repr(C)] structs carry the header's fields in the header's order."""
+    rust = _rust_block()
+    fns = set(re.findall(r"pub fn (rh3t_[a-z0-9_]+)\(", rust))
+    assert fns == set(header_functions()), fns ^ set(header_functions())
+    rstructs = {}
+    for name, body in re.findall(r"pub struct (rh3t_[a-z_]+)\s*\{(.*?)\}", rust, flags=re.S):
+        rstructs[name] = re.findall(r"pub ([a-z_0-9]+)\s*:", body)
+    for name, fields in header_structs().items():
+        assert name in rstructs, name
+        assert rstructs[name] == fields, (name, rstructs[name], fields)
+    for opaque in ("rh3t_problem", "rh3t_ctx", "rh3t_batch", "rh3t_adjoint"):
+        assert opaque in rstructs
+
+
 def test_no_device_fails_loudly(rh3t, workloads):
     if rh3t.device_count() > 0:
         pytest.skip("a GPU is visible")
