@@ -34,11 +34,11 @@ LIB_PATH = os.path.join(_HERE, "librh3t.so")
 
 OK = 0
 ERR_ARG, ERR_HIP, ERR_NONPHYSICAL, ERR_GHOST, ERR_GEOMETRY = 1, 2, 3, 4, 5
-ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_SINGULAR, ERR_DT = 6, 7, 8, 9
+ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_SINGULAR, ERR_DT, ERR_SIZE = 6, 7, 8, 9, 10
 STATUS_NAMES = {
     0: "ok", 1: "bad argument", 2: "HIP error", 3: "non-physical state", 4: "non-finite ghost state",
     5: "geometry/ghost build error", 6: "unsupported", 7: "no HIP device", 8: "singular ILW system",
-    9: "non-positive dt",
+    9: "non-positive dt", 10: "input beyond a fixed-capacity entry point",
 }
 
 BC_REFLECTIVE_WALL, BC_WALL, BC_PERIODIC, BC_CONSTANT = 0, 1, 2, 3
@@ -123,6 +123,8 @@ SIGNATURES = {
     "rh3t_l": (ctypes.c_int, [_P, _DP, _DP]),
     "rh3t_rk3_step": (ctypes.c_int, [_P, ctypes.c_double]),
     "rh3t_calc_global_dt": (ctypes.c_int, [_P, _DP]),
+    "rh3t_calc_global_dt_noalloc": (ctypes.c_int, [_P, _P, ctypes.POINTER(c_grid), ctypes.POINTER(c_physics), _P,
+                                                   _P]),
     "rh3t_advance": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _DP]),
     "rh3t_ctx_set_arith": (ctypes.c_int, [_P, ctypes.c_int32]),
     "rh3t_ctx_set_launch": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),

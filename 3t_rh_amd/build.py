@@ -55,7 +55,7 @@ def build(verbose=False, resource_usage=False):
     extra = ["-Rpass-analysis=kernel-resource-usage"] if resource_usage else []
     out = ""
     dev_objs = []
-    for name in ("engine", "adjoint"):
+    for name in ("engine", "adjoint", "noalloc"):
         obj = os.path.join(BUILD, f"{name}.o")
         out += _run([_hipcc(), *HIPCC_FLAGS, *inc, *extra, "-c", os.path.join(CSRC, f"{name}.hip"),
                      "-o", obj], verbose)

@@ -1098,6 +1098,8 @@ EigConst make_eig(const Phys& ph) {
 
 namespace rh3t_eng {
 
+Phys physics_of(const rh3t_physics& p) { return make_phys(p); }
+
 int apply_physics(Engine& E, const rh3t_physics& p) {
   E.src_phys = p;
   E.phys = make_phys(p);

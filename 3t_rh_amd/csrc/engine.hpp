@@ -207,6 +207,8 @@ void free_adjoint(Engine& E);
 // eigensystem constants, the Derived plane count); drops the step graph and
 // the dt minima of the old constants.  The stream must be idle.
 int apply_physics(Engine& E, const rh3t_physics& p);
+// the device constants (gamma - 1, gates, ...) of a caller's physics struct
+Phys physics_of(const rh3t_physics& p);
 // selects the flux / assembly arithmetic (drops the step graph on change)
 void set_arith(Engine& E, bool fast);
 

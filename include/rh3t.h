@@ -41,7 +41,8 @@ typedef enum {
   RH3T_ERR_UNSUPPORTED = 6,  /* feature outside this build (TimeDependent BC) */
   RH3T_ERR_NO_DEVICE = 7,    /* no HIP device visible                         */
   RH3T_ERR_SINGULAR = 8,     /* singular 6x6 Wall-ILW system, src/bc1.rs:468  */
-  RH3T_ERR_DT = 9            /* non-positive dt, src/main.rs:464              */
+  RH3T_ERR_DT = 9,           /* non-positive dt, src/main.rs:464              */
+  RH3T_ERR_SIZE = 10         /* input beyond a fixed-capacity entry point     */
 } rh3t_status;
 
 /* BCType, src/bc1.rs:16-28 (tagged union flattened to POD). */
@@ -277,6 +278,26 @@ int rh3t_profile(rh3t_ctx* ctx, int32_t enable);
 int rh3t_profile_read(rh3t_ctx* ctx, int32_t max_kernels, char* names,
                       int32_t names_cap, double* avg_ms, int64_t* launches,
                       int32_t* n_kernels);
+
+/* calc_global_dt (src/main.rs:375-408) of a state that already lives in
+ * device memory, on the caller's stream, allocating nothing.  No
+ * counterpart in the reference: it is the variant of rh3t_calc_global_dt
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops) and hold the state themselves.  Device
+ * pointers, on the current device: u_aos (nx*ny*6 doubles, AoS at
+ * l = i*ny + j), fluid (nx*ny bytes, nonzero = fluid cell), dt_cfl (one
+ * double, written).  grid and phys (NULL: src/constant.rs defaults) are
+ * host pointers; stream is a hipStream_t (NULL: the null stream).  The
+ * per-workgroup minima live in a fixed device array of the library (one
+ * per GPU, held until the process ends): nx*ny above
+ * RH3T_NOALLOC_MAX_CELLS returns RH3T_ERR_SIZE and launches nothing.
+ * Calls that overlap on one GPU share that array and race; a call's result
+ * is complete when its stream has reached it. */
+#define RH3T_NOALLOC_MAX_CELLS (1 << 25)
+int rh3t_calc_global_dt_noalloc(const double* u_aos, const uint8_t* fluid,
+                                const rh3t_grid* grid,
+                                const rh3t_physics* phys, double* dt_cfl,
+                                void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Batched solves: many independent problems (uneven grids, different      */
