@@ -44,6 +44,24 @@ struct Phys {
 enum { D_U = 0, D_V, D_EE, D_EI, D_ER, D_PE, D_PI, D_PR, D_TE, D_TI, D_TR, D_N };
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
+
+// a / b rounded to nearest, from y = RN(1/b) (a reciprocal shared by several
+// quotients, or a constant): q0 = RN(a y) lies within 1.5 ulp of a / b; the
+// FMA correction q' = RN(q + (a - b q) y), whose residual a - b q is exact,
+// brings q0 within one ulp, and applied once more returns RN(a / b)
+// (Markstein's theorem: y within half an ulp of 1/b and q within one ulp of
+// a / b give a correctly rounded q').  It therefore equals the IEEE quotient
+// bit for bit whenever a, b and a / b are finite and the residuals stay in
+// the normal range (|a| >= 2^-969 or a = 0; a = -0 gives +0): 5 FP64 FMA /
+// multiply instructions instead of the 11 of a division with its
+// quarter-rate reciprocal.
+__host__ __device__ __forceinline__ double div_rn(double a, double b, double y) {
+  const double q0 = a * y;
+  const double q1 = fma(fma(-b, q0, a), y, q0);
+  return fma(fma(-b, q1, a), y, q1);
+}
+constexpr double kRcp3 = 1.0 / 3.0;  // RN(1/3)
+constexpr double kRcp6 = 1.0 / 6.0;  // RN(1/6)
 // x.powi(3) = x * (x*x), x.powi(4) = (x*x)*(x*x)
 __device__ __forceinline__ double cube(double x) { return x * (x * x); }
 __device__ __forceinline__ double pow4(double x) { double x2 = x * x; return x2 * x2; }
@@ -81,33 +99,40 @@ __device__ __forceinline__ double weno5(double u0, double u1, double u2, double 
   double a1 = 0.6 / sq(eps + beta1);
   double a2 = 0.3 / sq(eps + beta2);
   double sum_o = a0 + a1 + a2;
-  double w0 = a0 / sum_o, w1 = a1 / sum_o, w2 = a2 / sum_o;
-  double p0 = u0 / 3.0 - 7.0 / 6.0 * u1 + 11.0 / 6.0 * u2;
-  double p1 = -u1 / 6.0 + 5.0 / 6.0 * u2 + u3 / 3.0;
-  double p2 = u2 / 3.0 + 5.0 / 6.0 * u3 - u4 / 6.0;
+  // the quotients by sum_o, 3 and 6 through their reciprocals (div_rn:
+  // the same roundings as the literal divisions)
+  const double ys = 1.0 / sum_o;
+  double w0 = div_rn(a0, sum_o, ys), w1 = div_rn(a1, sum_o, ys), w2 = div_rn(a2, sum_o, ys);
+  double p0 = div_rn(u0, 3.0, kRcp3) - 7.0 / 6.0 * u1 + 11.0 / 6.0 * u2;
+  double p1 = div_rn(-u1, 6.0, kRcp6) + 5.0 / 6.0 * u2 + div_rn(u3, 3.0, kRcp3);
+  double p2 = div_rn(u2, 3.0, kRcp3) + 5.0 / 6.0 * u3 - div_rn(u4, 6.0, kRcp6);
   return w0 * p0 + w1 * p1 + w2 * p2;
 }
 
 // Energy-mode row of L, src/weno.rs:110-233 (identical in X and Y).
+// The quotients by 6 cs2, 3 cs2, gt cs2 and b go through div_rn with the
+// reciprocals y6 = RN(1/(6 cs2)) (RN(1/(3 cs2)) = 2 y6 exactly), ygt =
+// RN(1/(gt cs2)) and yb = RN(1/b): the same roundings as the divisions.
 __device__ __forceinline__ void energy_row(int mode, double hk, double u, double v, double w2,
                                            double cs2, double gt, double b, double ge, double gi,
-                                           double gr, double row[6]) {
+                                           double gr, double y6, double ygt, double yb, double row[6]) {
   double gk = mode == 0 ? ge : (mode == 1 ? gi : gr);
-  row[0] = gk / gt - gk * w2 / (6.0 * cs2) - gk * gt * w2 / b * (hk - cs2 * w2);
-  row[1] = gk * u / (3.0 * cs2) + 2.0 * gk * gt * u / b * (hk - cs2 * w2);
-  row[2] = gk * v / (3.0 * cs2) + 2.0 * gk * gt * v / b * (hk - cs2 * w2);
+  const double c6 = 6.0 * cs2, c3 = 3.0 * cs2, y3 = 2.0 * y6, gtc = gt * cs2;
+  row[0] = gk / gt - div_rn(gk * w2, c6, y6) - div_rn(gk * gt * w2, b, yb) * (hk - cs2 * w2);
+  row[1] = div_rn(gk * u, c3, y3) + div_rn(2.0 * gk * gt * u, b, yb) * (hk - cs2 * w2);
+  row[2] = div_rn(gk * v, c3, y3) + div_rn(2.0 * gk * gt * v, b, yb) * (hk - cs2 * w2);
   if (mode == 0) {
-    row[3] = -sq(ge) / (gt * cs2) - 6.0 * ge / b * (ge * hk - gt * cs2 * w2);
-    row[4] = -ge * gi / (gt * cs2) - 6.0 * ge * gi / b * (hk - 6.0 * gr * cs2);
-    row[5] = -ge * gr / (gt * cs2) - 6.0 * ge * gr / b * (hk + 6.0 * gi * cs2);
+    row[3] = div_rn(-sq(ge), gtc, ygt) - div_rn(6.0 * ge, b, yb) * (ge * hk - gt * cs2 * w2);
+    row[4] = div_rn(-ge * gi, gtc, ygt) - div_rn(6.0 * ge * gi, b, yb) * (hk - 6.0 * gr * cs2);
+    row[5] = div_rn(-ge * gr, gtc, ygt) - div_rn(6.0 * ge * gr, b, yb) * (hk + 6.0 * gi * cs2);
   } else if (mode == 1) {
-    row[3] = -gi * ge / (gt * cs2) - 6.0 * gi * ge / b * (hk + 6.0 * gr * cs2);
-    row[4] = -sq(gi) / (gt * cs2) - 6.0 * gi / b * (gi * hk - gt * cs2 * w2);
-    row[5] = -gi * gr / (gt * cs2) - 6.0 * gi * gr / b * (hk - 6.0 * ge * cs2);
+    row[3] = div_rn(-gi * ge, gtc, ygt) - div_rn(6.0 * gi * ge, b, yb) * (hk + 6.0 * gr * cs2);
+    row[4] = div_rn(-sq(gi), gtc, ygt) - div_rn(6.0 * gi, b, yb) * (gi * hk - gt * cs2 * w2);
+    row[5] = div_rn(-gi * gr, gtc, ygt) - div_rn(6.0 * gi * gr, b, yb) * (hk - 6.0 * ge * cs2);
   } else {
-    row[3] = -gr * ge / (gt * cs2) - 6.0 * gr * ge / b * (hk - 6.0 * gi * cs2);
-    row[4] = -gr * gi / (gt * cs2) - 6.0 * gr * gi / b * (hk + 6.0 * ge * cs2);
-    row[5] = -sq(gr) / (gt * cs2) - 6.0 * gr / b * (gr * hk - gt * cs2 * w2);
+    row[3] = div_rn(-gr * ge, gtc, ygt) - div_rn(6.0 * gr * ge, b, yb) * (hk - 6.0 * gi * cs2);
+    row[4] = div_rn(-gr * gi, gtc, ygt) - div_rn(6.0 * gr * gi, b, yb) * (hk + 6.0 * ge * cs2);
+    row[5] = div_rn(-sq(gr), gtc, ygt) - div_rn(6.0 * gr, b, yb) * (gr * hk - gt * cs2 * w2);
   }
 }
 
@@ -119,6 +144,7 @@ __device__ __forceinline__ void energy_row(int mode, double hk, double u, double
 // entry of R, lamx an eigenvalue.
 struct RoeX {
   double u, v, w2, ee, ei, er, cs2, cs, h[3], b, ad;
+  double y6, yb, ygt;  // RN(1/(6 cs2)), RN(1/b), RN(1/(gt cs2)) for div_rn
 };
 
 __device__ __forceinline__ void roe_x(const Phys& ph, double rho3, double rho2, double u3, double v3, double ee3,
@@ -127,12 +153,13 @@ __device__ __forceinline__ void roe_x(const Phys& ph, double rho3, double rho2, 
   double r1 = sqrt(rho3);
   double r2 = sqrt(rho2);
   double rs = r1 + r2;
-  Q.u = (u3 * r1 + u2 * r2) / rs;
-  Q.v = (v3 * r1 + v2 * r2) / rs;
+  const double yrs = 1.0 / rs;  // the five divisions by rs through div_rn
+  Q.u = div_rn(u3 * r1 + u2 * r2, rs, yrs);
+  Q.v = div_rn(v3 * r1 + v2 * r2, rs, yrs);
   Q.w2 = sq(Q.u) + sq(Q.v);
-  Q.ee = (ee3 * r1 + ee2 * r2) / rs;
-  Q.ei = (ei3 * r1 + ei2 * r2) / rs;
-  Q.er = (er3 * r1 + er2 * r2) / rs;
+  Q.ee = div_rn(ee3 * r1 + ee2 * r2, rs, yrs);
+  Q.ei = div_rn(ei3 * r1 + ei2 * r2, rs, yrs);
+  Q.er = div_rn(er3 * r1 + er2 * r2, rs, yrs);
   const double ge = ph.ge, gi = ph.gi, gr = ph.gr, gt = ph.gt;
   Q.cs2 = ph.gamma_e * ge * Q.ee + ph.gamma_i * gi * Q.ei + ph.gamma_r * gr * Q.er;
   Q.cs = sqrt(Q.cs2);
@@ -141,30 +168,35 @@ __device__ __forceinline__ void roe_x(const Phys& ph, double rho3, double rho2, 
   Q.h[2] = 6.0 * ge * gi * (ph.gamma_e * Q.ee - ph.gamma_i * Q.ei) + ph.gamma_r * gt * Q.er * Q.w2;
   Q.b = gt * (36.0 * ge * gi * gr + gt * sq(Q.w2)) * Q.cs2;
   Q.ad = 12.0 * Q.cs2;
+  Q.y6 = 1.0 / (6.0 * Q.cs2);
+  Q.yb = 1.0 / Q.b;
+  Q.ygt = 1.0 / (gt * Q.cs2);
 }
 
 // row c of L (X rows: u-c, electron, ion, radiation, transverse, u+c)
 __device__ __forceinline__ void lx_row(const Phys& ph, const RoeX& Q, int c, double row[6]) {
   const double gt = ph.gt;
   if (c == 0 || c == 5) {
+    // ad = 12 cs2 = 2 (6 cs2) exactly, so RN(1/ad) = y6 / 2
+    const double yad = 0.5 * Q.y6;
     if (c == 0) {
-      row[0] = (gt * Q.w2 + 6.0 * Q.u * Q.cs) / Q.ad;
-      row[1] = (-2.0 * gt * Q.u - 6.0 * Q.cs) / Q.ad;
+      row[0] = div_rn(gt * Q.w2 + 6.0 * Q.u * Q.cs, Q.ad, yad);
+      row[1] = div_rn(-2.0 * gt * Q.u - 6.0 * Q.cs, Q.ad, yad);
     } else {
-      row[0] = (gt * Q.w2 - 6.0 * Q.u * Q.cs) / Q.ad;
-      row[1] = (-2.0 * gt * Q.u + 6.0 * Q.cs) / Q.ad;
+      row[0] = div_rn(gt * Q.w2 - 6.0 * Q.u * Q.cs, Q.ad, yad);
+      row[1] = div_rn(-2.0 * gt * Q.u + 6.0 * Q.cs, Q.ad, yad);
     }
-    row[2] = (-2.0 * gt * Q.v) / Q.ad;
-    row[3] = 6.0 * ph.ge / Q.ad;
-    row[4] = 6.0 * ph.gi / Q.ad;
-    row[5] = 6.0 * ph.gr / Q.ad;
+    row[2] = div_rn(-2.0 * gt * Q.v, Q.ad, yad);
+    row[3] = div_rn(6.0 * ph.ge, Q.ad, yad);
+    row[4] = div_rn(6.0 * ph.gi, Q.ad, yad);
+    row[5] = div_rn(6.0 * ph.gr, Q.ad, yad);
   } else if (c == 4) {
     row[0] = -Q.v;
     row[1] = 0.0;
     row[2] = 1.0;
     row[3] = row[4] = row[5] = 0.0;
   } else {
-    energy_row(c - 1, Q.h[c - 1], Q.u, Q.v, Q.w2, Q.cs2, gt, Q.b, ph.ge, ph.gi, ph.gr, row);
+    energy_row(c - 1, Q.h[c - 1], Q.u, Q.v, Q.w2, Q.cs2, gt, Q.b, ph.ge, ph.gi, ph.gr, Q.y6, Q.ygt, Q.yb, row);
   }
 }
 
@@ -175,12 +207,12 @@ __device__ __forceinline__ double rx_entry(const Phys& ph, const RoeX& Q, int a,
   if (a == 1) return c == 0 ? u - cs : (c == 5 ? u + cs : (c == 4 ? 0.0 : u));
   if (a == 2) return c == 4 ? 1.0 : v;
   const int k = a - 3;  // energy row e, i, r
-  if (c == 4) return v / 3.0;
+  if (c == 4) return div_rn(v, 3.0, kRcp3);
   if (c == 0 || c == 5) {
     const double gam = k == 0 ? ph.gamma_e : (k == 1 ? ph.gamma_i : ph.gamma_r);
     const double ek = k == 0 ? Q.ee : (k == 1 ? Q.ei : Q.er);
-    const double g_ = gam * ek + w2 / 6.0;
-    const double ucs = u * cs / 3.0;
+    const double g_ = gam * ek + div_rn(w2, 6.0, kRcp6);
+    const double ucs = div_rn(u * cs, 3.0, kRcp3);
     return c == 0 ? g_ - ucs : g_ + ucs;
   }
   const int j = c - 1;  // energy column
@@ -246,20 +278,23 @@ __device__ __forceinline__ double jump(const double* a) {
          q_pulse(a[0], a[1], a[2], a[3], a[4], a[5], a[6]);
 }
 
-// sixth_order_derivative, src/noncon.rs:26-44
+// sixth_order_derivative, src/noncon.rs:26-44; d60 = 60 dx, y60 = RN(1/d60)
 __device__ __forceinline__ double sixth(double qm3, double qm2, double qm1, double qp1, double qp2,
-                                        double qp3, double dx) {
-  return (qp3 - 9.0 * qp2 + 45.0 * qp1 - 45.0 * qm1 + 9.0 * qm2 - qm3) / (60.0 * dx);
+                                        double qp3, double d60, double y60) {
+  return div_rn(qp3 - 9.0 * qp2 + 45.0 * qp1 - 45.0 * qm1 + 9.0 * qm2 - qm3, d60, y60);
 }
 
 // nonconservative_direction_pre, src/noncon.rs:288-332.  vel[k], pe/pi/pr[k]
-// at the 9 points d-4..d+4; writes the ee, ei, er components.
+// at the 9 points d-4..d+4; writes the ee, ei, er components.  yds =
+// RN(1/ds) and y60 = RN(1/(60 ds)) (the divisions by ds and 60 ds through
+// div_rn, which rounds as the literal divisions do).
 __device__ __forceinline__ void noncon(const double vel[9], const double pe[9], const double pi[9],
-                                       const double pr[9], double ds, double out[3]) {
+                                       const double pr[9], double ds, double yds, double y60, double out[3]) {
   double v = vel[4];
-  double dpe = sixth(pe[1], pe[2], pe[3], pe[5], pe[6], pe[7], ds);
-  double dpi = sixth(pi[1], pi[2], pi[3], pi[5], pi[6], pi[7], ds);
-  double dpr = sixth(pr[1], pr[2], pr[3], pr[5], pr[6], pr[7], ds);
+  const double d60 = 60.0 * ds;
+  double dpe = sixth(pe[1], pe[2], pe[3], pe[5], pe[6], pe[7], d60, y60);
+  double dpi = sixth(pi[1], pi[2], pi[3], pi[5], pi[6], pi[7], d60, y60);
+  double dpr = sixth(pr[1], pr[2], pr[3], pr[5], pr[6], pr[7], d60, y60);
   double der_ee = 2.0 * dpe - dpi - dpr;
   double der_ei = 2.0 * dpi - dpe - dpr;
   double der_er = 2.0 * dpr - dpe - dpi;
@@ -270,11 +305,12 @@ __device__ __forceinline__ void noncon(const double vel[9], const double pe[9], 
     nei[k] = 2.0 * pi[k] - pe[k] - pr[k];
     ner[k] = 2.0 * pr[k] - pe[k] - pi[k];
   }
-  double cl = fmax(fmax(v, vel[3]), 0.0) / 3.0;
-  double cr = fmin(fmin(v, vel[5]), 0.0) / 3.0;
-  out[0] = v / 3.0 * der_ee + (cl * jump(nee) + cr * jump(nee + 1)) / ds;
-  out[1] = v / 3.0 * der_ei + (cl * jump(nei) + cr * jump(nei + 1)) / ds;
-  out[2] = v / 3.0 * der_er + (cl * jump(ner) + cr * jump(ner + 1)) / ds;
+  double cl = div_rn(fmax(fmax(v, vel[3]), 0.0), 3.0, kRcp3);
+  double cr = div_rn(fmin(fmin(v, vel[5]), 0.0), 3.0, kRcp3);
+  const double v3 = div_rn(v, 3.0, kRcp3);
+  out[0] = v3 * der_ee + div_rn(cl * jump(nee) + cr * jump(nee + 1), ds, yds);
+  out[1] = v3 * der_ei + div_rn(cl * jump(nei) + cr * jump(nei + 1), ds, yds);
+  out[2] = v3 * der_er + div_rn(cl * jump(ner) + cr * jump(ner + 1), ds, yds);
 }
 
 // noncon with the divisions as products: by 3 through 1/3, by ds and 60 ds

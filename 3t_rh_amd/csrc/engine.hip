@@ -187,7 +187,8 @@ __device__ __forceinline__ void rx_apply(const Phys& ph, const RoeX& Q, const do
 // register budget of the common case); the engine keeps the two kinds in
 // separate ranges [h0, h0 + count) of one list.
 constexpr int kMaxGhostsPerBlock = 4;
-constexpr int kGhostLds = 336 + 336 + 150 + 30 + 30 + 16;  // doubles per ghost
+constexpr int kGhostLds = 336 + 336 + 150 + 30 + 30 + 20;  // doubles per ghost
+static_assert(sizeof(RoeX) <= 20 * sizeof(double), "RoeX slot of the ghost LDS");
 
 template <bool WALL>
 __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs, const HeavyGhosts& hg,
@@ -602,7 +603,7 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
   const long fxpl = (long)(g.nx + 1) * g.ny, fypl = (long)g.nx * (g.ny + 1);
   const long ixl = (long)i * g.ny + j, ixr = ixl + g.ny;
   const long iyb = (long)i * (g.ny + 1) + j, iyt = iyb + 1;
-  const double inv_dx = 1.0 / g.dx, inv_dy = 1.0 / g.dy;
+  const double inv_dx = g.rdx, inv_dy = g.rdy;
   double r[6];
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
@@ -625,7 +626,7 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
     if (FAST)
       noncon_fast(vel, pe, pi, pr, inv_dx, ncx);
     else
-      noncon(vel, pe, pi, pr, g.dx, ncx);
+      noncon(vel, pe, pi, pr, g.dx, g.rdx, g.r60dx, ncx);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const long p = c0 + (k - 4);
@@ -637,7 +638,7 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
     if (FAST)
       noncon_fast(vel, pe, pi, pr, inv_dy, ncy);
     else
-      noncon(vel, pe, pi, pr, g.dy, ncy);
+      noncon(vel, pe, pi, pr, g.dy, g.rdy, g.r60dy, ncy);
   }
 #pragma unroll
   for (int e = 0; e < 3; ++e) r[3 + e] = r[3 + e] + ncx[e] + ncy[e];
@@ -649,7 +650,7 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
   }
   if (ph.diffusion) {  // src/main.rs:209-246, 288-295 with src/diffusion.rs:27-50
     const double kap[3] = {ph.kappa_e, ph.kappa_i, ph.kappa_r};
-    const double cx = -1.0 / (g.dx * g.dx), cy = -1.0 / (g.dy * g.dy);
+    const double cx = g.cdx, cy = g.cdy;
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
       double tx[7], ty[7];
@@ -1079,6 +1080,7 @@ EigConst make_eig(const Phys& ph) {
     K.six_g[k] = 6.0 * g[k];
     K.gam_gt[k] = gam[k] * gt;
     K.inv6g[k] = 1.0 / (6.0 * g[k]);
+    K.rcp_six_g[k] = 1.0 / K.six_g[k];
     for (int j = 0; j < 3; ++j) {
       K.num[k][j] = j == k ? -(g[k] * g[k]) : -g[k] * g[j];
       K.coef[k][j] = j == k ? 6.0 * g[k] : 6.0 * g[k] * g[j];
@@ -1182,6 +1184,12 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
     g.plane = (long)(g.nx + 8) * g.PY;
     g.dx = pb.grid.dx;
     g.dy = pb.grid.dy;
+    g.rdx = 1.0 / g.dx;
+    g.rdy = 1.0 / g.dy;
+    g.r60dx = 1.0 / (60.0 * g.dx);
+    g.r60dy = 1.0 / (60.0 * g.dy);
+    g.cdx = -1.0 / (g.dx * g.dx);
+    g.cdy = -1.0 / (g.dy * g.dy);
     const long ncell = (long)g.nx * g.ny;
     max_cells = std::max(max_cells, ncell);
     sh.fluid = pb.fluid;

@@ -40,6 +40,8 @@ struct EigConst {
   // Fast variant only
   double inv_gt;       // 1/gt
   double inv6g[3];     // 1/(6 g_k)
+  // Exact variant: RN(1 / six_g[k]), the reciprocal div_rn uses
+  double rcp_six_g[3];
 };
 
 // The same pointer, opaque to the compiler: the constants read through it
@@ -61,11 +63,17 @@ struct Roe {
   double cs2w2, gtcs2;
   // Fast: reciprocals
   double ics2, ib, iad, igtcs2;
+  // Exact: RN(1 / (6 cs2)), RN(1 / b), RN(1 / (gt cs2)) for div_rn (with
+  // 12 cs2 = 2 (6 cs2) and 3 cs2 = (6 cs2) / 2 exactly, RN(1/(12 cs2)) and
+  // RN(1/(3 cs2)) are y6 / 2 and 2 y6)
+  double y6, yb, ygt;
 };
 
+// a / b: FAST as a * ib (ib ~ 1/b), EXACT correctly rounded through div_rn
+// with ib = RN(1/b)
 template <bool FAST>
 __device__ __forceinline__ double mdiv(double a, double b, double ib) {
-  return FAST ? a * ib : a / b;
+  return FAST ? a * ib : div_rn(a, b, ib);
 }
 
 template <int DIR, bool FAST>
@@ -84,10 +92,11 @@ __device__ __forceinline__ void roe_state(const EigConst& K, double rho3, double
     R.w2 = fma(R.u, R.u, R.v * R.v);
     R.cs2 = fma(K.gamg[2], e[2], fma(K.gamg[1], e[1], K.gamg[0] * e[0]));
   } else {
-    R.u = (u3 * r1 + u2 * r2) / rs;
-    R.v = (v3 * r1 + v2 * r2) / rs;
+    const double yrs = 1.0 / rs;  // the five divisions by rs through div_rn
+    R.u = div_rn(u3 * r1 + u2 * r2, rs, yrs);
+    R.v = div_rn(v3 * r1 + v2 * r2, rs, yrs);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) e[k] = (e3[k] * r1 + e2[k] * r2) / rs;
+    for (int k = 0; k < 3; ++k) e[k] = div_rn(e3[k] * r1 + e2[k] * r2, rs, yrs);
     R.w2 = sq(R.u) + sq(R.v);
     R.cs2 = K.gamg[0] * e[0] + K.gamg[1] * e[1] + K.gamg[2] * e[2];
   }
@@ -108,16 +117,20 @@ __device__ __forceinline__ void roe_state(const EigConst& K, double rho3, double
   }
   R.gtcs2 = K.gt * cs2;
   R.cs2w2 = cs2 * w2;
-  const double w6 = FAST ? w2 * (1.0 / 6.0) : w2 / 6.0;
+  const double w6 = FAST ? w2 * (1.0 / 6.0) : div_rn(w2, 6.0, kRcp6);
 #pragma unroll
   for (int k = 0; k < 3; ++k) R.gk_[k] = K.gam[k] * e[k] + w6;
   const double wv = DIR == 0 ? R.u : R.v;
-  R.pcs = FAST ? wv * R.cs * (1.0 / 3.0) : wv * R.cs / 3.0;
+  R.pcs = FAST ? wv * R.cs * (1.0 / 3.0) : div_rn(wv * R.cs, 3.0, kRcp3);
   if (FAST) {
     R.ics2 = 1.0 / cs2;
     R.ib = 1.0 / R.b;
     R.iad = R.ics2 * (1.0 / 12.0);
     R.igtcs2 = R.ics2 * K.inv_gt;
+  } else {
+    R.y6 = 1.0 / (6.0 * cs2);
+    R.yb = 1.0 / R.b;
+    R.ygt = 1.0 / R.gtcs2;
   }
 }
 
@@ -144,12 +157,13 @@ __device__ __forceinline__ void l_row(const EigConst& K, const Roe<DIR, FAST>& R
       a2 = c == 0 ? K.m2gt * v - six_cs : K.m2gt * v + six_cs;
     }
     (void)sgn;
-    row[0] = mdiv<FAST>(a0, ad, R.iad);
-    row[1] = mdiv<FAST>(a1, ad, R.iad);
-    row[2] = mdiv<FAST>(a2, ad, R.iad);
-    row[3] = mdiv<FAST>(K.six_g[0], ad, R.iad);
-    row[4] = mdiv<FAST>(K.six_g[1], ad, R.iad);
-    row[5] = mdiv<FAST>(K.six_g[2], ad, R.iad);
+    const double yad = FAST ? R.iad : 0.5 * R.y6;
+    row[0] = mdiv<FAST>(a0, ad, yad);
+    row[1] = mdiv<FAST>(a1, ad, yad);
+    row[2] = mdiv<FAST>(a2, ad, yad);
+    row[3] = mdiv<FAST>(K.six_g[0], ad, yad);
+    row[4] = mdiv<FAST>(K.six_g[1], ad, yad);
+    row[5] = mdiv<FAST>(K.six_g[2], ad, yad);
     return;
   }
   if (c == tr) {
@@ -169,9 +183,10 @@ __device__ __forceinline__ void l_row(const EigConst& K, const Roe<DIR, FAST>& R
     row[1] = fma(gk * u, ics3, K.two_g_gt[k] * u * t);
     row[2] = fma(gk * v, ics3, K.two_g_gt[k] * v * t);
   } else {
-    row[0] = K.g_over_gt[k] - gk * w2 / (6.0 * cs2) - K.g_gt[k] * w2 / R.b * A;
-    row[1] = gk * u / (3.0 * cs2) + K.two_g_gt[k] * u / R.b * A;
-    row[2] = gk * v / (3.0 * cs2) + K.two_g_gt[k] * v / R.b * A;
+    const double c6 = 6.0 * cs2, c3 = 3.0 * cs2, y3 = 2.0 * R.y6;
+    row[0] = K.g_over_gt[k] - div_rn(gk * w2, c6, R.y6) - div_rn(K.g_gt[k] * w2, R.b, R.yb) * A;
+    row[1] = div_rn(gk * u, c3, y3) + div_rn(K.two_g_gt[k] * u, R.b, R.yb) * A;
+    row[2] = div_rn(gk * v, c3, y3) + div_rn(K.two_g_gt[k] * v, R.b, R.yb) * A;
   }
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
@@ -186,7 +201,7 @@ __device__ __forceinline__ void l_row(const EigConst& K, const Roe<DIR, FAST>& R
     if (FAST)
       row[3 + j] = fma(K.num[k][j], R.igtcs2, -(K.coef[k][j] * R.ib) * inner);
     else
-      row[3 + j] = K.num[k][j] / R.gtcs2 - K.coef[k][j] / R.b * inner;
+      row[3 + j] = div_rn(K.num[k][j], R.gtcs2, R.ygt) - div_rn(K.coef[k][j], R.b, R.yb) * inner;
   }
 }
 
@@ -207,7 +222,7 @@ __device__ __forceinline__ double r_entry(const EigConst& K, const Roe<DIR, FAST
       *one = is_one;
       return is_one ? 1.0 : 0.0;
     }
-    return FAST ? w * (1.0 / 3.0) : w / 3.0;
+    return FAST ? w * (1.0 / 3.0) : div_rn(w, 3.0, kRcp3);
   }
   if (r == 0) {
     *one = true;
@@ -224,7 +239,7 @@ __device__ __forceinline__ double r_entry(const EigConst& K, const Roe<DIR, FAST
   if (r == 2) return v;
   const int k = DIR == 0 ? c - 1 : c - 2;  // energy column
   const int j = r - 3;
-  if (j == k) return FAST ? K.gt * R.w2 * K.inv6g[k] : K.gt * R.w2 / K.six_g[k];
+  if (j == k) return FAST ? K.gt * R.w2 * K.inv6g[k] : div_rn(K.gt * R.w2, K.six_g[k], K.rcp_six_g[k]);
   const int m = 3 - k - j;
   return (j == (k + 1) % 3) ? K.g[m] : -K.g[m];
 }

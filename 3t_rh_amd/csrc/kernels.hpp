@@ -21,6 +21,9 @@ struct Geo {
   int PY;        // pitch of a padded row
   long plane;    // (nx + 8) * PY
   double dx, dy;
+  // host-rounded reciprocals (the same IEEE quotients a kernel would form):
+  // 1/dx, 1/dy, 1/(60 dx), 1/(60 dy), -1/(dx dx), -1/(dy dy)
+  double rdx, rdy, r60dx, r60dy, cdx, cdy;
 };
 
 __host__ __device__ inline long pad_index(const Geo& g, long i, long j) {

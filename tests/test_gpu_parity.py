@@ -194,6 +194,46 @@ def test_rk3_double_mach_steps(rh3t, dmr, arith):
     s.close()
 
 
+def _developed(f, u, p, steps):
+    for _ in range(steps):
+        u, _t = ob.rk3_step(p, u, 0.05 * ob.global_dt(p, u))
+    return u
+
+
+@pytest.mark.parametrize("case", ["box64", "cylinder", "double_mach", "shock128"])
+def test_exact_is_bitwise_oracle(rh3t, workloads, box64, cyl, dmr, case):
+    """EXACT keeps the reference's rounding sequence (divisions through
+    shared or constant reciprocals are correctly rounded, hence the same
+    quotients): l(), the ghost values and one full RK3 step equal the
+    oracle bit for bit at default physics, on every boundary kind."""
+    if case == "box64":
+        f, u, p = box64
+    elif case == "cylinder":
+        f, u, p = cyl
+        u = _developed(f, u, p, 2)
+    elif case == "double_mach":
+        f, u, p = dmr
+        u = _developed(f, u, p, 3)
+    else:
+        f = workloads.translating_shock(128)
+        u = workloads.translating_shock_state(f, seed=2)
+        p = ob.OracleProblem(f)
+    _say(f"EXACT bitwise against the oracle: {case}")
+    s = f.solver(arith=0)
+    s.set_state(u, 0.0)
+    rhs, gv = s.l(with_ghosts=True)
+    ref, gref = ob.l(p, u, with_ghosts=True)
+    m = f.fluid
+    assert np.array_equal(gv, gref), np.abs(gv - gref).max()
+    assert np.array_equal(rhs[m], ref[m]), np.abs(rhs[m] - ref[m]).max()
+    dt = 0.05 * ob.global_dt(p, u)
+    s.rk3_ssp(dt)
+    got, _ = s.get_state()
+    want, _ = ob.rk3_step(p, u, dt)
+    assert np.array_equal(got[m], want[m]), np.abs(got[m] - want[m]).max()
+    s.close()
+
+
 def test_l_translating_shock(rh3t, workloads):
     _say("l() translating shock 128")
     f = workloads.translating_shock(128)
