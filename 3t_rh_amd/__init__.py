@@ -44,6 +44,7 @@ STATUS_NAMES = {
 BC_REFLECTIVE_WALL, BC_WALL, BC_PERIODIC, BC_CONSTANT = 0, 1, 2, 3
 BC_TIME_DEPENDENT, BC_OUTFLOW, BC_ZEROTH_ORDER, BC_FARFIELD = 4, 5, 6, 7
 ARITH_EXACT, ARITH_FAST = 0, 1
+TD_MOVING_FRONT = 1  # TimeDependent profile: planar front normal to x
 
 
 class RH3TError(RuntimeError):
@@ -56,8 +57,9 @@ class RH3TError(RuntimeError):
 # ----------------------------------------------------------------- C types
 
 class c_bc(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("state", ctypes.c_double * 6),
-                ("p_inf", ctypes.c_double), ("sigma", ctypes.c_double), ("l_domain", ctypes.c_double)]
+    _fields_ = [("kind", ctypes.c_int32), ("profile", ctypes.c_int32), ("state", ctypes.c_double * 6),
+                ("p_inf", ctypes.c_double), ("sigma", ctypes.c_double), ("l_domain", ctypes.c_double),
+                ("state2", ctypes.c_double * 6), ("speed", ctypes.c_double), ("front0", ctypes.c_double)]
 
 
 class c_physics(ctypes.Structure):
@@ -105,6 +107,7 @@ SIGNATURES = {
     "rh3t_problem_cylinder_state": (ctypes.c_int, [_P, ctypes.c_double, _DP]),
     "rh3t_problem_clone": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "rh3t_problem_init_double_mach": (ctypes.c_int, [ctypes.c_double, ctypes.POINTER(_P)]),
+    "rh3t_problem_init_double_mach_top": (ctypes.c_int, [ctypes.c_double, ctypes.POINTER(c_bc), ctypes.POINTER(_P)]),
     "rh3t_problem_double_mach_state": (ctypes.c_int, [_P, _DP]),
     "rh3t_problem_set_bc_state": (ctypes.c_int, [_P, ctypes.c_int32, _DP]),
     "rh3t_problem_grid": (ctypes.c_int, [_P, ctypes.POINTER(c_grid)]),
@@ -245,6 +248,19 @@ class BCType:
     farfield = staticmethod(lambda state: BCType._mk(BC_FARFIELD, state))
     outflow = staticmethod(lambda p_inf, sigma, l_domain: BCType._mk(BC_OUTFLOW, None, p_inf, sigma, l_domain))
 
+    @staticmethod
+    def moving_front(behind, ahead, speed, front0=0.0):
+        """TimeDependent (src/bc1.rs:634-636) with the built-in moving front
+        profile: `behind` where P0.x <= front0 + speed * t, `ahead`
+        elsewhere -- the moving Mach-10 shock closure of init_double_mach
+        (src/init.rs:61-65) with behind = post, ahead = pre, speed = 10."""
+        b = BCType._mk(BC_TIME_DEPENDENT, behind)
+        b.profile = TD_MOVING_FRONT
+        for k in range(6):
+            b.state2[k] = float(ahead[k])
+        b.speed, b.front0 = float(speed), float(front0)
+        return b
+
 
 class GridInfo:
     def __init__(self, nx, ny, dx, dy, x0, y0):
@@ -299,11 +315,17 @@ class Field:
         return f
 
     @classmethod
-    def init_double_mach(cls, h=1.0 / 320.0):
+    def init_double_mach(cls, h=1.0 / 320.0, top=None):
         """init::init_double_mach (src/init.rs:26-113) with mesh width h
-        (1/320 in the reference); ghosts built."""
+        (1/320 in the reference); ghosts built.  top: the BC of the top side
+        (None: the reference's ZerothOrder; e.g. double_mach_moving_shock()
+        for the closure the reference defines there)."""
         hd = ctypes.c_void_p()
-        _check(lib().rh3t_problem_init_double_mach(float(h), ctypes.byref(hd)), "init_double_mach")
+        if top is None:
+            st = lib().rh3t_problem_init_double_mach(float(h), ctypes.byref(hd))
+        else:
+            st = lib().rh3t_problem_init_double_mach_top(float(h), ctypes.byref(top), ctypes.byref(hd))
+        _check(st, "init_double_mach")
         return cls(None, None, None, _handle=hd)
 
     def double_mach_state(self):

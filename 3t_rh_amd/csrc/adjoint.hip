@@ -720,6 +720,10 @@ int init_adjoint(Engine& E) {
   std::vector<std::pair<long, int>> ents;  // (source slot, ghost list index)
   for (int k = 0; k < E.cheap.n; ++k) {
     const int kind = E.h_cheap_kind[k];
+    if (kind == RH3T_BC_TIME_DEPENDENT) {
+      std::snprintf(E.message, sizeof(E.message), "the adjoint does not cover TimeDependent ghosts");
+      return RH3T_ERR_UNSUPPORTED;
+    }
     if (kind == RH3T_BC_CONSTANT) continue;
     const long src = E.h_cheap_src[k];
     const long i = src / g.PY - 4, j = src % g.PY - 4;
@@ -808,7 +812,7 @@ void launch_vjp(Engine& E, int buf, bool taped, const double* LAM, double rscale
   const SolveDesc& d = E.h_desc[0];
   const Geo& g = d.g;
   if (!taped) {
-    launch_ghosts(E, buf, 2u, false);
+    launch_ghosts(E, buf, 2u, GT_FIELD, false);
     launch_derived(E, buf);
   }
   AdjDesc A;
@@ -1033,7 +1037,7 @@ void recompute_segment(Engine& E, const rh3t_adjoint* A, int k) {
     if (n + 1 < n1) {
       launch_stage(E, 2, kStage3, false);
     } else {
-      launch_ghosts(E, 2, 6u, false);
+      launch_ghosts(E, 2, 6u, GT_STAGE3, false);
       launch_derived(E, 2);
     }
   }

@@ -79,7 +79,7 @@ __device__ __forceinline__ void raise_error(Scalars* sc, unsigned phase, unsigne
 // ------------------------------------------------------------ ghosts
 
 __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict__ descs, CheapGhosts cg, int buf,
-                                                     unsigned phase) {
+                                                     unsigned phase, int gt) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= cg.n) return;
   const SolveDesc& sd = descs[cg.sid[k]];
@@ -91,6 +91,19 @@ __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict
   if (kind == RH3T_BC_CONSTANT) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) v[f] = cg.cstate[6L * k + f];
+  } else if (kind == RH3T_BC_TIME_DEPENDENT) {
+    // the moving front profile at the time of the state being evaluated:
+    // Field.time of an l() call; t, t + dt, t + 0.5 dt for the three l()
+    // calls of rk3_ssp (src/main.rs:356-366); the closure compares P0.x
+    // with front0 + speed * t (src/init.rs:61-65: 10.0 * t)
+    const Scalars* sc = sd.sc;
+    const double t = gt == GT_FIELD ? sc->time
+                                    : (gt == GT_STAGE1 ? sc->t_n
+                                                       : (gt == GT_STAGE2 ? sc->t_n + sc->dt : sc->t_n + 0.5 * sc->dt));
+    const double xf = cg.front[2L * k + 1] + cg.front[2L * k] * t;
+    const double* st = cg.p0x[k] <= xf ? cg.cstate + 6L * k : cg.cstate2 + 6L * k;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) v[f] = st[f];
   } else {
     long s = cg.src[k];
 #pragma unroll
@@ -802,6 +815,7 @@ __global__ __launch_bounds__(1024) void k_dt_final(const SolveDesc* __restrict__
       double dt = sc->dt_fixed > 0.0 ? sc->dt_fixed : sc->cfl_factor * dt_cfl;
       sc->dt = dt;
       if (!(dt > 0.0)) atomicMin(&sc->err, err_key(sc->step, 1, RH3T_ERR_DT, 0));
+      sc->t_n = sc->time;
       sc->time = sc->time + dt;
     }
   }
@@ -903,10 +917,10 @@ int build_tables(Engine& E) {
 
 namespace rh3t_eng {
 
-void launch_ghosts(Engine& E, int buf, unsigned phase, bool capture) {
+void launch_ghosts(Engine& E, int buf, unsigned phase, int gt, bool capture) {
   if (E.cheap.n > 0) {
     Launcher L(E, K_GHOST_CHEAP);
-    k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.cheap, buf, phase);
+    k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.cheap, buf, phase, gt);
   }
   if (E.heavy.n > 0) {
     Launcher L(E, K_GHOST_HEAVY);
@@ -930,7 +944,7 @@ void launch_derived(Engine& E, int buf) {
 }
 
 void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture) {
-  launch_ghosts(E, src, sa.mode == 0 ? 2u : (unsigned)(2 * sa.mode), capture);
+  launch_ghosts(E, src, sa.mode == 0 ? 2u : (unsigned)(2 * sa.mode), sa.mode, capture);
   launch_derived(E, src);
   dim3 blk(E.bx, E.by);
   {
@@ -1164,7 +1178,7 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   std::vector<int> c_sid, c_kind, c_gid, h_sid, h_kind, h_gid, h_tab;
   std::vector<std::pair<const void*, int>> table_base;  // shared heavy table -> first row
   std::vector<long> c_pad, c_src, h_pad, h_near, h_st;
-  std::vector<double> c_n, c_state, h_n, h_dist, h_state, h_of, h_d5, h_S, h_W;
+  std::vector<double> c_n, c_state, h_n, h_dist, h_state, h_of, h_d5, h_S, h_W, c_p0x, c_state2, c_front;
   long max_cells = 0, max_ghost = 1;
   for (int s = 0; s < n; ++s) {
     const rh3t::Problem& pb = *P[s];
@@ -1207,10 +1221,6 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
         std::snprintf(E.message, sizeof(E.message), "ghost outside the halo");
         return RH3T_ERR_GEOMETRY;
       }
-      if (gh.bc_kind == RH3T_BC_TIME_DEPENDENT) {
-        std::snprintf(E.message, sizeof(E.message), "TimeDependent boundary conditions are not part of this path");
-        return RH3T_ERR_UNSUPPORTED;
-      }
       gpad[k] = pad_index(g, gh.i, gh.j);
       pt[gpad[k]] = 2;
       const auto& bc = pb.bc_list(gh.boundary)[gh.side_id];
@@ -1223,6 +1233,11 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
         c_n.push_back(gh.project.normal.x);
         c_n.push_back(gh.project.normal.y);
         for (int f = 0; f < 6; ++f) c_state.push_back(bc.state[f]);
+        const bool td = gh.bc_kind == RH3T_BC_TIME_DEPENDENT;
+        c_p0x.push_back(gh.project.point.x);
+        for (int f = 0; f < 6; ++f) c_state2.push_back(td ? bc.state2[f] : 0.0);
+        c_front.push_back(td ? bc.speed : 0.0);
+        c_front.push_back(td ? bc.front0 : 0.0);
       } else {
         const auto& hb = (*pb.heavy)[gh.heavy_id];
         h_sid.push_back(s);
@@ -1331,6 +1346,9 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   UP(c_gid, ip) E.cheap.gid = ip;
   UP(c_n, dp) E.cheap.normal = dp;
   UP(c_state, dp) E.cheap.cstate = dp;
+  UP(c_p0x, dp) E.cheap.p0x = dp;
+  UP(c_state2, dp) E.cheap.cstate2 = dp;
+  UP(c_front, dp) E.cheap.front = dp;
   UP(h_sid, ip) E.heavy.sid = ip;
   UP(h_pad, lp) E.heavy.pad = lp;
   UP(h_kind, ip) E.heavy.kind = ip;
@@ -1583,6 +1601,8 @@ int rh3t_rk3_step(rh3t_ctx* ctx, double dt) {
   int st = reset_errors(E);
   if (st) return st;
   HIP_TRY(hipMemcpyAsync(&E.d_sc[0].dt, &dt, sizeof(double), hipMemcpyHostToDevice, E.stream));
+  // the step starts at Field.time (stage times t, t + dt, t + dt/2)
+  HIP_TRY(hipMemcpyAsync(&E.d_sc[0].t_n, &E.d_sc[0].time, sizeof(double), hipMemcpyDeviceToDevice, E.stream));
   launch_rk3(E);
   HIP_TRY(hipGetLastError());
   E.dt_partials_valid = true;

@@ -40,9 +40,18 @@ struct CheapGhosts {
   const long* src;      // source slot (reflective / zeroth / periodic)
   const int* kind;      // rh3t_bc_kind
   const double* normal; // 2 per ghost
-  const double* cstate; // 6 per ghost (Constant)
+  const double* cstate; // 6 per ghost (Constant; TimeDependent behind the front)
   const int* gid;       // sorted ghost id within its solve
+  // TimeDependent (RH3T_TD_MOVING_FRONT) per ghost: P0.x, the state ahead of
+  // the front (6), front speed and position at t = 0 (2); zero otherwise
+  const double* p0x;
+  const double* cstate2;
+  const double* front;
 };
+
+// Which time a ghost update evaluates TimeDependent ghosts at: the state's
+// own Field.time (an l() call), or stage 1..3 of the current RK3 step.
+enum GhostTime { GT_FIELD = 0, GT_STAGE1 = 1, GT_STAGE2 = 2, GT_STAGE3 = 3 };
 
 struct HeavyGhosts {
   int n;
@@ -189,8 +198,9 @@ struct Launcher {
 inline int total(const std::vector<int>& t) { return t.back(); }
 
 // ghosts of every solve on buffer buf (+ the first-failure state capture
-// unless capture is false; the failure key is raised either way)
-void launch_ghosts(Engine& E, int buf, unsigned phase, bool capture = true);
+// unless capture is false; the failure key is raised either way), with
+// TimeDependent ghosts at time gt
+void launch_ghosts(Engine& E, int buf, unsigned phase, int gt, bool capture = true);
 // Derived::from_state on every fluid cell and ghost of buffer buf
 void launch_derived(Engine& E, int buf);
 // ghosts -> derived -> x/y fluxes -> assemble (one l() + stage) on buffer src

@@ -53,6 +53,8 @@ struct Scalars {     // device-resident time-loop scalars
   double dt;         // dt of the current step
   double dt_cfl;     // calc_global_dt of the current step
   double time;       // Field.time
+  double t_n;        // Field.time at the start of the current RK3 step (the
+                     // stage times are t_n, t_n + dt, t_n + dt/2)
   double cfl_factor; // dt = cfl_factor * dt_cfl  (0.05 in main.rs:461)
   double dt_fixed;   // if > 0, used instead of the CFL rule
   unsigned step;     // step counter (error ordering)

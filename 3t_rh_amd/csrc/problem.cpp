@@ -799,7 +799,13 @@ int guarded(F&& f) {
   }
 }
 
-bool valid_bc(const rh3t_bc& b) { return b.kind >= 0 && b.kind <= 7; }
+// RH3T_OK, RH3T_ERR_ARG (unknown kind) or RH3T_ERR_UNSUPPORTED (a
+// TimeDependent BC without a built-in profile of this build)
+int check_bc(const rh3t_bc& b) {
+  if (b.kind < 0 || b.kind > 7) return RH3T_ERR_ARG;
+  if (b.kind == RH3T_BC_TIME_DEPENDENT && b.profile != RH3T_TD_MOVING_FRONT) return RH3T_ERR_UNSUPPORTED;
+  return RH3T_OK;
+}
 
 }  // namespace
 
@@ -830,9 +836,9 @@ int rh3t_problem_create(const rh3t_grid* grid, const double* outer_xy, int32_t n
   if (grid->nx <= 0 || grid->ny <= 0 || !(grid->dx > 0.0) || !(grid->dy > 0.0)) return RH3T_ERR_ARG;
   if (n_inner > 0 && (!inner_xy || !bc_inner)) return RH3T_ERR_ARG;
   for (int k = 0; k < n_outer; ++k)
-    if (!valid_bc(bc_outer[k])) return RH3T_ERR_ARG;
+    if (int st = check_bc(bc_outer[k])) return st;
   for (int k = 0; k < n_inner; ++k)
-    if (!valid_bc(bc_inner[k])) return RH3T_ERR_ARG;
+    if (int st = check_bc(bc_inner[k])) return st;
   *out = nullptr;
   return guarded([&]() {
     auto* pb = new rh3t_problem();
@@ -984,10 +990,14 @@ void double_mach_states(double pre[6], double post[6]) {
 }  // namespace
 
 int rh3t_problem_init_double_mach(double h, rh3t_problem** out) {
+  return rh3t_problem_init_double_mach_top(h, nullptr, out);
+}
+
+int rh3t_problem_init_double_mach_top(double h, const rh3t_bc* top, rh3t_problem** out) {
   // src/init.rs:26-113 (h = 1/320 in the reference): the polygon
   // A(-1/2 - sqrt3/12, 0) B(0, 0) C(23 sqrt3/12, 23/12) D(C.x, 23/12 +
   // sqrt3/2) E(A.x, D.y), Constant(post) on AB and EA, the 30-degree Wall on
-  // BC, Constant(pre) on CD, ZerothOrder on DE.
+  // BC, Constant(pre) on CD, ZerothOrder (or the caller's top BC) on DE.
   if (!out || !(h > 0.0) || h > 0.25) return RH3T_ERR_ARG;
   *out = nullptr;
   return guarded([&]() {
@@ -1018,7 +1028,7 @@ int rh3t_problem_init_double_mach(double h, rh3t_problem** out) {
     std::memset(&zeroth, 0, sizeof(zeroth));
     zeroth.kind = RH3T_BC_ZEROTH_ORDER;
     const double pts[10] = {ax, ay, bx, by, cx, cy, dx, dy, ex, ey};
-    const rh3t_bc bcs[5] = {c_post, wall, c_pre, zeroth, c_post};
+    const rh3t_bc bcs[5] = {c_post, wall, c_pre, top ? *top : zeroth, c_post};
     const double inner[8] = {-1002.0, -1002.0, -1001.0, -1002.0, -1001.0, -1001.0, -1002.0, -1001.0};
     const rh3t_bc bc_inner[4] = {wall, wall, wall, wall};
     rh3t_problem* pb = nullptr;

@@ -139,6 +139,25 @@ def cylinder(nx=121, mach=3.0):
     return Field.init_cylinder(nx, mach)
 
 
+def euler_to_three_energy(rho, ux, uy, p, gamma=1.4):
+    """src/init.rs:7-24: conservative 3-T state of an Euler state, the total
+    energy split equally into thirds."""
+    kinetic = 0.5 * rho * (ux * ux + uy * uy)
+    e_total = p / (gamma - 1.0) + kinetic
+    return [rho, rho * ux, rho * uy, e_total / 3.0, e_total / 3.0, e_total / 3.0]
+
+
+def double_mach_moving_shock():
+    """The TimeDependent BC that init_double_mach defines for its top side
+    (src/init.rs:57-65): post-shock where P0.x <= 10 t, pre-shock elsewhere
+    (Mach-10 states of src/init.rs:45-49), as the built-in moving front
+    profile."""
+    from . import BCType
+    pre = euler_to_three_energy(1.4, 0.0, 0.0, 1.0)
+    post = euler_to_three_energy(8.0, 8.25, 0.0, 116.5)
+    return BCType.moving_front(post, pre, 10.0, 0.0)
+
+
 CYLINDER_CLASSES = (121, 161, 201, 241)
 
 

@@ -38,7 +38,9 @@ typedef enum {
   RH3T_ERR_GHOST = 4,        /* non-finite ghost state, src/ghost.rs:229-263  */
   RH3T_ERR_GEOMETRY = 5,     /* ghost build failed (uncached stencil target,
                                 no boundary side, substencil cannot fit)      */
-  RH3T_ERR_UNSUPPORTED = 6,  /* feature outside this build (TimeDependent BC) */
+  RH3T_ERR_UNSUPPORTED = 6,  /* feature outside this build (an unknown
+                                TimeDependent profile; reverse mode of the
+                                heavy and TimeDependent ghost kinds)          */
   RH3T_ERR_NO_DEVICE = 7,    /* no HIP device visible                         */
   RH3T_ERR_SINGULAR = 8,     /* singular 6x6 Wall-ILW system, src/bc1.rs:468  */
   RH3T_ERR_DT = 9,           /* non-positive dt, src/main.rs:464              */
@@ -57,13 +59,30 @@ typedef enum {
   RH3T_BC_FARFIELD = 7
 } rh3t_bc_kind;
 
+/* TimeDependent(f), src/bc1.rs:634-636: the reference's BC is a host
+ * closure f(P0, n, t) -> State of the boundary point P0, its normal and the
+ * stage time.  Here the closure is one of a registry of built-in profiles,
+ * evaluated on the device at the stage time (t, t + dt, t + dt/2 in
+ * rk3_ssp's three l() calls, src/main.rs:351-373):
+ *   RH3T_TD_MOVING_FRONT  a planar front normal to x at front0 + speed * t:
+ *                         state where P0.x <= front0 + speed * t, state2
+ *                         elsewhere -- with state = post-shock, state2 =
+ *                         pre-shock, speed = 10, front0 = 0 the moving
+ *                         Mach-10 shock closure of init_double_mach
+ *                         (src/init.rs:61-65). */
+#define RH3T_TD_MOVING_FRONT 1
+
 typedef struct {
   int32_t kind;        /* rh3t_bc_kind                                        */
-  int32_t reserved;
-  double state[6];     /* Constant(State) / FarField(State)                   */
+  int32_t profile;     /* TimeDependent: RH3T_TD_* profile; 0 otherwise       */
+  double state[6];     /* Constant(State) / FarField(State) / TimeDependent
+                          state behind the front                             */
   double p_inf;        /* Outflow { p_inf, sigma, l_domain }                  */
   double sigma;
   double l_domain;
+  double state2[6];    /* TimeDependent: state ahead of the front             */
+  double speed;        /* TimeDependent: front speed along x                  */
+  double front0;       /* TimeDependent: front position at t = 0              */
 } rh3t_bc;
 
 /* Physics constants, src/constant.rs:1-25, as a runtime struct.  The
@@ -169,6 +188,16 @@ int rh3t_problem_init_cylinder(int32_t nx, double mach, rh3t_problem** out);
  * reflection polygon with its Constant / Wall / ZerothOrder sides.  The
  * returned problem has its ghosts built. */
 int rh3t_problem_init_double_mach(double h, rh3t_problem** out);
+
+/* init_double_mach with the BC of its top side D -> E given by the caller
+ * (top == NULL: the reference's ZerothOrder).  The reference defines the
+ * moving Mach-10 shock closure for that side (src/init.rs:61-65) but
+ * assigns ZerothOrder (src/init.rs:67-73); passing TimeDependent with
+ * RH3T_TD_MOVING_FRONT (state = post, state2 = pre, speed = 10,
+ * front0 = 0) runs the closure.  The returned problem has its ghosts
+ * built. */
+int rh3t_problem_init_double_mach_top(double h, const rh3t_bc* top,
+                                      rh3t_problem** out);
 
 /* Initial state of init_double_mach (src/init.rs:97-104: post-shock where
  * x <= 0, pre-shock elsewhere, on fluid cells; zero elsewhere), AoS

@@ -91,11 +91,15 @@ struct plain_problem {
   const uint32_t* h_stencil;
   const double* h_s;
   const double* h_w;
+  double time;
+  const double* g_p0x;
+  const double* g_bcstate2;
+  const double* g_front;
 };
 
 struct DualProblem {
   ro_problem p;
-  std::vector<Dual> normal, dist, bcstate, outflow, hd, hs, hw;
+  std::vector<Dual> normal, dist, bcstate, outflow, hd, hs, hw, p0x, bcstate2, front;
 };
 
 std::vector<Dual> lift(const double* x, size_t n) {
@@ -113,6 +117,9 @@ void make_problem(const plain_problem* q, DualProblem& D) {
   D.dist = lift(q->g_dist, ng);
   D.bcstate = lift(q->g_bcstate, 6 * ng);
   D.outflow = lift(q->g_outflow, 3 * ng);
+  D.p0x = lift(q->g_p0x, ng);
+  D.bcstate2 = lift(q->g_bcstate2, 6 * ng);
+  D.front = lift(q->g_front, 2 * ng);
   const size_t nh = (size_t)(n_heavy > 0 ? n_heavy : 1);
   D.hd = lift(n_heavy ? q->h_d : nullptr, 5 * nh);
   D.hs = lift(n_heavy ? q->h_s : nullptr, 979 * nh);
@@ -137,6 +144,10 @@ void make_problem(const plain_problem* q, DualProblem& D) {
   p.h_stencil = q->h_stencil;
   p.h_s = D.hs.data();
   p.h_w = D.hw.data();
+  p.time = Dual(q->time);
+  p.g_p0x = D.p0x.data();
+  p.g_bcstate2 = D.bcstate2.data();
+  p.g_front = D.front.data();
 }
 
 ro_physics make_physics(const plain_physics* ph, const plain_physics* dph) {

@@ -631,7 +631,14 @@ static int ghost_value(const ro_problem* pb, const ro_physics* ph, const double*
     memcpy(out, u + 6 * (size_t)pb->g_src[g], sizeof(double) * 6);
     return 0;
   }
-  if (kind == 4) return 6; /* TimeDependent: not in this path */
+  if (kind == 4) { /* TimeDependent, src/bc1.rs:634-636: the moving front
+                      closure of src/init.rs:61-65 at the state's time */
+    const double* fr = pb->g_front + 2 * g;
+    double x_front = fr[1] + fr[0] * pb->time;
+    const double* st = pb->g_p0x[g] <= x_front ? pb->g_bcstate + 6 * g : pb->g_bcstate2 + 6 * g;
+    memcpy(out, st, sizeof(double) * 6);
+    return 0;
+  }
   if (kind != 1 && kind != 5 && kind != 7) {
     memset(out, 0, sizeof(double) * 6);
     return 0;
@@ -891,11 +898,18 @@ int ro_rk3_step(const ro_problem* pb, const ro_physics* ph, double* u, double* t
   double* u1 = (double*)calloc(n, sizeof(double));
   double* u2 = (double*)calloc(n, sizeof(double));
   double* r = (double*)malloc(n * sizeof(double));
-  int st = ro_l(pb, ph, u, r, NULL, nt, err);
+  /* the three l() calls see the stage states' Field.time: u.time,
+   * u1.time = u.time + dt, u2.time = u.time + 0.5 * dt (src/main.rs:356-366) */
+  const double t0 = time ? *time : pb->time;
+  ro_problem ps = *pb;
+  ps.time = t0;
+  int st = ro_l(&ps, ph, u, r, NULL, nt, err);
   if (!st) st = stage(pb, u, NULL, u1, r, dt, 0, 0, 1, nt, err);
-  if (!st) st = ro_l(pb, ph, u1, r, NULL, nt, err);
+  ps.time = t0 + dt;
+  if (!st) st = ro_l(&ps, ph, u1, r, NULL, nt, err);
   if (!st) st = stage(pb, u, u1, u2, r, dt / 4.0, 0.75, 0.25, 2, nt, err);
-  if (!st) st = ro_l(pb, ph, u2, r, NULL, nt, err);
+  ps.time = t0 + 0.5 * dt;
+  if (!st) st = ro_l(&ps, ph, u2, r, NULL, nt, err);
   if (!st) st = stage(pb, u, u2, u1, r, 2.0 * dt / 3.0, 1.0 / 3.0, 2.0 / 3.0, 3, nt, err);
   if (!st) {
     const size_t ncell = (size_t)pb->nx * pb->ny;

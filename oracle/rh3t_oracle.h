@@ -54,6 +54,11 @@ typedef struct {
   const uint32_t* h_stencil;  /* 55 per heavy                                 */
   const double* h_s;          /* 979 per heavy                                */
   const double* h_w;          /* 225 per heavy                                */
+  /* TimeDependent (moving front profile) and the state's Field.time */
+  double time;                /* time of the state ro_l evaluates             */
+  const double* g_p0x;        /* per ghost: x of the projection point P0       */
+  const double* g_bcstate2;   /* 6 per ghost: TimeDependent state ahead        */
+  const double* g_front;      /* 2 per ghost: TimeDependent speed, front0      */
 } ro_problem;
 
 typedef struct {

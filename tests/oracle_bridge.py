@@ -39,7 +39,7 @@ class ro_problem(ctypes.Structure):
                 ("fluid", _U8P), ("access", _U32P), ("n_ghost", ctypes.c_int32), ("g_kind", _I32P),
                 ("g_src", _I32P), ("g_heavy", _I32P), ("g_normal", _DP), ("g_dist", _DP), ("g_bcstate", _DP),
                 ("g_outflow", _DP), ("h_nearest", _I32P), ("h_d", _DP), ("h_stencil", _U32P), ("h_s", _DP),
-                ("h_w", _DP)]
+                ("h_w", _DP), ("time", ctypes.c_double), ("g_p0x", _DP), ("g_bcstate2", _DP), ("g_front", _DP)]
 
 
 class ro_error(ctypes.Structure):
@@ -158,6 +158,9 @@ class OracleProblem:
         self.g_bcstate = np.ascontiguousarray(np.array([list(b.state) for b in bcs] or [[0.0] * 6]).reshape(-1))
         self.g_outflow = np.ascontiguousarray(np.array([[b.p_inf, b.sigma, b.l_domain] for b in bcs]
                                                        or [[0.0] * 3]).reshape(-1))
+        self.g_p0x = np.ascontiguousarray(gh["p0x"].astype(np.float64) if n else np.zeros(1))
+        self.g_bcstate2 = np.ascontiguousarray(np.array([list(b.state2) for b in bcs] or [[0.0] * 6]).reshape(-1))
+        self.g_front = np.ascontiguousarray(np.array([[b.speed, b.front0] for b in bcs] or [[0.0] * 2]).reshape(-1))
         nh = field.counts()[2]
         near, d5, st, S, W = [], [], [], [], []
         for h in range(nh):
@@ -177,7 +180,8 @@ class OracleProblem:
                             self.g_src.ctypes.data_as(_I32P), self.g_heavy.ctypes.data_as(_I32P),
                             _dp(self.g_normal), _dp(self.g_dist), _dp(self.g_bcstate), _dp(self.g_outflow),
                             self.h_nearest.ctypes.data_as(_I32P), _dp(self.h_d),
-                            self.h_stencil.ctypes.data_as(_U32P), _dp(self.h_s), _dp(self.h_w))
+                            self.h_stencil.ctypes.data_as(_U32P), _dp(self.h_s), _dp(self.h_w), 0.0,
+                            _dp(self.g_p0x), _dp(self.g_bcstate2), _dp(self.g_front))
 
 
 class OracleError(RuntimeError):
@@ -188,8 +192,11 @@ class OracleError(RuntimeError):
         super().__init__(f"oracle status {status}: {self.info}")
 
 
-def l(prob, u, phys=None, nthreads=8, with_ghosts=False):
+def l(prob, u, phys=None, nthreads=8, with_ghosts=False, time=0.0):
+    """l() of state u at Field.time `time` (the time TimeDependent ghosts
+    are evaluated at)."""
     phys = phys or physics()
+    prob.c.time = float(time)
     u = np.ascontiguousarray(u, dtype=np.float64)
     rhs = np.zeros_like(u)
     gv = np.zeros((max(prob.n_ghost, 1), 6))
@@ -290,9 +297,10 @@ def heavy_terms(prob, u, g, phys=None):
     return {"L": L.reshape(6, 6), "R": R.reshape(6, 6), "lam": lam, "v": v.reshape(5, 6), "uk": uk.reshape(5, 6)}
 
 
-def update_ghosts(prob, u, phys=None, nthreads=8):
+def update_ghosts(prob, u, phys=None, nthreads=8, time=0.0):
     """GhostGrid::update_values_parallel: ghost values (n_ghost, 6)."""
     phys = phys or physics()
+    prob.c.time = float(time)
     u = np.ascontiguousarray(u, dtype=np.float64)
     gv = np.zeros((max(prob.n_ghost, 1), 6))
     err = ro_error()

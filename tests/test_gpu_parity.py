@@ -406,6 +406,79 @@ def test_rk3_periodic_box_2048_oracle(rh3t, box2048, arith):
     s.close()
 
 
+# ------------------------------------------------ TimeDependent
+
+def test_time_dependent_moving_shock(rh3t, workloads):
+    """init_double_mach at h = 1/40 with the top side's TimeDependent
+    closure (the moving Mach-10 shock of src/init.rs:61-65, as the built-in
+    moving front profile): EXACT l() at several Field.times, one rk3_ssp
+    step over which the front passes two columns of top ghosts, and 20
+    device steps of the time loop (rh3t_advance, graph replay, dt =
+    0.05 dt_cfl on the device), each bitwise against the oracle."""
+    f = rh3t.Field.init_double_mach(1.0 / 40.0, top=workloads.double_mach_moving_shock())
+    p = ob.OracleProblem(f)
+    gh = f.ghosts()
+    td = gh["bc_kind"] == rh3t.BC_TIME_DEPENDENT
+    assert td.sum() > 50
+    u = f.double_mach_state()
+    m = f.fluid
+    s = f.solver(arith=0)
+    for t in (0.0, 0.05, 0.11):
+        _say(f"TimeDependent top: l() at t = {t}")
+        s.set_state(u, t)
+        rhs, gv = s.l(with_ghosts=True)
+        ref, gref = ob.l(p, u, with_ghosts=True, time=t)
+        assert np.array_equal(gv, gref), (t, np.abs(gv - gref).max())
+        assert np.array_equal(rhs[m], ref[m]), (t, np.abs(rhs[m] - ref[m]).max())
+    xu = np.unique(gh["p0x"][td])
+    xa, xb = xu[xu > 0.5][:2]
+    t0 = (xa - 0.04 * (xb - xa)) / 10.0
+    dt = (xb - 10.0 * t0) / 10.0 * 1.01
+    _say(f"TimeDependent top: rk3 step t0 = {t0:.6f}, dt = {dt:.3e}")
+    s.set_state(u, t0)
+    s.rk3_ssp(dt)
+    got, t1 = s.get_state()
+    want, tw = ob.rk3_step(p, u, dt, time=t0)
+    assert t1 == tw
+    assert np.array_equal(got[m], want[m]), np.abs(got[m] - want[m]).max()
+    _say("TimeDependent top: 20 device steps against the oracle's loop")
+    s.set_state(u, 0.0)
+    t = s.advance(20, 0.05)
+    got, _ = s.get_state()
+    ref, tref = u.copy(), 0.0
+    for _ in range(20):
+        ref, tref = ob.rk3_step(p, ref, 0.05 * ob.global_dt(p, ref), time=tref)
+    assert t == tref
+    assert np.array_equal(got[m], ref[m]), np.abs(got[m] - ref[m]).max()
+    # the moving front changed the solution against the ZerothOrder top
+    fz = rh3t.Field.init_double_mach(1.0 / 40.0)
+    sz = fz.solver(arith=0)
+    sz.set_state(u, 0.0)
+    sz.advance(20, 0.05)
+    uz, _ = sz.get_state()
+    assert not np.array_equal(uz[m], got[m])
+    sz.close()
+    s.close()
+
+
+def test_time_dependent_profile_refusals(rh3t, workloads):
+    """An unknown TimeDependent profile is refused at problem creation
+    (ERR_UNSUPPORTED), and the reverse mode refuses TimeDependent ghosts."""
+    _say("TimeDependent refusals")
+    bad = workloads.double_mach_moving_shock()
+    bad.profile = 7
+    with pytest.raises(rh3t.RH3TError) as ei:
+        rh3t.Field.init_double_mach(1.0 / 40.0, top=bad)
+    assert ei.value.status == rh3t.ERR_UNSUPPORTED
+    f = rh3t.Field.init_double_mach(1.0 / 40.0, top=workloads.double_mach_moving_shock())
+    s = f.solver()
+    s.set_state(f.double_mach_state())
+    with pytest.raises(rh3t.RH3TError) as ei:
+        s.l_vjp(np.zeros((f.grid.nx, f.grid.ny, 6)))
+    assert ei.value.status == rh3t.ERR_UNSUPPORTED
+    s.close()
+
+
 # ------------------------------------------------ failure paths
 
 def test_nonfinite_ghost_error(rh3t, cyl):

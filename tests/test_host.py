@@ -404,3 +404,21 @@ def test_smoothness_forms_closed_form(rh3t, ghost):
         if r >= 2:
             n2 = loc[:, 0] ** 2
             assert abs(n2 @ Sr @ n2 - 13.0 * h ** 4 / 3.0) <= 1e-8 * h ** 4
+
+
+def test_time_dependent_profile_refused_on_the_host(rh3t, workloads):
+    """A TimeDependent BC without a built-in profile of this build is
+    refused when the problem is created (no device needed); the moving front
+    profile builds, and its top-side ghosts are TimeDependent."""
+    bad = workloads.double_mach_moving_shock()
+    bad.profile = 7
+    with pytest.raises(rh3t.RH3TError) as ei:
+        rh3t.Field.init_double_mach(1.0 / 40.0, top=bad)
+    assert ei.value.status == rh3t.ERR_UNSUPPORTED
+    f = rh3t.Field.init_double_mach(1.0 / 40.0, top=workloads.double_mach_moving_shock())
+    gh = f.ghosts()
+    td = gh["bc_kind"] == rh3t.BC_TIME_DEPENDENT
+    assert td.sum() > 50 and np.all(gh["heavy_id"][td] == -1)
+    # the same ghost set as the reference's ZerothOrder top
+    g0 = rh3t.Field.init_double_mach(1.0 / 40.0).ghosts()
+    assert np.array_equal(g0[["i", "j"]], gh[["i", "j"]])

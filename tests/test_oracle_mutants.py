@@ -43,6 +43,8 @@ MUTANTS = {
                                  "4.0 * A * OEI * OER * powi(CVE, 4) * CVE *", pins.check_local_dt_exchange),
     "s_table_offset": ("kSOff[6] = {0, 1, 17, 98, 354, 979};", "kSOff[6] = {0, 1, 17, 98, 353, 979};",
                        pins.check_precomputed_matches_per_call),
+    "td_stage3_time": ("ps.time = t0 + 0.5 * dt;", "ps.time = t0 + dt;", pins.check_time_dependent_front),
+    "td_front_side": ("pb->g_p0x[g] <= x_front ?", "pb->g_p0x[g] >= x_front ?", pins.check_time_dependent_front),
     "w_table_offset": ("kWOff[6] = {0, 1, 9, 36, 100, 225};", "kWOff[6] = {0, 1, 9, 36, 99, 225};",
                        pins.check_precomputed_matches_per_call),
 }

@@ -480,3 +480,59 @@ def check_precomputed_matches_per_call(rh3t, workloads, oracle, stride=11):
 
 def test_precomputed_matches_per_call(rh3t, workloads, oracle):
     check_precomputed_matches_per_call(rh3t, workloads, oracle)
+
+
+# ------------------------------------------------------- TimeDependent
+
+def _double_mach_td(rh3t, workloads, h=1.0 / 40.0):
+    return rh3t.Field.init_double_mach(h, top=workloads.double_mach_moving_shock())
+
+
+def check_time_dependent_front(rh3t, workloads, oracle):
+    """The moving Mach-10 shock closure of init_double_mach (src/init.rs:
+    61-65) on its top side: a ghost takes the post-shock state where its
+    boundary point has P0.x <= 10 t, the pre-shock state elsewhere, at the
+    time of the state being evaluated; rk3_ssp evaluates its three l() calls
+    at t, t + dt and t + dt/2 (src/main.rs:356-366).  The stage times are
+    pinned by composing the step from three l() calls at those times with
+    the (separately pinned) SSP-RK3 weights."""
+    f = _double_mach_td(rh3t, workloads)
+    p = oracle.OracleProblem(f)
+    gh = f.ghosts()
+    td = np.nonzero(gh["bc_kind"] == rh3t.BC_TIME_DEPENDENT)[0]
+    assert len(td) > 50
+    bc = workloads.double_mach_moving_shock()
+    post, pre = np.array(bc.state), np.array(bc.state2)
+    u = f.double_mach_state()
+    x0 = gh["p0x"][td]
+    for t in (0.0, 0.05, 0.11):
+        gv = oracle.update_ghosts(p, u, time=t)
+        behind = x0 <= 10.0 * t
+        assert 0 < behind.sum() < len(td) or t == 0.0
+        assert np.array_equal(gv[td][behind], np.tile(post, (behind.sum(), 1)))
+        assert np.array_equal(gv[td][~behind], np.tile(pre, ((~behind).sum(), 1)))
+    # a step over which the front passes top ghosts: stage times t, t+dt, t+dt/2
+    xu = np.unique(x0)
+    xa, xb = xu[xu > 0.5][:2]  # two neighbouring ghost columns
+    t0 = (xa - 0.04 * (xb - xa)) / 10.0  # the front just before xa
+    dt = (xb - 10.0 * t0) / 10.0 * 1.01  # ... passes xa by t0 + dt/2 and xb by t0 + dt
+    assert np.sum((x0 > 10.0 * t0) & (x0 <= 10.0 * (t0 + 0.5 * dt))) >= 1
+    assert np.sum((x0 > 10.0 * (t0 + 0.5 * dt)) & (x0 <= 10.0 * (t0 + dt))) >= 1
+    got, t1 = oracle.rk3_step(p, u, dt, time=t0)
+    assert t1 == t0 + dt
+    m = f.fluid
+    r1 = oracle.l(p, u, time=t0)
+    u1 = u + r1 * dt
+    r2 = oracle.l(p, u1, time=t0 + dt)
+    u2 = u * 0.75 + u1 * 0.25 + r2 * (dt / 4.0)
+    r3 = oracle.l(p, u2, time=t0 + 0.5 * dt)
+    want = u * (1.0 / 3.0) + u2 * (2.0 / 3.0) + r3 * (2.0 * dt / 3.0)
+    assert np.array_equal(got[m], want[m])
+    # the stage times matter for this step: evaluating every stage at t0
+    # gives another state
+    r2b = oracle.l(p, u1, time=t0)
+    assert not np.array_equal(r2b[m], r2[m])
+
+
+def test_time_dependent_front(rh3t, workloads, oracle):
+    check_time_dependent_front(rh3t, workloads, oracle)
