@@ -482,12 +482,288 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
     raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
 }
 
-// FarField / Outflow: five wavefronts per SIMD (96 VGPRs; a few values go
-// to scratch) to hide the latency of the table reads; measured faster than
-// four (110 VGPRs, no scratch) and than six
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_ghost_open(
+// FarField / Outflow ghosts, kOpenPerWave of them per 64-lane wavefront:
+// lane group g = lane / LG (LG = 64 / kOpenPerWave lanes) takes one ghost.
+// What is uniform over a ghost -- the nearest state's Roe average, the rows
+// of L and R, the Outflow relaxation, the Taylor sum -- is evaluated once
+// per lane group instead of once per 64-lane wavefront, so the wavefront's
+// instructions serve kOpenPerWave ghosts at a time.  The 70 table rows of a
+// ghost (55 rows of S_r, the 15 rows of w_r) go to the group's lanes
+// round-robin (row t to lane t mod LG, in order), which also spreads the
+// long rows of r = 3, 4 over the lanes.  Every sum keeps the order of
+// weno_extrapolation_pre and the BC branches (src/bc1.rs:193-255,
+// 760-928), so the values are the same as one ghost per wavefront.
+constexpr int kOpenPerWave = 2;
+constexpr int kOpenLG = 64 / kOpenPerWave;
+constexpr int kOpenLds = 336 + 150 + 30 + 30 + 20;  // doubles per ghost
+static_assert(kOpenLG >= 28, "the S rows of a lane: at most two");
+// the (k, r) with k > r, whose v_k,r are zero
+__constant__ int kZeroK[10] = {1, 2, 3, 4, 2, 3, 4, 3, 4, 4};
+__constant__ int kZeroR[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+
+// Table row t of a ghost: t < 55 row a = t of S_r; 55..63 the rows of w_4
+// (k = 0..4) and w_3 (k = 0..3); 64..69 the w_r rows of r <= 2.  The block
+// offsets are the closed forms of the tables' cumulative sizes: E_r points
+// sum (i+1)^2, S_r entries sum (i+1)^4, w_r entries sum (i+1)^3 over i < r.
+struct OpenRow {
+  int r, k, a;        // a >= 0: S row a (k unused); a < 0: w row (r, k)
+  int m, dat, off;    // row length (r+1)^2, first E_r point, offset in S or W
+};
+
+__device__ __forceinline__ OpenRow open_row(int t) {
+  OpenRow o;
+  if (t < 55) {
+    o.a = t;
+    o.r = (t >= 1) + (t >= 5) + (t >= 14) + (t >= 30);
+    o.k = 0;
+  } else if (t < 64) {
+    const int q = t - 55;
+    o.a = -1;
+    o.r = q < 5 ? 4 : 3;
+    o.k = q < 5 ? q : q - 5;
+  } else {
+    const int q = t - 64;  // (2,0) (2,1) (2,2) (1,0) (1,1) (0,0)
+    o.a = -1;
+    o.r = q < 3 ? 2 : (q < 5 ? 1 : 0);
+    o.k = q < 3 ? q : (q < 5 ? q - 3 : 0);
+  }
+  const int r = o.r;
+  o.m = (r + 1) * (r + 1);
+  o.dat = r * (r + 1) * (2 * r + 1) / 6;
+  if (o.a >= 0)
+    o.off = r * (r + 1) * (2 * r + 1) * (3 * r * r + 3 * r - 1) / 30 + (o.a - o.dat) * o.m;
+  else
+    o.off = r * r * (r + 1) * (r + 1) / 4 + o.k * o.m;
+  return o;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ghost_open(
     const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
-  ghost_heavy<false>(descs, hg, ph, buf, phase, h0, count);
+  constexpr int LG = kOpenLG;
+  const int lane = threadIdx.x;
+  const int grp = lane / LG, l = lane % LG;
+  const int w = threadIdx.y;
+  const int hl = (xcd_tile(blockIdx.x, gridDim.x) * blockDim.y + w) * kOpenPerWave + grp;
+  // per-ghost LDS (kOpenLds doubles): data[6][56] (L . u at the E_r points,
+  // then, once the table rows have been applied, the products data * (S_r
+  // data) in the same slots; slot 55 keeps L . u_inf), vk[5][6][5],
+  // alpha[6][5], v[5][6], the Roe scalars
+  extern __shared__ double s_dyn[];
+  double* const s_data = s_dyn + ((long)w * kOpenPerWave + grp) * kOpenLds;
+  double* const s_part = s_data;
+  double* const s_vk = s_data + 336;
+  double* const s_alpha = s_vk + 150;
+  double* const s_v = s_alpha + 30;
+  RoeX* const s_q = (RoeX*)(s_v + 30);
+  const bool active = hl < count;
+  const int h = h0 + (active ? hl : 0);
+  const long row = hg.tab[h];
+  const SolveDesc& sd = descs[hg.sid[h]];
+  double* S = sd.U[buf];
+  const long plane = sd.g.plane;
+  const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
+  const bool far = hg.kind[h] == RH3T_BC_FARFIELD;
+  // build_local_plain, src/bc1.rs:294-312: the nearest state in the (n, t)
+  // frame, X-direction eigenvectors of its Roe average with itself
+  RoeX Q;
+  {
+    double val[6];
+    const long q = hg.nearest[h];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) val[f] = S[f * plane + q];
+    double s1[6] = {val[0], val[1] * nx + val[2] * ny, -val[1] * ny + val[2] * nx, val[3], val[4], val[5]};
+    double d1[D_N];
+    derived_from_state(ph, s1, d1, false);
+    roe_x(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI],
+          d1[D_ER], Q);
+  }
+  // L . u at the E_r points (a < 55) and, for a FarField ghost, L . u_inf
+  // (src/bc1.rs:829-924) in the free slot 55: point a = l + p LG, p = 0, 1
+  {
+    constexpr int NP = (56 + LG - 1) / LG;
+    double u6[NP][6];
+    bool has[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int a = l + p * LG;
+      has[p] = active && (a < 55 || (a == 55 && far));
+      double s6[6] = {0, 0, 0, 0, 0, 0};
+      if (has[p]) {
+        if (a < 55) {
+          const long q = hg.stencil[55L * row + a];
+#pragma unroll
+          for (int f = 0; f < 6; ++f) s6[f] = S[f * plane + q];
+        } else {
+#pragma unroll
+          for (int f = 0; f < 6; ++f) s6[f] = hg.bcstate[6L * h + f];
+        }
+      }
+      u6[p][0] = s6[0];
+      u6[p][1] = s6[1] * nx + s6[2] * ny;
+      u6[p][2] = -s6[1] * ny + s6[2] * nx;
+      u6[p][3] = s6[3];
+      u6[p][4] = s6[4];
+      u6[p][5] = s6[5];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double lr[6];
+      lx_row(ph, Q, c, lr);
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        if (has[p]) s_data[c * 56 + l + p * LG] = dot6(lr, u6[p]);
+    }
+  }
+  if (active && l == 0) *s_q = Q;
+  __syncthreads();
+  // the products of the lane's S rows (rows l and l + LG; at most two are S
+  // rows) wait in registers until every lane has read the data
+  double part[2][6];
+  int pa[2] = {-1, -1};
+  if (active) {
+    // the group's table rows t = l, l + LG, ... (sums over b in increasing
+    // order, as weno_extrapolation_pre)
+    const double* Sh = hg.S + 979L * row;
+    const double* Wh = hg.W + 225L * row;
+    for (int t = l; t < 70; t += LG) {
+      const OpenRow o = open_row(t);
+      const int r = o.r, k = o.k, a = o.a, m = o.m;
+      const double* trow = (a >= 0 ? Sh : Wh) + o.off;
+      const double* dat = s_data + o.dat;
+      double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      // the whole row's loads in flight at once (one round trip per row),
+      // then the sums in order
+      double tv[25];
+#pragma unroll
+      for (int b = 0; b < 25; ++b) tv[b] = b < m ? trow[b] : 0.0;
+#pragma unroll
+      for (int b = 0; b < 25; ++b)
+        if (b < m)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) acc[c] += tv[b] * dat[c * 56 + b];
+      if (a >= 0) {
+        const int slot = t < LG ? 0 : 1;
+        pa[slot] = a;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) part[slot][c] = s_data[c * 56 + a] * acc[c];
+      } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s_vk[(k * 6 + c) * 5 + r] = acc[c];
+      }
+    }
+    // v_k = 0 for k > r (weno_extrapolation_pre)
+    for (int z = l; z < 10; z += LG) {
+      const int kz = kZeroK[z], rz = kZeroR[z];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s_vk[(kz * 6 + c) * 5 + rz] = 0.0;
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+      if (pa[q] >= 0)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s_part[c * 56 + pa[q]] = part[q][c];
+  }
+  __syncthreads();
+  if (active) {  // beta_r and alpha_r = d_r / (eps + beta_r)^q, task (r, c)
+    for (int t = l; t < 30; t += LG) {
+      const int r = t / 6, c = t % 6;
+      const int m = (r + 1) * (r + 1);
+      const double* p = &s_part[c * 56 + r * (r + 1) * (2 * r + 1) / 6];
+      double beta = 0.0;
+      for (int a = 0; a < m; ++a) beta += p[a];
+      s_alpha[c * 5 + r] = hg.d5[5L * row + r] / weno_pow(ph.eps + beta, ph.weno_q);
+    }
+  }
+  __syncthreads();
+  if (active) {  // V^(k)_c = sum_r (alpha_r / sum alpha) v_k,r, task (k, c)
+    for (int t = l; t < 30; t += LG) {
+      const int k = t / 6, c = t % 6;
+      double asum = 0.0;
+#pragma unroll
+      for (int r = 0; r < 5; ++r) asum += s_alpha[c * 5 + r];
+      double res = 0.0;
+#pragma unroll
+      for (int r = 0; r < 5; ++r) res += (s_alpha[c * 5 + r] / asum) * s_vk[(k * 6 + c) * 5 + r];
+      s_v[k * 6 + c] = res;
+    }
+  }
+  __syncthreads();
+  const RoeX Qs = *s_q;
+  const double dist = hg.dist[h];
+  // FarField, src/bc1.rs:829-924: incoming characteristics of u^(0) from
+  // u_inf, of u^(k >= 1) zero.  Outflow, src/bc1.rs:760-828: the u - c
+  // characteristic of u^(1) from the pressure relaxation when incoming.
+  double* const s_uk = s_vk;        // [k][i]
+  double* const s_ug = s_vk + 30;   // Taylor sums
+  double* const s_fix = s_vk + 42;  // Outflow: flag, rhs_1[0]
+  if (active && l < 6 && !far) {    // Outflow: u^(0) = R V^(0)
+    double r[6], x[6];
+    rx_row_rt(ph, Qs, l, r);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) x[c] = s_v[c];
+    s_uk[l] = dot6(r, x);
+  }
+  __syncthreads();
+  if (active && l == 0 && !far) {
+    const double* uk = s_uk;
+    double rho0 = uk[0];
+    double un0 = uk[1] / rho0, ut0 = uk[2] / rho0;
+    double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
+    double ee1 = uk[3] / rho0 - w2 / 6.0;
+    double ei1 = uk[4] / rho0 - w2 / 6.0;
+    double er1 = uk[5] / rho0 - w2 / 6.0;
+    double p0 = rho0 * (ph.ge * ee1 + ph.gi * ei1 + ph.gr * er1);
+    double a2 = ph.gamma_e * ph.ge * ee1 + ph.gamma_i * ph.gi * ei1 + ph.gamma_r * ph.gr * er1;
+    double a = sqrt(a2);
+    double lambda1 = un0 - a;
+    s_fix[0] = 0.0;
+    if (lambda1 < 0.0) {
+      const double p_inf = hg.outflow[3L * h], sigma = hg.outflow[3L * h + 1], l_dom = hg.outflow[3L * h + 2];
+      double ml = fabs(un0 / a);
+      double l1 = sigma * (1.0 - sq(ml)) * a / l_dom * (p0 - p_inf);
+      s_fix[0] = 1.0;
+      s_fix[1] = l1 / lambda1;
+    }
+  }
+  __syncthreads();
+  if (active) {  // u^(k) = R rhs_k, task (k, i) (u^(0) of Outflow done)
+    for (int t = l; t < 30; t += LG) {
+      if (!far && t < 6) continue;
+      const int k = t / 6, i = t % 6;
+      double rhs[6], r[6];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) rhs[m] = s_v[k * 6 + m];
+      if (far) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+          if (lamx(Qs, m) < 0.0) rhs[m] = k == 0 ? s_data[m * 56 + 55] : 0.0;
+      } else if (k == 1 && s_fix[0] != 0.0) {
+        rhs[0] = s_fix[1];
+      }
+      rx_row_rt(ph, Qs, i, r);
+      s_uk[k * 6 + i] = dot6(r, rhs);
+    }
+  }
+  __syncthreads();
+  if (active && l < 6) {  // Taylor expansion to the ghost point
+    double ug = s_uk[l];
+    double coef = 1.0;
+    for (int k = 1; k <= 4; ++k) {
+      coef *= dist / (double)k;
+      ug = ug + coef * s_uk[k * 6 + l];
+    }
+    s_ug[l] = ug;
+  }
+  __syncthreads();
+  if (active && l < 6) {  // rotation back to (x, y)
+    const int f = l;
+    const double out = f == 1 ? s_ug[1] * nx - s_ug[2] * ny : (f == 2 ? s_ug[1] * ny + s_ug[2] * nx : s_ug[f]);
+    S[f * plane + hg.pad[h]] = out;
+    if (!isfinite(out)) raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
+  }
 }
 
 // Wall: three wavefronts per SIMD (168 VGPRs, 4 spilled; faster than two)
@@ -926,9 +1202,11 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, int gt, bool capture) {
     Launcher L(E, K_GHOST_HEAVY);
     const int g = E.ghosts_per_block, n_open = E.heavy.n_open, n_wall = E.heavy.n - n_open;
     const size_t lds = (size_t)g * kGhostLds * sizeof(double);
-    if (n_open > 0)
-      k_ghost_open<<<(n_open + g - 1) / g, dim3(64, g), lds, E.stream>>>(E.d_desc, E.heavy, E.phys, buf,
-                                                                                  phase, 0, n_open);
+    if (n_open > 0) {
+      const int per = g * kOpenPerWave;  // ghosts per workgroup
+      k_ghost_open<<<(n_open + per - 1) / per, dim3(64, g), (size_t)per * kOpenLds * sizeof(double), E.stream>>>(
+          E.d_desc, E.heavy, E.phys, buf, phase, 0, n_open);
+    }
     if (n_wall > 0)
       k_ghost_wall<<<(n_wall + g - 1) / g, dim3(64, g), lds, E.stream>>>(E.d_desc, E.heavy, E.phys, buf,
                                                                                  phase, n_open, n_wall);

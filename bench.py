@@ -225,6 +225,34 @@ def plain_leg(rh3t, field, u0, arith, steps, warmup):
             "finite": bool(abs(u_end[field.fluid]).max() < 1e300)}
 
 
+def cylinder_leg(rh3t, nx=121, mach=3.0, steps=200, arith="exact"):
+    """init_cylinder (src/init.rs:116-510) at 121x481, Mach 3: `steps` SSP-RK3
+    steps of dt = 0.05 dt_cfl on the device (graph replay), from the
+    uniform freestream; the per-kernel breakdown of 3 eagerly launched
+    steps with hipEvents."""
+    f = rh3t.Field.init_cylinder(nx, mach)
+    n_fluid = int(f.fluid.sum())
+    s = f.solver(arith=rh3t.ARITH_FAST if arith == "fast" else rh3t.ARITH_EXACT)
+    u = f.cylinder_state()
+    s.set_state(u, 0.0)
+    s.advance(2, 0.05)
+    s.profile(True)
+    s.advance(3, 0.05)
+    kern = {k: round(v[0] * v[1] / 3, 4) for k, v in s.profile_read().items() if v[1]}
+    s.profile(False)
+    s.set_state(u, 0.0)
+    s.advance(1, 0.05)  # graph captured again outside the timed region
+    s.set_state(u, 0.0)
+    t0 = time.perf_counter()
+    s.advance(steps, 0.05)
+    wall = time.perf_counter() - t0
+    counts = f.counts()
+    s.close()
+    return {"workload": f"init_cylinder {nx}x{f.grid.ny} Mach {mach}", "arith": arith, "fluid_cells": n_fluid,
+            "ghosts": counts[1], "heavy_ghosts": counts[2], "steps": steps, "ms_per_step": wall / steps * 1e3,
+            "cell_stage_per_s": n_fluid * 3 * steps / wall, "kernels_ms_per_step": kern}
+
+
 def _gpu_ms(profile):
     """Summed kernel time (ms) of a solver's per-kernel profile."""
     return sum(avg * cnt for avg, cnt in profile.values())
@@ -344,6 +372,8 @@ def main():
                     help="run only the batched-vs-plain leg (for profiling it); prints its JSON object")
     ap.add_argument("--double-mach-only", action="store_true",
                     help="run only the init_double_mach leg (for profiling it); prints its JSON object")
+    ap.add_argument("--cylinder-only", action="store_true",
+                    help="run only the init_cylinder 121x481 leg (for profiling it); prints its JSON object")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (three launches), the reference for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -373,6 +403,9 @@ def main():
         return
     if args.double_mach_only:
         print(json.dumps({"double_mach": double_mach_leg(rh3t)}), flush=True)
+        return
+    if args.cylinder_only:
+        print(json.dumps({"cylinder": cylinder_leg(rh3t, steps=args.steps, arith=args.arith)}), flush=True)
         return
     n = args.size
     log(f"building {n}x{n} periodic box")
@@ -434,6 +467,9 @@ def main():
         log(f"batched exact: {json.dumps(line['batched'])}")
         line["batched_fast"] = batched_leg(rh3t, args.batched_solves, args.batched_steps, arith=1)
         log(f"batched fast: {json.dumps(line['batched_fast'])}")
+    log("cylinder leg")
+    line["cylinder"] = cylinder_leg(rh3t)
+    log(f"cylinder: {json.dumps(line['cylinder'])}")
     if not args.no_double_mach:
         log("double Mach leg")
         line["double_mach"] = double_mach_leg(rh3t)
