@@ -488,54 +488,19 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
 // of L and R, the Outflow relaxation, the Taylor sum -- is evaluated once
 // per lane group instead of once per 64-lane wavefront, so the wavefront's
 // instructions serve kOpenPerWave ghosts at a time.  The 70 table rows of a
-// ghost (55 rows of S_r, the 15 rows of w_r) go to the group's lanes
-// round-robin (row t to lane t mod LG, in order), which also spreads the
-// long rows of r = 3, 4 over the lanes.  Every sum keeps the order of
+// ghost (55 rows of S_r, the 15 rows of w_r) run on the group's 32 lanes in
+// two segments of 25 and 16 steps (41 instead of the 75 that three rounds
+// of 25-entry rows would take), from the row block swizzled on the host so
+// that each step's loads are coalesced.  Every sum keeps the order of
 // weno_extrapolation_pre and the BC branches (src/bc1.rs:193-255,
 // 760-928), so the values are the same as one ghost per wavefront.
 constexpr int kOpenPerWave = 2;
 constexpr int kOpenLG = 64 / kOpenPerWave;
 constexpr int kOpenLds = 336 + 150 + 30 + 30 + 20;  // doubles per ghost
-static_assert(kOpenLG >= 28, "the S rows of a lane: at most two");
+static_assert(kOpenLG == 32, "the swizzled table schedule is for 32-lane groups");
 // the (k, r) with k > r, whose v_k,r are zero
 __constant__ int kZeroK[10] = {1, 2, 3, 4, 2, 3, 4, 3, 4, 4};
 __constant__ int kZeroR[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
-
-// Table row t of a ghost: t < 55 row a = t of S_r; 55..63 the rows of w_4
-// (k = 0..4) and w_3 (k = 0..3); 64..69 the w_r rows of r <= 2.  The block
-// offsets are the closed forms of the tables' cumulative sizes: E_r points
-// sum (i+1)^2, S_r entries sum (i+1)^4, w_r entries sum (i+1)^3 over i < r.
-struct OpenRow {
-  int r, k, a;        // a >= 0: S row a (k unused); a < 0: w row (r, k)
-  int m, dat, off;    // row length (r+1)^2, first E_r point, offset in S or W
-};
-
-__device__ __forceinline__ OpenRow open_row(int t) {
-  OpenRow o;
-  if (t < 55) {
-    o.a = t;
-    o.r = (t >= 1) + (t >= 5) + (t >= 14) + (t >= 30);
-    o.k = 0;
-  } else if (t < 64) {
-    const int q = t - 55;
-    o.a = -1;
-    o.r = q < 5 ? 4 : 3;
-    o.k = q < 5 ? q : q - 5;
-  } else {
-    const int q = t - 64;  // (2,0) (2,1) (2,2) (1,0) (1,1) (0,0)
-    o.a = -1;
-    o.r = q < 3 ? 2 : (q < 5 ? 1 : 0);
-    o.k = q < 3 ? q : (q < 5 ? q - 3 : 0);
-  }
-  const int r = o.r;
-  o.m = (r + 1) * (r + 1);
-  o.dat = r * (r + 1) * (2 * r + 1) / 6;
-  if (o.a >= 0)
-    o.off = r * (r + 1) * (2 * r + 1) * (3 * r * r + 3 * r - 1) / 30 + (o.a - o.dat) * o.m;
-  else
-    o.off = r * r * (r + 1) * (r + 1) / 4 + o.k * o.m;
-  return o;
-}
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ghost_open(
     const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
@@ -616,37 +581,72 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   }
   if (active && l == 0) *s_q = Q;
   __syncthreads();
-  // the products of the lane's S rows (rows l and l + LG; at most two are S
-  // rows) wait in registers until every lane has read the data
-  double part[2][6];
-  int pa[2] = {-1, -1};
   if (active) {
-    // the group's table rows t = l, l + LG, ... (sums over b in increasing
-    // order, as weno_extrapolation_pre)
-    const double* Sh = hg.S + 979L * row;
-    const double* Wh = hg.W + 225L * row;
-    for (int t = l; t < 70; t += LG) {
-      const OpenRow o = open_row(t);
-      const int r = o.r, k = o.k, a = o.a, m = o.m;
-      const double* trow = (a >= 0 ? Sh : Wh) + o.off;
-      const double* dat = s_data + o.dat;
+    // The 70 table rows on a fixed two-segment schedule (kSwz* in
+    // engine.hpp): step s of the swizzled row block holds the entry each of
+    // the group's 32 lanes applies at that step, so every load of the block
+    // is 32 consecutive doubles.  Each row sums over b in increasing order
+    // from 0.0, as weno_extrapolation_pre.  An S row's product data[a] * (S
+    // data)[a] goes straight into data slot a once no lane reads that E_r
+    // segment any more (the group is one wavefront: LDS accesses complete in
+    // program order), a w row's sum into vk.
+    const double* T = hg.T + (long)kSwzLen * row + l;
+    {  // segment A: S_4 rows (lanes 0..24), w_4 rows (25..29), S_0 / w_0 (30, 31)
+      const int m = l < 30 ? 25 : 1, d0 = l < 30 ? 30 : 0;
+      double tv[kSwzA];
+#pragma unroll
+      for (int b = 0; b < kSwzA; ++b) tv[b] = T[32 * b];
       double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      // the whole row's loads in flight at once (one round trip per row),
-      // then the sums in order
-      double tv[25];
 #pragma unroll
-      for (int b = 0; b < 25; ++b) tv[b] = b < m ? trow[b] : 0.0;
-#pragma unroll
-      for (int b = 0; b < 25; ++b)
+      for (int b = 0; b < kSwzA; ++b)
         if (b < m)
 #pragma unroll
-          for (int c = 0; c < 6; ++c) acc[c] += tv[b] * dat[c * 56 + b];
-      if (a >= 0) {
-        const int slot = t < LG ? 0 : 1;
-        pa[slot] = a;
+          for (int c = 0; c < 6; ++c) acc[c] += tv[b] * s_data[c * 56 + d0 + b];
+      if (l < 25 || l == 30) {
+        const int a = l < 25 ? 30 + l : 0;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) part[slot][c] = s_data[c * 56 + a] * acc[c];
+        for (int c = 0; c < 6; ++c) s_data[c * 56 + a] = s_data[c * 56 + a] * acc[c];
       } else {
+        const int k = l < 30 ? l - 25 : 0, r = l < 30 ? 4 : 0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s_vk[(k * 6 + c) * 5 + r] = acc[c];
+      }
+    }
+    {  // segment B: S_3 / w_3 rows (lanes 0..19) over 16 steps; S_2 / w_2
+       // rows (20..31) over the first 9, then S_1 / w_1 rows (20..25) over 4
+      double tv[kSwzB];
+#pragma unroll
+      for (int b = 0; b < kSwzB; ++b) tv[b] = T[32 * (kSwzA + b)];
+      const int d0 = l < 20 ? 14 : 5;
+      double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int b = 0; b < 9; ++b)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] += tv[b] * s_data[c * 56 + d0 + b];
+      if (l >= 20) {
+        if (l < 29) {
+          const int a = 5 + (l - 20);
+#pragma unroll
+          for (int c = 0; c < 6; ++c) s_data[c * 56 + a] = s_data[c * 56 + a] * acc[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 6; ++c) s_vk[((l - 29) * 6 + c) * 5 + 2] = acc[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] = 0.0;
+      }
+      const int m = l < 20 ? 16 : (l < 26 ? 13 : 9), d1 = l < 20 ? 14 : -8;
+#pragma unroll
+      for (int b = 9; b < kSwzB; ++b)
+        if (b < m)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) acc[c] += tv[b] * s_data[c * 56 + d1 + b];
+      if (l < 16 || (l >= 20 && l < 24)) {
+        const int a = l < 16 ? 14 + l : 1 + (l - 20);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s_data[c * 56 + a] = s_data[c * 56 + a] * acc[c];
+      } else if (l < 20 || l < 26) {
+        const int k = l < 20 ? l - 16 : l - 24, r = l < 20 ? 3 : 1;
 #pragma unroll
         for (int c = 0; c < 6; ++c) s_vk[(k * 6 + c) * 5 + r] = acc[c];
       }
@@ -657,14 +657,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 #pragma unroll
       for (int c = 0; c < 6; ++c) s_vk[(kz * 6 + c) * 5 + rz] = 0.0;
     }
-  }
-  __syncthreads();
-  if (active) {
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-      if (pa[q] >= 0)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) s_part[c * 56 + pa[q]] = part[q][c];
   }
   __syncthreads();
   if (active) {  // beta_r and alpha_r = d_r / (eps + beta_r)^q, task (r, c)
@@ -1428,6 +1420,47 @@ void free_engine(Engine& E) {
   E.allocs.clear();
 }
 
+// Appends the swizzled row block (kSwzLen doubles, layout at kSwzA in
+// engine.hpp) of one heavy ghost's S (979) and w (225) tables.  S_r row a is
+// at sum_{i<r} (i+1)^4 + a (r+1)^2, w_r row k at sum_{i<r} (i+1)^3 + k (r+1)^2.
+static void swizzle_rows(const double* S, const double* W, std::vector<double>& out) {
+  auto s_at = [&](int r, int a, int b) {
+    const int m = (r + 1) * (r + 1);
+    return S[r * (r + 1) * (2 * r + 1) * (3 * r * r + 3 * r - 1) / 30 + a * m + b];
+  };
+  auto w_at = [&](int r, int k, int b) {
+    const int m = (r + 1) * (r + 1);
+    return W[r * r * (r + 1) * (r + 1) / 4 + k * m + b];
+  };
+  const size_t base = out.size();
+  out.resize(base + kSwzLen, 0.0);
+  double* o = out.data() + base;
+  for (int b = 0; b < kSwzA; ++b)
+    for (int l = 0; l < 32; ++l) {
+      double v = 0.0;
+      if (l < 25)
+        v = s_at(4, l, b);
+      else if (l < 30)
+        v = w_at(4, l - 25, b);
+      else if (b == 0)
+        v = l == 30 ? s_at(0, 0, 0) : w_at(0, 0, 0);
+      o[32 * b + l] = v;
+    }
+  for (int b = 0; b < kSwzB; ++b)
+    for (int l = 0; l < 32; ++l) {
+      double v = 0.0;
+      if (l < 16)
+        v = s_at(3, l, b);
+      else if (l < 20)
+        v = w_at(3, l - 16, b);
+      else if (b < 9)
+        v = l < 29 ? s_at(2, l - 20, b) : w_at(2, l - 29, b);
+      else if (b < 13 && l < 26)
+        v = l < 24 ? s_at(1, l - 20, b - 9) : w_at(1, l - 24, b - 9);
+      o[32 * (kSwzA + b) + l] = v;
+    }
+}
+
 int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_physics& phys, int device) {
   E.device = device;
   E.src_phys = phys;
@@ -1456,7 +1489,7 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   std::vector<int> c_sid, c_kind, c_gid, h_sid, h_kind, h_gid, h_tab;
   std::vector<std::pair<const void*, int>> table_base;  // shared heavy table -> first row
   std::vector<long> c_pad, c_src, h_pad, h_near, h_st;
-  std::vector<double> c_n, c_state, h_n, h_dist, h_state, h_of, h_d5, h_S, h_W, c_p0x, c_state2, c_front;
+  std::vector<double> c_n, c_state, h_n, h_dist, h_state, h_of, h_d5, h_S, h_W, h_T, c_p0x, c_state2, c_front;
   long max_cells = 0, max_ghost = 1;
   for (int s = 0; s < n; ++s) {
     const rh3t::Problem& pb = *P[s];
@@ -1542,6 +1575,7 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
         }
         h_S.insert(h_S.end(), hb.s, hb.s + 979);
         h_W.insert(h_W.end(), hb.w, hb.w + 225);
+        swizzle_rows(hb.s, hb.w, h_T);
       }
     }
     SolveDesc& d = E.h_desc[s];
@@ -1641,6 +1675,7 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   UP(h_st, lp) E.heavy.stencil = lp;
   UP(h_S, dp) E.heavy.S = dp;
   UP(h_W, dp) E.heavy.W = dp;
+  UP(h_T, dp) E.heavy.T = dp;
 #undef UP
   if ((st = build_tables(E))) return st;
   return RH3T_OK;
