@@ -26,7 +26,12 @@
 // The parameters are (omega_ei, omega_er, kappa_e, kappa_i, kappa_r)
 // (src/constant.rs:17-25); their cotangents are formed whether or not the
 // forward skips the zero-valued terms.  The time step is not differentiated
-// (dt is frozen).  Heavy (FarField / Outflow / Wall) ghosts are not covered.
+// (dt is frozen).
+//   k_adj_heavy      per FarField / Outflow ghost: the pullback of its
+//                    value (src/bc1.rs:193-255, 760-928) to the states of
+//                    its E_r points and its nearest point
+//   k_adj_heavy_T    adds those into the cells (fixed order)
+// Wall (ILW) ghosts are not covered.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -597,6 +602,392 @@ __global__ __launch_bounds__(256) void k_adj_ghost_T(int nrows, const long* __re
   for (int f = 0; f < 6; ++f) OUT[f * P + c] = acc[f];
 }
 
+// ---------------------------------------- heavy ghosts (FarField / Outflow)
+
+// E_r geometry of the heavy-ghost tables (GhostBC, src/bc1.rs:193-255): the
+// sub-stencil r holds (r+1)^2 points from slot sum_{i<r} (i+1)^2; S_r row a
+// is at sum_{i<r} (i+1)^4 + a (r+1)^2, w_r row k at sum_{i<r} (i+1)^3 +
+// k (r+1)^2.
+__device__ __forceinline__ int er_of(int a) { return (a >= 1) + (a >= 5) + (a >= 14) + (a >= 30); }
+__device__ __forceinline__ int er_seg(int r) { return r * (r + 1) * (2 * r + 1) / 6; }
+__device__ __forceinline__ int er_soff(int r) { return r * (r + 1) * (2 * r + 1) * (3 * r * r + 3 * r - 1) / 30; }
+__device__ __forceinline__ int er_woff(int r) { return r * r * (r + 1) * (r + 1) / 4; }
+
+// The recomputed forward quantities of one ghost and their cotangents.
+struct HeavyLds {
+  double u[56][6];       // E_r point states in the (n, t) frame; 55: u_inf (FarField)
+  double L[6][6], R[6][6], lam[6];  // rows of L, R[i][m], eigenvalues
+  double d[6][56];       // L . u
+  double sd[6][55];      // (S_r d)[a]
+  double v[5][5][6];     // v_{k,r}[c] (0 for k > r)
+  double beta[5][6], alpha[5][6], om[5][6], asum[6];
+  double V[5][6], rhs[5][6];
+  double wV[5][6], wR[6][6], wL[6][6], wom[5][6];
+  double wv[5][5][6], wbeta[5][6];
+  double wd[6][56];
+};
+
+// rhs_1[0] = l1 / lambda1 of the Outflow relaxation (src/bc1.rs:783-787)
+// from u^(0) in the (n, t) frame, and its gradient g; false (g = 0) when the
+// u - c characteristic is outgoing and nothing is replaced.
+__device__ bool outflow_relax(const Phys& ph, const double uk[6], double p_inf, double sigma, double l_dom,
+                              double* phi, double g[6]) {
+  const double rho = uk[0], un = uk[1] / rho, ut = uk[2] / rho;
+  const double w2 = un * un + ut * ut;
+  const double e[3] = {uk[3] / rho - w2 / 6.0, uk[4] / rho - w2 / 6.0, uk[5] / rho - w2 / 6.0};
+  const double gk[3] = {ph.ge, ph.gi, ph.gr};
+  const double gam[3] = {ph.gamma_e, ph.gamma_i, ph.gamma_r};
+  const double Pm = gk[0] * e[0] + gk[1] * e[1] + gk[2] * e[2];
+  const double p0 = rho * Pm;
+  const double a2 = gam[0] * gk[0] * e[0] + gam[1] * gk[1] * e[1] + gam[2] * gk[2] * e[2];
+  const double a = sqrt(a2);
+  const double lam1 = un - a;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) g[f] = 0.0;
+  if (!(lam1 < 0.0)) return false;
+  const double K1 = sigma / l_dom, dp = p0 - p_inf;
+  const double f1 = 1.0 - un * un / a2;
+  const double ml = fabs(un / a);
+  *phi = sigma * (1.0 - sq(ml)) * a / l_dom * dp / lam1;  // the forward's value
+  // d phi / d (un, a2, p0)
+  const double g_un = (K1 * a * dp * (-2.0 * un / a2) - *phi) / lam1;
+  const double g_a2 = (K1 * dp * (a * un * un / (a2 * a2) + f1 / (2.0 * a)) + *phi / (2.0 * a)) / lam1;
+  const double g_p0 = K1 * f1 * a / lam1;
+  // to u^(0)
+  double grho = g_p0 * Pm, gw2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double ge = g_p0 * rho * gk[k] + g_a2 * gam[k] * gk[k];
+    g[3 + k] = ge / rho;
+    grho -= ge * uk[3 + k] / (rho * rho);
+    gw2 -= ge / 6.0;
+  }
+  const double gun = g_un + 2.0 * un * gw2, gut = 2.0 * ut * gw2;
+  g[1] = gun / rho;
+  g[2] = gut / rho;
+  grho -= (gun * un + gut * ut) / rho;
+  g[0] = grho;
+  return true;
+}
+
+// Pullback of one FarField / Outflow ghost value (weno_extrapolation_pre and
+// the BC branches, src/bc1.rs:193-255, 294-312, 760-928) to the states it
+// reads: its 55 E_r points and its nearest point.  One 64-lane workgroup per
+// ghost recomputes the forward quantities from the stage state with the
+// forward kernel's own operations (so the derivative is taken at the very
+// values the forward formed: the WENO weights d_r / (eps + beta_r)^q make
+// it sensitive to the last bits of beta_r) and sweeps them in reverse.  The
+// eigensystem is that of the Roe average of the nearest state with itself,
+// whose derivative with respect to that state's (u, v, e_e, e_i, e_r) is the
+// identity, so the eigensystem pullback chains to the state through
+// derived_vjp.  The incoming-characteristic choice of
+// FarField and the Outflow branch take the derivative of the branch the
+// forward value takes.  GB: 56 x 6 per ghost (the E_r points a = 0..54, then
+// the nearest point), in the (x, y) frame.
+__global__ __launch_bounds__(64) void k_adj_heavy(AdjDesc A, HeavyGhosts hg, const EigConst* __restrict__ Kp,
+                                                  Phys ph, double* __restrict__ GB) {
+  __shared__ HeavyLds T;
+  const int h = blockIdx.x, lane = threadIdx.x;
+  const long P = A.g.plane;
+  const double* __restrict__ S = A.S;
+  const long row = hg.tab[h];
+  const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
+  const bool far = hg.kind[h] == RH3T_BC_FARFIELD;
+  const double* Sh = hg.S + 979L * row;
+  const double* Wh = hg.W + 225L * row;
+  double s1[6];
+  {
+    const long q = hg.nearest[h];
+    double val[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) val[f] = S[f * P + q];
+    s1[0] = val[0];
+    s1[1] = val[1] * nx + val[2] * ny;
+    s1[2] = -val[1] * ny + val[2] * nx;
+    s1[3] = val[3];
+    s1[4] = val[4];
+    s1[5] = val[5];
+  }
+  // the forward's eigensystem (roe_x, lx_row, rx_row_rt: the same values as
+  // k_ghost_open), and its intermediates for the pullback
+  RoeV Rv;
+  RoeX Q;
+  {
+    double d1[D_N];
+    derived_from_state(ph, s1, d1, false);
+    const double e[3] = {d1[D_EE], d1[D_EI], d1[D_ER]};
+    roe_values<0>(*fresh(Kp), d1[D_U], d1[D_V], e, Rv);
+    roe_x(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI],
+          d1[D_ER], Q);
+  }
+  if (lane < 6) {
+    double rw[6];
+    lx_row(ph, Q, lane, rw);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) T.L[lane][m] = rw[m];
+    rx_row_rt(ph, Q, lane, rw);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) T.R[lane][m] = rw[m];
+    T.lam[lane] = lamx(Q, lane);
+  }
+  if (lane < 56) {
+    double s6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (lane < 55) {
+      const long q = hg.stencil[55L * row + lane];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) s6[f] = S[f * P + q];
+    } else if (far) {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) s6[f] = hg.bcstate[6L * h + f];
+    }
+    T.u[lane][0] = s6[0];
+    T.u[lane][1] = s6[1] * nx + s6[2] * ny;
+    T.u[lane][2] = -s6[1] * ny + s6[2] * nx;
+    T.u[lane][3] = s6[3];
+    T.u[lane][4] = s6[4];
+    T.u[lane][5] = s6[5];
+  }
+  __syncthreads();
+  if (lane < 56)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) T.d[c][lane] = dot6(T.L[c], T.u[lane]);
+  __syncthreads();
+  if (lane < 55) {  // (S_r d)[a]
+    const int a = lane, r = er_of(a), m = (r + 1) * (r + 1), seg = er_seg(r);
+    const double* tr = Sh + er_soff(r) + (a - seg) * m;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double acc = 0.0;
+      for (int b = 0; b < m; ++b) acc += tr[b] * T.d[c][seg + b];
+      T.sd[c][a] = acc;
+    }
+  }
+  if (lane < 25) {  // v_{k,r} = w_r row k . d
+    const int r = lane / 5, k = lane % 5, m = (r + 1) * (r + 1), seg = er_seg(r);
+    const double* tr = Wh + er_woff(r) + k * m;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double acc = 0.0;
+      if (k <= r)
+        for (int b = 0; b < m; ++b) acc += tr[b] * T.d[c][seg + b];
+      T.v[k][r][c] = acc;
+    }
+  }
+  __syncthreads();
+  if (lane < 30) {
+    const int r = lane / 6, c = lane % 6, m = (r + 1) * (r + 1), seg = er_seg(r);
+    double beta = 0.0;
+    for (int a = 0; a < m; ++a) beta += T.d[c][seg + a] * T.sd[c][seg + a];
+    T.beta[r][c] = beta;
+    T.alpha[r][c] = hg.d5[5L * row + r] / weno_pow(ph.eps + beta, ph.weno_q);
+  }
+  __syncthreads();
+  if (lane < 6) {
+    double as = 0.0;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) as += T.alpha[r][lane];
+    T.asum[lane] = as;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) T.om[r][lane] = T.alpha[r][lane] / as;
+  }
+  __syncthreads();
+  if (lane < 30) {
+    const int k = lane / 6, c = lane % 6;
+    double V = 0.0;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) V += T.om[r][c] * T.v[k][r][c];
+    T.V[k][c] = V;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    // forward: rhs_k of the BC branch
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+      for (int m = 0; m < 6; ++m) T.rhs[k][m] = T.V[k][m];
+    bool fix = false;
+    double gphi[6];
+    if (far) {
+#pragma unroll
+      for (int m = 0; m < 6; ++m)
+        if (T.lam[m] < 0.0) {
+          T.rhs[0][m] = T.d[m][55];
+#pragma unroll
+          for (int k = 1; k < 5; ++k) T.rhs[k][m] = 0.0;
+        }
+    } else {
+      double u0[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) u0[i] = dot6(T.R[i], T.V[0]);
+      double phi = 0.0;
+      fix = outflow_relax(ph, u0, hg.outflow[3L * h], hg.outflow[3L * h + 1], hg.outflow[3L * h + 2], &phi, gphi);
+      if (fix) T.rhs[1][0] = phi;
+    }
+    // reverse: the ghost cotangent in the (n, t) frame, through the Taylor
+    // sum to u^(k) = R rhs_k
+    const long pad = hg.pad[h];
+    const double w1 = A.SB[1 * P + pad], w2 = A.SB[2 * P + pad];
+    const double wug[6] = {A.SB[pad], w1 * nx + w2 * ny, -w1 * ny + w2 * nx, A.SB[3 * P + pad],
+                           A.SB[4 * P + pad], A.SB[5 * P + pad]};
+    const double dist = hg.dist[h];
+    double coef[5];
+    coef[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k < 5; ++k) coef[k] = coef[k - 1] * (dist / (double)k);
+    double wR[6][6], wrhs[5][6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int m = 0; m < 6; ++m) wR[i][m] = 0.0;
+    double wu0[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) wu0[i] = wug[i];
+    for (int k = 4; k >= 0; --k) {
+      double wuk[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) wuk[i] = k == 0 ? wu0[i] : coef[k] * wug[i];
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          acc += T.R[i][m] * wuk[i];
+          wR[i][m] += wuk[i] * T.rhs[k][m];
+        }
+        wrhs[k][m] = acc;
+      }
+      if (k == 1 && fix) {  // rhs_1[0] = phi(u^(0)), u^(0) = R V^(0)
+#pragma unroll
+        for (int i = 0; i < 6; ++i) wu0[i] += gphi[i] * wrhs[1][0];
+        wrhs[1][0] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) T.wd[c][55] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const bool in = far && T.lam[m] < 0.0;
+      if (in) T.wd[m][55] = wrhs[0][m];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) T.wV[k][m] = in ? 0.0 : wrhs[k][m];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int m = 0; m < 6; ++m) T.wR[i][m] = wR[i][m];
+  }
+  __syncthreads();
+  if (lane < 30) {  // V_k = sum_r om_r v_{k,r}
+    const int r = lane / 6, c = lane % 6;
+    double wom = 0.0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      wom += T.wV[k][c] * T.v[k][r][c];
+      T.wv[k][r][c] = T.om[r][c] * T.wV[k][c];
+    }
+    T.wom[r][c] = wom;
+  }
+  __syncthreads();
+  if (lane < 30) {  // om_r = alpha_r / sum alpha, alpha_r = d_r / (eps + beta_r)^q
+    const int r = lane / 6, c = lane % 6;
+    double dot = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) dot += T.om[q][c] * T.wom[q][c];
+    const double walpha = (T.wom[r][c] - dot) / T.asum[c];
+    T.wbeta[r][c] = -walpha * ph.weno_q * T.alpha[r][c] / (ph.eps + T.beta[r][c]);
+  }
+  __syncthreads();
+  if (lane < 55) {  // d at the E_r points: w rows and beta = d . S d
+    const int p = lane, r = er_of(p), m = (r + 1) * (r + 1), seg = er_seg(r), b = p - seg;
+    const double* sc = Sh + er_soff(r) + b;
+    const double* wc = Wh + er_woff(r) + b;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double acc = 0.0;
+      for (int k = 0; k <= r; ++k) acc += wc[k * m] * T.wv[k][r][c];
+      double st = 0.0;
+      for (int a = 0; a < m; ++a) st += sc[a * m] * T.d[c][seg + a];
+      T.wd[c][p] = acc + T.wbeta[r][c] * (T.sd[c][p] + st);
+    }
+  }
+  __syncthreads();
+  if (lane < 55) {  // d = L . u: the E_r points' state cotangents, (x, y) frame
+    double wu[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      double acc = 0.0;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) acc += T.wd[c][lane] * T.L[c][f];
+      wu[f] = acc;
+    }
+    double* o = GB + ((long)h * 56 + lane) * 6;
+    o[0] = wu[0];
+    o[1] = wu[1] * nx - wu[2] * ny;
+    o[2] = wu[1] * ny + wu[2] * nx;
+    o[3] = wu[3];
+    o[4] = wu[4];
+    o[5] = wu[5];
+  }
+  if (lane < 36) {
+    const int c = lane / 6, f = lane % 6;
+    double acc = 0.0;
+    for (int a = 0; a < 56; ++a) acc += T.wd[c][a] * T.u[a][f];
+    T.wL[c][f] = acc;
+  }
+  __syncthreads();
+  if (lane == 0) {  // the eigensystem to the nearest state
+    RoeBar B;
+    roe_bar_zero(B);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double Rb[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) Rb[i] = T.wR[i][c];
+      row_col_bar<0>(*fresh(Kp), Rv, c, T.wL[c], Rb, 0.0, B);
+    }
+    double o5[5];
+    roe_bar_final<0>(*fresh(Kp), Rv, B, o5);
+    double db[D_N];
+#pragma unroll
+    for (int k = 0; k < D_N; ++k) db[k] = 0.0;
+    db[D_U] = o5[0];
+    db[D_V] = o5[1];
+    db[D_EE] = o5[2];
+    db[D_EI] = o5[3];
+    db[D_ER] = o5[4];
+    double sb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    derived_vjp(ph, s1, db, sb);
+    double* o = GB + ((long)h * 56 + 55) * 6;
+    o[0] = sb[0];
+    o[1] = sb[1] * nx - sb[2] * ny;
+    o[2] = sb[1] * ny + sb[2] * nx;
+    o[3] = sb[3];
+    o[4] = sb[4];
+    o[5] = sb[5];
+  }
+}
+
+// One lane per cell read by a heavy ghost: adds the ghosts' pullbacks to it
+// (ghost, then point order).
+__global__ __launch_bounds__(256) void k_adj_heavy_T(int nrows, const long* __restrict__ row_cell,
+                                                     const int* __restrict__ row_ptr, const int* __restrict__ ent,
+                                                     const double* __restrict__ GB, double* __restrict__ OUT,
+                                                     long P) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const long c = row_cell[r];
+  double acc[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) acc[f] = OUT[f * P + c];
+  for (int e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+    const double* g = GB + (long)ent[e] * 6;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) acc[f] += g[f];
+  }
+#pragma unroll
+  for (int f = 0; f < 6; ++f) OUT[f * P + c] = acc[f];
+}
+
 // acc[k] += sum over blocks of PG (fixed order: per-lane strided sums, then a
 // tree); one workgroup.
 __global__ __launch_bounds__(256) void k_adj_param(const double* __restrict__ PG, int nblocks, double* acc) {
@@ -676,6 +1067,13 @@ struct AdjWork {
   long* ent_pad = nullptr;
   int* ent_kind = nullptr;
   double* ent_n = nullptr;
+  // heavy ghosts: pullbacks (56 x 6 per ghost) and their transpose lists
+  int n_heavy = 0;
+  double* GB = nullptr;
+  int hrows = 0;
+  long* hrow_cell = nullptr;
+  int* hrow_ptr = nullptr;
+  int* hent = nullptr;
   double* dts = nullptr;   // frozen dt schedule (device)
   int dts_cap = 0;
   // the ghosts outside the fluid box are gathered on a second stream, beside
@@ -707,15 +1105,42 @@ int init_adjoint(Engine& E) {
     std::snprintf(E.message, sizeof(E.message), "the adjoint covers single solves");
     return RH3T_ERR_UNSUPPORTED;
   }
-  if (E.heavy.n > 0) {
-    std::snprintf(E.message, sizeof(E.message),
-                  "the adjoint covers Periodic / ReflectiveWall / ZerothOrder / Constant ghosts only");
+  if (E.heavy.n > E.heavy.n_open) {
+    std::snprintf(E.message, sizeof(E.message), "the adjoint does not cover Wall (ILW) ghosts");
     return RH3T_ERR_UNSUPPORTED;
   }
   HIP_TRY(hipSetDevice(E.device));
   const SolveDesc& d = E.h_desc[0];
   const Geo& g = d.g;
   const SolveHost& sh = E.solves[0];
+  auto is_fluid_slot = [&](long q) {
+    const long i = q / g.PY - 4, j = q % g.PY - 4;
+    return i >= 0 && i < g.nx && j >= 0 && j < g.ny && sh.fluid[i * g.ny + j];
+  };
+  // heavy-ghost transpose lists grouped by cell: (cell slot, ghost * 56 +
+  // point), the points being the 55 E_r points and the nearest point
+  std::vector<std::pair<long, int>> hents;
+  for (int h = 0; h < E.heavy.n_open; ++h)
+    for (int a = 0; a < 56; ++a) {
+      const long q = a < 55 ? E.h_heavy_stencil[55L * E.h_heavy_tab[h] + a] : E.h_heavy_near[h];
+      if (!is_fluid_slot(q)) {
+        std::snprintf(E.message, sizeof(E.message), "heavy ghost reads a point that is not a fluid cell");
+        return RH3T_ERR_UNSUPPORTED;
+      }
+      hents.push_back({q, h * 56 + a});
+    }
+  std::stable_sort(hents.begin(), hents.end(),
+                   [](const std::pair<long, int>& a, const std::pair<long, int>& b) { return a.first < b.first; });
+  std::vector<long> hrow_cell;
+  std::vector<int> hrow_ptr, hent;
+  for (size_t e = 0; e < hents.size(); ++e) {
+    if (e == 0 || hents[e].first != hents[e - 1].first) {
+      hrow_cell.push_back(hents[e].first);
+      hrow_ptr.push_back((int)e);
+    }
+    hent.push_back(hents[e].second);
+  }
+  hrow_ptr.push_back((int)hents.size());
   // ghost transpose lists grouped by source cell (sources must be fluid)
   std::vector<std::pair<long, int>> ents;  // (source slot, ghost list index)
   for (int k = 0; k < E.cheap.n; ++k) {
@@ -768,8 +1193,8 @@ int init_adjoint(Engine& E) {
   if ((st = dev_alloc(E, &W->YB, 36 * (size_t)g.nx * (g.ny + 1)))) return st;
   W->pg_blocks = ((g.nx + kGy - 1) / kGy) * ((g.ny + kGx - 1) / kGx);
   std::vector<long> out_pads;
-  for (int k = 0; k < E.cheap.n; ++k) {
-    const long p = E.h_cheap_pad[k];
+  for (int k = 0; k < E.cheap.n + E.heavy.n_open; ++k) {
+    const long p = k < E.cheap.n ? E.h_cheap_pad[k] : E.h_heavy_pad[k - E.cheap.n];
     const long ip = p / g.PY, jp = p % g.PY;
     if (ip < 4 || ip >= g.nx + 4 || jp < 4 || jp >= g.ny + 4) out_pads.push_back(p);
   }
@@ -784,6 +1209,14 @@ int init_adjoint(Engine& E) {
   if ((st = upload(E, &W->ent_pad, ent_pad))) return st;
   if ((st = upload(E, &W->ent_kind, ent_kind))) return st;
   if ((st = upload(E, &W->ent_n, ent_n))) return st;
+  W->n_heavy = E.heavy.n_open;
+  if (W->n_heavy > 0) {
+    if ((st = dev_alloc(E, &W->GB, (size_t)W->n_heavy * 56 * 6))) return st;
+    W->hrows = (int)hrow_cell.size();
+    if ((st = upload(E, &W->hrow_cell, hrow_cell))) return st;
+    if ((st = upload(E, &W->hrow_ptr, hrow_ptr))) return st;
+    if ((st = upload(E, &W->hent, hent))) return st;
+  }
   if (W->n_out > 0) {
     HIP_TRY(hipStreamCreateWithFlags(&W->side, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&W->ev_iface, hipEventDisableTiming));
@@ -862,6 +1295,12 @@ void launch_vjp(Engine& E, int buf, bool taped, const double* LAM, double rscale
     Launcher L(E, K_ADJ_GHOST);
     k_adj_ghost_T<<<(W.nrows + 255) / 256, 256, 0, E.stream>>>(W.nrows, W.row_cell, W.row_ptr, W.ent_pad,
                                                                W.ent_kind, W.ent_n, W.SB, OUT, g.plane);
+  }
+  if (W.n_heavy > 0) {
+    Launcher L(E, K_ADJ_GHOST);
+    k_adj_heavy<<<W.n_heavy, 64, 0, E.stream>>>(A, E.heavy, E.d_eig, E.phys, W.GB);
+    k_adj_heavy_T<<<(W.hrows + 255) / 256, 256, 0, E.stream>>>(W.hrows, W.hrow_cell, W.hrow_ptr, W.hent, W.GB,
+                                                               OUT, g.plane);
   }
 }
 

@@ -1639,6 +1639,11 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
     permute(h_state, 6);
     permute(h_of, 3);
   }
+  E.h_heavy_kind = h_kind;
+  E.h_heavy_tab = h_tab;
+  E.h_heavy_pad = h_pad;
+  E.h_heavy_near = h_near;
+  E.h_heavy_stencil = h_st;
   long* lp;
   int* ip;
   double* dp;

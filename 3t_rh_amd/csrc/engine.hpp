@@ -131,6 +131,10 @@ struct Engine {
   std::vector<int> h_cheap_sid, h_cheap_kind;
   std::vector<long> h_cheap_pad, h_cheap_src;
   std::vector<double> h_cheap_normal;
+  // host copies of the heavy-ghost lists in launch order, and the stencil
+  // slots of every table row (the adjoint transposes the heavy ghosts)
+  std::vector<int> h_heavy_kind, h_heavy_tab;
+  std::vector<long> h_heavy_pad, h_heavy_near, h_heavy_stencil;
 
   // launch shapes (autotuned): flux / assemble workgroup (bx, by), heavy
   // ghosts per workgroup

@@ -6,7 +6,9 @@ Checks, in increasing scope:
     itself pinned to central differences by test_oracle.py):
     <w, J v> (oracle, forward mode) == <J^T w, v> (GPU, reverse mode) for
     seeded state + parameter directions v and cotangents w, on periodic and
-    on walled (ReflectiveWall / Constant / ZerothOrder) boxes;
+    on walled (ReflectiveWall / Constant / ZerothOrder) boxes, through the
+    FarField ghosts of the cylinder and the Outflow ghosts of the
+    translating shock;
   * the checkpoint schedule does not change the gradient (bitwise);
   * the differentiation workload (512x512 smooth-mode box, exchange on,
     20 steps, checkpoints every 5, J = sum of a Gaussian window times the
@@ -173,13 +175,74 @@ def test_interleaved_rollouts_keep_their_physics(rh3t, workloads):
     s.close()
 
 
-def test_heavy_ghosts_are_rejected(rh3t, workloads):
-    """The reverse mode covers the cheap ghost kinds; FarField / Outflow /
-    Wall problems must fail loudly rather than return a partial gradient."""
-    _say("adjoint on the cylinder: unsupported")
-    f = workloads.cylinder(121, 3.0)
+def _heavy_case(workloads, which):
+    """FarField: the Mach-3 cylinder 121x481 (4,480 FarField ghosts), its
+    freestream perturbed by 1% of a smooth direction so that the WENO
+    weights and the characteristic projections vary along the boundary.
+    Outflow: the translating Mach-2 shock on 64x64 (LODI relaxation active:
+    subsonic flow at the right side), perturbed the same way.  The
+    perturbation also keeps the max / min of the flux splitting and of the
+    noncon limiter off ties, where the dual oracle and the reverse mode may
+    take the derivatives of different (equal-valued) branches."""
+    if which == "farfield":
+        f = workloads.cylinder(121, 3.0)
+        u0 = f.cylinder_state()
+    else:
+        f = workloads.translating_shock(64)
+        u0 = workloads.translating_shock_state(f)
+    u = u0 + 0.01 * workloads.smooth_direction(f, u0, 7)
+    u[~f.fluid] = u0[~f.fluid]
+    return f, u, ob.OracleProblem(f)
+
+
+@pytest.mark.parametrize("which", ["farfield", "outflow"])
+def test_l_vjp_dot_product_heavy_ghosts(rh3t, workloads, which):
+    """The VJP of l() through the FarField / Outflow ghosts (their WENO
+    extrapolation, characteristic BC branch and Taylor sum): dot-product
+    identity against the dual-number oracle, which differentiates the same
+    ghost code as the forward oracle."""
+    f, u, p = _heavy_case(workloads, which)
+    _say(f"l_vjp dot product through {which} ghosts {f.grid.nx}x{f.grid.ny}")
+    s = f.solver(physics=rh3t.Physics(**KW_ACTIVE))
+    s.set_state(u)
+    rng = np.random.default_rng(51)
+    w = rng.standard_normal(u.shape)
+    w[~f.fluid] = 0.0
+    du = workloads.state_direction(u, f.fluid, 52)
+    dpar = rng.standard_normal(5) * PSCALE
+    ubar, pg = s.l_vjp(w)
+    _, jv = ob.l_jvp(p, u, du, ob.physics(**KW_ACTIVE), dpar)
+    assert np.all(ubar[~f.fluid] == 0.0)
+    _dot_check(float(np.sum(w * jv)), [ubar * du, pg * dpar], f"l_vjp {which}")
+    s.close()
+
+
+def test_rk3_vjp_dot_product_cylinder(rh3t, workloads):
+    """One RK3 step of the perturbed cylinder: the three stage VJPs, each
+    through the FarField ghosts of its stage state."""
+    f, u, p = _heavy_case(workloads, "farfield")
+    _say("rk3_vjp dot product cylinder 121x481")
     s = f.solver()
-    s.set_state(f.cylinder_state())
+    s.set_state(u)
+    dt = 0.05 * s.calc_global_dt()
+    rng = np.random.default_rng(61)
+    w = rng.standard_normal(u.shape)
+    w[~f.fluid] = 0.0
+    du = workloads.state_direction(u, f.fluid, 62)
+    dpar = rng.standard_normal(5) * PSCALE
+    ubar, pg = s.rk3_vjp(dt, w)
+    _, jv = ob.rk3_jvp(p, u, du, [dt], ob.physics(), dpar)
+    _dot_check(float(np.sum(w * jv)), [ubar * du, pg * dpar], "rk3_vjp cylinder")
+    s.close()
+
+
+def test_wall_ghosts_are_rejected(rh3t, workloads):
+    """The reverse mode does not cover Wall (ILW) ghosts: the double Mach
+    problem must fail loudly rather than return a partial gradient."""
+    _say("adjoint on the double Mach reflection: unsupported")
+    f = rh3t.Field.init_double_mach(1.0 / 40.0)
+    s = f.solver()
+    s.set_state(f.double_mach_state())
     with pytest.raises(rh3t.RH3TError) as ei:
         s.l_vjp(np.zeros((f.grid.nx, f.grid.ny, 6)))
     assert ei.value.status == rh3t.ERR_UNSUPPORTED
