@@ -27,11 +27,11 @@
 // (src/constant.rs:17-25); their cotangents are formed whether or not the
 // forward skips the zero-valued terms.  The time step is not differentiated
 // (dt is frozen).
-//   k_adj_heavy      per FarField / Outflow ghost: the pullback of its
-//                    value (src/bc1.rs:193-255, 760-928) to the states of
-//                    its E_r points and its nearest point
+//   k_adj_heavy      per FarField / Outflow / Wall ghost: the pullback of
+//                    its value (src/bc1.rs:193-255, 691-928) to the states
+//                    of its E_r points and its nearest point
 //   k_adj_heavy_T    adds those into the cells (fixed order)
-// Wall (ILW) ghosts are not covered.
+// TimeDependent ghosts are not covered.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(256) void k_adj_ghost_T(int nrows, const long* __re
   for (int f = 0; f < 6; ++f) OUT[f * P + c] = acc[f];
 }
 
-// ---------------------------------------- heavy ghosts (FarField / Outflow)
+// --------------------------------- heavy ghosts (FarField / Outflow / Wall)
 
 // E_r geometry of the heavy-ghost tables (GhostBC, src/bc1.rs:193-255): the
 // sub-stencil r holds (r+1)^2 points from slot sum_{i<r} (i+1)^2; S_r row a
@@ -625,6 +625,7 @@ struct HeavyLds {
   double wV[5][6], wR[6][6], wL[6][6], wom[5][6];
   double wv[5][5][6], wbeta[5][6];
   double wd[6][56];
+  double wLw[6][6];      // Wall: cotangents of the rows of L through the ILW systems
 };
 
 // rhs_1[0] = l1 / lambda1 of the Outflow relaxation (src/bc1.rs:783-787)
@@ -670,9 +671,9 @@ __device__ bool outflow_relax(const Phys& ph, const double uk[6], double p_inf, 
   return true;
 }
 
-// Pullback of one FarField / Outflow ghost value (weno_extrapolation_pre and
-// the BC branches, src/bc1.rs:193-255, 294-312, 760-928) to the states it
-// reads: its 55 E_r points and its nearest point.  One 64-lane workgroup per
+// Pullback of one FarField / Outflow / Wall ghost value
+// (weno_extrapolation_pre and the BC branches, src/bc1.rs:193-255, 294-312,
+// 691-928) to the states it reads: its 55 E_r points and its nearest point.  One 64-lane workgroup per
 // ghost recomputes the forward quantities from the stage state with the
 // forward kernel's own operations (so the derivative is taken at the very
 // values the forward formed: the WENO weights d_r / (eps + beta_r)^q make
@@ -682,7 +683,9 @@ __device__ bool outflow_relax(const Phys& ph, const double uk[6], double p_inf, 
 // identity, so the eigensystem pullback chains to the state through
 // derived_vjp.  The incoming-characteristic choice of
 // FarField and the Outflow branch take the derivative of the branch the
-// forward value takes.  GB: 56 x 6 per ghost (the E_r points a = 0..54, then
+// forward value takes; Wall's two ILW systems (x = A^-1 b) pull back as
+// bbar = A^-T xbar, Abar = -bbar x^T, the ILW row's entries being functions
+// of u^(0).  GB: 56 x 6 per ghost (the E_r points a = 0..54, then
 // the nearest point), in the (x, y) frame.
 __global__ __launch_bounds__(64) void k_adj_heavy(AdjDesc A, HeavyGhosts hg, const EigConst* __restrict__ Kp,
                                                   Phys ph, double* __restrict__ GB) {
@@ -692,7 +695,7 @@ __global__ __launch_bounds__(64) void k_adj_heavy(AdjDesc A, HeavyGhosts hg, con
   const double* __restrict__ S = A.S;
   const long row = hg.tab[h];
   const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
-  const bool far = hg.kind[h] == RH3T_BC_FARFIELD;
+  const bool far = hg.kind[h] == RH3T_BC_FARFIELD, wall = hg.kind[h] == RH3T_BC_WALL;
   const double* Sh = hg.S + 979L * row;
   const double* Wh = hg.W + 225L * row;
   double s1[6];
@@ -800,31 +803,7 @@ __global__ __launch_bounds__(64) void k_adj_heavy(AdjDesc A, HeavyGhosts hg, con
   }
   __syncthreads();
   if (lane == 0) {
-    // forward: rhs_k of the BC branch
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-#pragma unroll
-      for (int m = 0; m < 6; ++m) T.rhs[k][m] = T.V[k][m];
-    bool fix = false;
-    double gphi[6];
-    if (far) {
-#pragma unroll
-      for (int m = 0; m < 6; ++m)
-        if (T.lam[m] < 0.0) {
-          T.rhs[0][m] = T.d[m][55];
-#pragma unroll
-          for (int k = 1; k < 5; ++k) T.rhs[k][m] = 0.0;
-        }
-    } else {
-      double u0[6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) u0[i] = dot6(T.R[i], T.V[0]);
-      double phi = 0.0;
-      fix = outflow_relax(ph, u0, hg.outflow[3L * h], hg.outflow[3L * h + 1], hg.outflow[3L * h + 2], &phi, gphi);
-      if (fix) T.rhs[1][0] = phi;
-    }
-    // reverse: the ghost cotangent in the (n, t) frame, through the Taylor
-    // sum to u^(k) = R rhs_k
+    // the ghost cotangent in the (n, t) frame and the Taylor weights
     const long pad = hg.pad[h];
     const double w1 = A.SB[1 * P + pad], w2 = A.SB[2 * P + pad];
     const double wug[6] = {A.SB[pad], w1 * nx + w2 * ny, -w1 * ny + w2 * nx, A.SB[3 * P + pad],
@@ -838,38 +817,147 @@ __global__ __launch_bounds__(64) void k_adj_heavy(AdjDesc A, HeavyGhosts hg, con
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
-      for (int m = 0; m < 6; ++m) wR[i][m] = 0.0;
-    double wu0[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) wu0[i] = wug[i];
-    for (int k = 4; k >= 0; --k) {
-      double wuk[6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) wuk[i] = k == 0 ? wu0[i] : coef[k] * wug[i];
-#pragma unroll
       for (int m = 0; m < 6; ++m) {
-        double acc = 0.0;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          acc += T.R[i][m] * wuk[i];
-          wR[i][m] += wuk[i] * T.rhs[k][m];
-        }
-        wrhs[k][m] = acc;
+        wR[i][m] = 0.0;
+        T.wLw[i][m] = 0.0;
       }
-      if (k == 1 && fix) {  // rhs_1[0] = phi(u^(0)), u^(0) = R V^(0)
-#pragma unroll
-        for (int i = 0; i < 6; ++i) wu0[i] += gphi[i] * wrhs[1][0];
-        wrhs[1][0] = 0.0;
-      }
-    }
 #pragma unroll
     for (int c = 0; c < 6; ++c) T.wd[c][55] = 0.0;
+    if (wall) {
+      // forward, src/bc1.rs:691-756: u^(0) from L with row 0 replaced by
+      // the zero normal momentum, u^(1) from L with row 0 the ILW row of
+      // u^(0) (build_ilw_row1, src/bc1.rs:448-465), both right-hand sides
+      // V^(k) with entry 0 set to 0; u^(k) = R V^(k) for k >= 2
+      double A0[6][6], A1[6][6], b[6], u0[6], u1[6];
 #pragma unroll
-    for (int m = 0; m < 6; ++m) {
-      const bool in = far && T.lam[m] < 0.0;
-      if (in) T.wd[m][55] = wrhs[0][m];
+      for (int r = 0; r < 6; ++r)
 #pragma unroll
-      for (int k = 0; k < 5; ++k) T.wV[k][m] = in ? 0.0 : wrhs[k][m];
+        for (int q = 0; q < 6; ++q) A0[r][q] = A1[r][q] = T.L[r][q];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) A0[0][q] = q == 1 ? 1.0 : 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) b[m] = m == 0 ? 0.0 : T.V[0][m];
+      bool ok = solve6(A0, b, u0);
+      const double rho = u0[0], mt = u0[2];
+      A1[0][0] = -ph.gt * sq(mt) / (6.0 * sq(rho));
+      A1[0][1] = 0.0;
+      A1[0][2] = ph.gt * mt / (6.0 * rho);
+      A1[0][3] = -ph.ge;
+      A1[0][4] = -ph.gi;
+      A1[0][5] = -ph.gr;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) b[m] = m == 0 ? 0.0 : T.V[1][m];
+      ok = ok && solve6(A1, b, u1);
+      // reverse: k >= 2 through R
+#pragma unroll
+      for (int k = 2; k < 5; ++k)
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          double acc = 0.0;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) {
+            acc += T.R[i][m] * coef[k] * wug[i];
+            wR[i][m] += coef[k] * wug[i] * T.V[k][m];
+          }
+          wrhs[k][m] = acc;
+        }
+      // x = A^-1 b: bbar = A^-T xbar, Abar = -bbar x^T
+      double At[6][6], xb[6], bb[6];
+      double wu0[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) xb[i] = coef[1] * wug[i];
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) At[r][q] = A1[q][r];
+      ok = ok && solve6(At, xb, bb);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) wrhs[1][m] = m == 0 ? 0.0 : bb[m];
+#pragma unroll
+      for (int r = 1; r < 6; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) T.wLw[r][q] -= bb[r] * u1[q];
+      {  // row 0 of A1 as a function of u^(0)
+        const double a00 = -bb[0] * u1[0], a02 = -bb[0] * u1[2];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) wu0[i] = wug[i];
+        wu0[0] += a00 * ph.gt * sq(mt) / (3.0 * rho * rho * rho) - a02 * ph.gt * mt / (6.0 * rho * rho);
+        wu0[2] += -a00 * ph.gt * mt / (3.0 * rho * rho) + a02 * ph.gt / (6.0 * rho);
+      }
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) At[r][q] = A0[q][r];
+      ok = ok && solve6(At, wu0, bb);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) wrhs[0][m] = m == 0 ? 0.0 : bb[m];
+#pragma unroll
+      for (int r = 1; r < 6; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) T.wLw[r][q] -= bb[r] * u0[q];
+      if (!ok)  // singular ILW system: the forward raised RH3T_ERR_SINGULAR
+#pragma unroll
+        for (int m = 0; m < 6; ++m) wrhs[0][m] = NAN;
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int m = 0; m < 6; ++m) T.wV[k][m] = wrhs[k][m];
+    } else {
+      // forward: rhs_k of the BC branch
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int m = 0; m < 6; ++m) T.rhs[k][m] = T.V[k][m];
+      bool fix = false;
+      double gphi[6];
+      if (far) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m)
+          if (T.lam[m] < 0.0) {
+            T.rhs[0][m] = T.d[m][55];
+#pragma unroll
+            for (int k = 1; k < 5; ++k) T.rhs[k][m] = 0.0;
+          }
+      } else {
+        double u0[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) u0[i] = dot6(T.R[i], T.V[0]);
+        double phi = 0.0;
+        fix = outflow_relax(ph, u0, hg.outflow[3L * h], hg.outflow[3L * h + 1], hg.outflow[3L * h + 2], &phi,
+                            gphi);
+        if (fix) T.rhs[1][0] = phi;
+      }
+      // reverse through u^(k) = R rhs_k
+      double wu0[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) wu0[i] = wug[i];
+      for (int k = 4; k >= 0; --k) {
+        double wuk[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) wuk[i] = k == 0 ? wu0[i] : coef[k] * wug[i];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          double acc = 0.0;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) {
+            acc += T.R[i][m] * wuk[i];
+            wR[i][m] += wuk[i] * T.rhs[k][m];
+          }
+          wrhs[k][m] = acc;
+        }
+        if (k == 1 && fix) {  // rhs_1[0] = phi(u^(0)), u^(0) = R V^(0)
+#pragma unroll
+          for (int i = 0; i < 6; ++i) wu0[i] += gphi[i] * wrhs[1][0];
+          wrhs[1][0] = 0.0;
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        const bool in = far && T.lam[m] < 0.0;
+        if (in) T.wd[m][55] = wrhs[0][m];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) T.wV[k][m] = in ? 0.0 : wrhs[k][m];
+      }
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i)
@@ -930,7 +1018,7 @@ __global__ __launch_bounds__(64) void k_adj_heavy(AdjDesc A, HeavyGhosts hg, con
   }
   if (lane < 36) {
     const int c = lane / 6, f = lane % 6;
-    double acc = 0.0;
+    double acc = T.wLw[c][f];
     for (int a = 0; a < 56; ++a) acc += T.wd[c][a] * T.u[a][f];
     T.wL[c][f] = acc;
   }
@@ -1105,10 +1193,6 @@ int init_adjoint(Engine& E) {
     std::snprintf(E.message, sizeof(E.message), "the adjoint covers single solves");
     return RH3T_ERR_UNSUPPORTED;
   }
-  if (E.heavy.n > E.heavy.n_open) {
-    std::snprintf(E.message, sizeof(E.message), "the adjoint does not cover Wall (ILW) ghosts");
-    return RH3T_ERR_UNSUPPORTED;
-  }
   HIP_TRY(hipSetDevice(E.device));
   const SolveDesc& d = E.h_desc[0];
   const Geo& g = d.g;
@@ -1120,7 +1204,7 @@ int init_adjoint(Engine& E) {
   // heavy-ghost transpose lists grouped by cell: (cell slot, ghost * 56 +
   // point), the points being the 55 E_r points and the nearest point
   std::vector<std::pair<long, int>> hents;
-  for (int h = 0; h < E.heavy.n_open; ++h)
+  for (int h = 0; h < E.heavy.n; ++h)
     for (int a = 0; a < 56; ++a) {
       const long q = a < 55 ? E.h_heavy_stencil[55L * E.h_heavy_tab[h] + a] : E.h_heavy_near[h];
       if (!is_fluid_slot(q)) {
@@ -1193,7 +1277,7 @@ int init_adjoint(Engine& E) {
   if ((st = dev_alloc(E, &W->YB, 36 * (size_t)g.nx * (g.ny + 1)))) return st;
   W->pg_blocks = ((g.nx + kGy - 1) / kGy) * ((g.ny + kGx - 1) / kGx);
   std::vector<long> out_pads;
-  for (int k = 0; k < E.cheap.n + E.heavy.n_open; ++k) {
+  for (int k = 0; k < E.cheap.n + E.heavy.n; ++k) {
     const long p = k < E.cheap.n ? E.h_cheap_pad[k] : E.h_heavy_pad[k - E.cheap.n];
     const long ip = p / g.PY, jp = p % g.PY;
     if (ip < 4 || ip >= g.nx + 4 || jp < 4 || jp >= g.ny + 4) out_pads.push_back(p);
@@ -1209,7 +1293,7 @@ int init_adjoint(Engine& E) {
   if ((st = upload(E, &W->ent_pad, ent_pad))) return st;
   if ((st = upload(E, &W->ent_kind, ent_kind))) return st;
   if ((st = upload(E, &W->ent_n, ent_n))) return st;
-  W->n_heavy = E.heavy.n_open;
+  W->n_heavy = E.heavy.n;
   if (W->n_heavy > 0) {
     if ((st = dev_alloc(E, &W->GB, (size_t)W->n_heavy * 56 * 6))) return st;
     W->hrows = (int)hrow_cell.size();

@@ -399,6 +399,60 @@ __device__ __forceinline__ double weno_pow(double x, double q) {
   return rh + rl;
 }
 
+// solve6 (LAPACK dgesv): LU with partial pivoting, first maximal pivot.
+// Fully unrolled: the row interchange is a predicated exchange with
+// compile-time row indices, so the matrix stays in registers.
+__device__ __forceinline__ bool solve6(const double A0[6][6], const double b0[6], double x[6]) {
+  double A[6][6], b[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    b[i] = b0[i];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) A[i][j] = A0[i][j];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    int p = c;
+    double best = fabs(A[c][c]);
+#pragma unroll
+    for (int r = c + 1; r < 6; ++r)
+      if (fabs(A[r][c]) > best) {
+        best = fabs(A[r][c]);
+        p = r;
+      }
+    if (best == 0.0) return false;
+#pragma unroll
+    for (int r = c + 1; r < 6; ++r)
+      if (r == p) {
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          double t = A[c][q];
+          A[c][q] = A[r][q];
+          A[r][q] = t;
+        }
+        double t = b[c];
+        b[c] = b[r];
+        b[r] = t;
+      }
+#pragma unroll
+    for (int r = c + 1; r < 6; ++r) {
+      double f = A[r][c] / A[c][c];
+      A[r][c] = f;
+#pragma unroll
+      for (int q = c + 1; q < 6; ++q) A[r][q] -= f * A[c][q];
+      b[r] -= f * b[c];
+    }
+  }
+#pragma unroll
+  for (int r = 5; r >= 0; --r) {
+    double s = b[r];
+#pragma unroll
+    for (int q = r + 1; q < 6; ++q) s -= A[r][q] * x[q];
+    x[r] = s / A[r][r];
+  }
+  return true;
+}
+
 // assert_admissible, src/main.rs:22-33
 __device__ __forceinline__ bool admissible(const double s[6]) {
 #pragma unroll

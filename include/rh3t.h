@@ -369,9 +369,8 @@ int rh3t_batch_profile_read(rh3t_batch* b, int32_t max_kernels, char* names,
 /*   pgrad[5] = (omega_ei, omega_er, kappa_e, kappa_i, kappa_r)             */
 /* (src/constant.rs:17-25), and a checkpoint-and-recompute adjoint of a     */
 /* sequence of RK3 steps with a frozen dt schedule.  Covered ghost kinds:  */
-/* Periodic, ReflectiveWall, ZerothOrder, Constant, FarField, Outflow      */
-/* (Wall and TimeDependent return RH3T_ERR_UNSUPPORTED); single-solve      */
-/* contexts only.  Cotangent arrays                                         */
+/* every kind but TimeDependent (which returns RH3T_ERR_UNSUPPORTED);     */
+/* single-solve contexts only.  Cotangent arrays                            */
 /* use the state's AoS layout; entries of non-fluid cells are ignored on   */
 /* input and zero on output.  pgrad may be NULL.                            */
 /* ------------------------------------------------------------------------ */

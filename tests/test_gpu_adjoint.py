@@ -7,8 +7,8 @@ Checks, in increasing scope:
     <w, J v> (oracle, forward mode) == <J^T w, v> (GPU, reverse mode) for
     seeded state + parameter directions v and cotangents w, on periodic and
     on walled (ReflectiveWall / Constant / ZerothOrder) boxes, through the
-    FarField ghosts of the cylinder and the Outflow ghosts of the
-    translating shock;
+    FarField ghosts of the cylinder, the Outflow ghosts of the translating
+    shock and the Wall (ILW) ghosts of the double Mach reflection;
   * the checkpoint schedule does not change the gradient (bitwise);
   * the differentiation workload (512x512 smooth-mode box, exchange on,
     20 steps, checkpoints every 5, J = sum of a Gaussian window times the
@@ -236,11 +236,36 @@ def test_rk3_vjp_dot_product_cylinder(rh3t, workloads):
     s.close()
 
 
-def test_wall_ghosts_are_rejected(rh3t, workloads):
-    """The reverse mode does not cover Wall (ILW) ghosts: the double Mach
-    problem must fail loudly rather than return a partial gradient."""
-    _say("adjoint on the double Mach reflection: unsupported")
+def test_l_vjp_dot_product_wall_ghosts(rh3t, workloads):
+    """The VJP of l() through the Wall (ILW) ghosts of the double Mach
+    reflection at h = 1/40 (its two ILW linear systems per ghost), with the
+    Constant / ZerothOrder sides around them; initial state perturbed by 1%
+    of a smooth direction (off the ties of the piecewise-constant state)."""
     f = rh3t.Field.init_double_mach(1.0 / 40.0)
+    u0 = f.double_mach_state()
+    u = u0 + 0.01 * workloads.smooth_direction(f, u0, 9)
+    u[~f.fluid] = u0[~f.fluid]
+    p = ob.OracleProblem(f)
+    _say(f"l_vjp dot product through wall ghosts {f.grid.nx}x{f.grid.ny}")
+    s = f.solver(physics=rh3t.Physics(**KW_ACTIVE))
+    s.set_state(u)
+    rng = np.random.default_rng(71)
+    w = rng.standard_normal(u.shape)
+    w[~f.fluid] = 0.0
+    du = workloads.state_direction(u, f.fluid, 72)
+    dpar = rng.standard_normal(5) * PSCALE
+    ubar, pg = s.l_vjp(w)
+    _, jv = ob.l_jvp(p, u, du, ob.physics(**KW_ACTIVE), dpar)
+    _dot_check(float(np.sum(w * jv)), [ubar * du, pg * dpar], "l_vjp wall")
+    s.close()
+
+
+def test_time_dependent_ghosts_are_rejected(rh3t, workloads):
+    """The reverse mode does not cover TimeDependent ghosts: the double Mach
+    problem with the moving-shock top must fail loudly rather than return a
+    partial gradient."""
+    _say("adjoint on the double Mach reflection with a moving-shock top: unsupported")
+    f = rh3t.Field.init_double_mach(1.0 / 40.0, top=workloads.double_mach_moving_shock())
     s = f.solver()
     s.set_state(f.double_mach_state())
     with pytest.raises(rh3t.RH3TError) as ei:
