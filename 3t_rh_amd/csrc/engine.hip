@@ -35,7 +35,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "engine.hpp"
@@ -429,12 +431,51 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
     raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
 }
 
+// One frame per lane: build_local_plain (src/bc1.rs:294-312), the nearest
+// state rotated to the (n, t) frame and the X-direction eigensystem of its
+// Roe average with itself -- the rows of L and R and the eigenvalues, the
+// values k_ghost_open's ghosts of this frame apply.
+__global__ __launch_bounds__(64) void k_ghost_frames(const SolveDesc* __restrict__ descs, GhostFrames fr, Phys ph,
+                                                     int buf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= fr.n) return;
+  const SolveDesc& sd = descs[fr.sid[k]];
+  const double* S = sd.U[buf];
+  const long plane = sd.g.plane;
+  const double nx = fr.normal[2L * k], ny = fr.normal[2L * k + 1];
+  const long q = fr.nearest[k];
+  double val[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) val[f] = S[f * plane + q];
+  double s1[6] = {val[0], val[1] * nx + val[2] * ny, -val[1] * ny + val[2] * nx, val[3], val[4], val[5]};
+  double d1[D_N];
+  derived_from_state(ph, s1, d1, false);
+  RoeX Q;
+  roe_x(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI],
+        d1[D_ER], Q);
+  double* o = fr.out + (long)kFrameLen * k;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double row[6];
+    lx_row(ph, Q, c, row);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) o[6 * c + m] = row[m];
+  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int m = 0; m < 6; ++m) o[36 + 6 * a + m] = rx_entry(ph, Q, a, m);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) o[72 + c] = lamx(Q, c);
+}
+
 // FarField / Outflow ghosts, kOpenPerWave of them per 64-lane wavefront:
 // lane group g = lane / LG (LG = 64 / kOpenPerWave lanes) takes one ghost.
-// What is uniform over a ghost -- the nearest state's Roe average, the rows
-// of L and R, the Outflow relaxation, the Taylor sum -- is evaluated once
-// per lane group instead of once per 64-lane wavefront, so the wavefront's
-// instructions serve kOpenPerWave ghosts at a time.  The 70 table rows of a
+// The eigensystem of the ghost's frame comes from k_ghost_frames (formed
+// once per nearest point and normal, about one in four ghosts); what else is
+// uniform over a ghost -- the Outflow relaxation, the Taylor sum -- is
+// evaluated once per lane group, so the wavefront's instructions serve
+// kOpenPerWave ghosts at a time.  The 70 table rows of a
 // ghost (55 rows of S_r, the 15 rows of w_r) run on the group's 32 lanes in
 // two segments of 25 and 16 steps (41 instead of the 75 that three rounds
 // of 25-entry rows would take), from the row block swizzled on the host so
@@ -443,13 +484,13 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
 // 760-928), so the values are the same as one ghost per wavefront.
 constexpr int kOpenPerWave = 2;
 constexpr int kOpenLG = 64 / kOpenPerWave;
-constexpr int kOpenLds = 336 + 150 + 30 + 30 + 20;  // doubles per ghost
+constexpr int kOpenLds = 336 + 150 + 30 + 30 + kFrameLen;  // doubles per ghost
 static_assert(kOpenLG == 32, "the swizzled table schedule is for 32-lane groups");
 // the (k, r) with k > r, whose v_k,r are zero
 __constant__ int kZeroK[10] = {1, 2, 3, 4, 2, 3, 4, 3, 4, 4};
 __constant__ int kZeroR[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ghost_open(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ghost_open(
     const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
   constexpr int LG = kOpenLG;
   const int lane = threadIdx.x;
@@ -459,14 +500,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   // per-ghost LDS (kOpenLds doubles): data[6][56] (L . u at the E_r points,
   // then, once the table rows have been applied, the products data * (S_r
   // data) in the same slots; slot 55 keeps L . u_inf), vk[5][6][5],
-  // alpha[6][5], v[5][6], the Roe scalars
+  // alpha[6][5], v[5][6], the frame (rows of L, rows of R, eigenvalues)
   extern __shared__ double s_dyn[];
   double* const s_data = s_dyn + ((long)w * kOpenPerWave + grp) * kOpenLds;
   double* const s_part = s_data;
   double* const s_vk = s_data + 336;
   double* const s_alpha = s_vk + 150;
   double* const s_v = s_alpha + 30;
-  RoeX* const s_q = (RoeX*)(s_v + 30);
+  double* const s_F = s_v + 30;
   const bool active = hl < count;
   const int h = h0 + (active ? hl : 0);
   const long row = hg.tab[h];
@@ -475,20 +516,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const long plane = sd.g.plane;
   const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
   const bool far = hg.kind[h] == RH3T_BC_FARFIELD;
-  // build_local_plain, src/bc1.rs:294-312: the nearest state in the (n, t)
-  // frame, X-direction eigenvectors of its Roe average with itself
-  RoeX Q;
-  {
-    double val[6];
-    const long q = hg.nearest[h];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) val[f] = S[f * plane + q];
-    double s1[6] = {val[0], val[1] * nx + val[2] * ny, -val[1] * ny + val[2] * nx, val[3], val[4], val[5]};
-    double d1[D_N];
-    derived_from_state(ph, s1, d1, false);
-    roe_x(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI],
-          d1[D_ER], Q);
-  }
+  // The ghost's loads are issued together: the state gathers at its E_r
+  // points (behind the stencil slots of its table row), its frame's
+  // eigensystem (k_ghost_frames) and the table entries of the first
+  // segment, so that their latencies overlap.
+  const double* T = hg.T + (long)kSwzLen * row + l;
+  double tvA[kSwzA];
   // L . u at the E_r points (a < 55) and, for a FarField ghost, L . u_inf
   // (src/bc1.rs:829-924) in the free slot 55: point a = l + p LG, p = 0, 1
   {
@@ -517,16 +550,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       u6[p][4] = s6[4];
       u6[p][5] = s6[5];
     }
+    if (active) {
+      const double* Fg = hg.F + (long)kFrameLen * hg.frame[h];
+      double fv[(kFrameLen + LG - 1) / LG];
+#pragma unroll
+      for (int q = 0; q < (kFrameLen + LG - 1) / LG; ++q)
+        if (l + q * LG < kFrameLen) fv[q] = Fg[l + q * LG];
+#pragma unroll
+      for (int b = 0; b < kSwzA; ++b) tvA[b] = T[32 * b];
+#pragma unroll
+      for (int q = 0; q < (kFrameLen + LG - 1) / LG; ++q)
+        if (l + q * LG < kFrameLen) s_F[l + q * LG] = fv[q];
+    }
+    __syncthreads();
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       double lr[6];
-      lx_row(ph, Q, c, lr);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) lr[m] = s_F[6 * c + m];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
         if (has[p]) s_data[c * 56 + l + p * LG] = dot6(lr, u6[p]);
     }
   }
-  if (active && l == 0) *s_q = Q;
   __syncthreads();
   if (active) {
     // The 70 table rows on a fixed two-segment schedule (kSwz* in
@@ -537,18 +583,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // data)[a] goes straight into data slot a once no lane reads that E_r
     // segment any more (the group is one wavefront: LDS accesses complete in
     // program order), a w row's sum into vk.
-    const double* T = hg.T + (long)kSwzLen * row + l;
     {  // segment A: S_4 rows (lanes 0..24), w_4 rows (25..29), S_0 / w_0 (30, 31)
       const int m = l < 30 ? 25 : 1, d0 = l < 30 ? 30 : 0;
-      double tv[kSwzA];
-#pragma unroll
-      for (int b = 0; b < kSwzA; ++b) tv[b] = T[32 * b];
       double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int b = 0; b < kSwzA; ++b)
         if (b < m)
 #pragma unroll
-          for (int c = 0; c < 6; ++c) acc[c] += tv[b] * s_data[c * 56 + d0 + b];
+          for (int c = 0; c < 6; ++c) acc[c] += tvA[b] * s_data[c * 56 + d0 + b];
       if (l < 25 || l == 30) {
         const int a = l < 25 ? 30 + l : 0;
 #pragma unroll
@@ -630,7 +672,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     }
   }
   __syncthreads();
-  const RoeX Qs = *s_q;
   const double dist = hg.dist[h];
   // FarField, src/bc1.rs:829-924: incoming characteristics of u^(0) from
   // u_inf, of u^(k >= 1) zero.  Outflow, src/bc1.rs:760-828: the u - c
@@ -640,9 +681,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   double* const s_fix = s_vk + 42;  // Outflow: flag, rhs_1[0]
   if (active && l < 6 && !far) {    // Outflow: u^(0) = R V^(0)
     double r[6], x[6];
-    rx_row_rt(ph, Qs, l, r);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) x[c] = s_v[c];
+    for (int c = 0; c < 6; ++c) {
+      r[c] = s_F[36 + 6 * l + c];
+      x[c] = s_v[c];
+    }
     s_uk[l] = dot6(r, x);
   }
   __syncthreads();
@@ -678,11 +721,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       if (far) {
 #pragma unroll
         for (int m = 0; m < 6; ++m)
-          if (lamx(Qs, m) < 0.0) rhs[m] = k == 0 ? s_data[m * 56 + 55] : 0.0;
+          if (s_F[72 + m] < 0.0) rhs[m] = k == 0 ? s_data[m * 56 + 55] : 0.0;
       } else if (k == 1 && s_fix[0] != 0.0) {
         rhs[0] = s_fix[1];
       }
-      rx_row_rt(ph, Qs, i, r);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) r[m] = s_F[36 + 6 * i + m];
       s_uk[k * 6 + i] = dot6(r, rhs);
     }
   }
@@ -1142,6 +1186,7 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, int gt, bool capture) {
     const int g = E.ghosts_per_block, n_open = E.heavy.n_open, n_wall = E.heavy.n - n_open;
     const size_t lds = (size_t)g * kGhostLds * sizeof(double);
     if (n_open > 0) {
+      k_ghost_frames<<<(E.frames.n + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.frames, E.phys, buf);
       const int per = g * kOpenPerWave;  // ghosts per workgroup
       k_ghost_open<<<(n_open + per - 1) / per, dim3(64, g), (size_t)per * kOpenLds * sizeof(double), E.stream>>>(
           E.d_desc, E.heavy, E.phys, buf, phase, 0, n_open);
@@ -1586,6 +1631,29 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
     permute(h_state, 6);
     permute(h_of, 3);
   }
+  // FarField / Outflow ghosts of one solve that share a nearest point and a
+  // normal share a frame
+  std::vector<int> h_frame(E.heavy.n, 0), f_sid;
+  std::vector<long> f_near;
+  std::vector<double> f_n;
+  {
+    std::map<std::tuple<int, long, uint64_t, uint64_t>, int> fid;
+    for (int k = 0; k < E.heavy.n_open; ++k) {
+      uint64_t bx, by;
+      std::memcpy(&bx, &h_n[2 * (size_t)k], sizeof(bx));
+      std::memcpy(&by, &h_n[2 * (size_t)k + 1], sizeof(by));
+      const auto key = std::make_tuple(h_sid[k], h_near[k], bx, by);
+      auto it = fid.find(key);
+      if (it == fid.end()) {
+        it = fid.emplace(key, (int)f_sid.size()).first;
+        f_sid.push_back(h_sid[k]);
+        f_near.push_back(h_near[k]);
+        f_n.push_back(h_n[2 * (size_t)k]);
+        f_n.push_back(h_n[2 * (size_t)k + 1]);
+      }
+      h_frame[k] = it->second;
+    }
+  }
   E.h_heavy_kind = h_kind;
   E.h_heavy_tab = h_tab;
   E.h_heavy_pad = h_pad;
@@ -1628,6 +1696,13 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   UP(h_S, dp) E.heavy.S = dp;
   UP(h_W, dp) E.heavy.W = dp;
   UP(h_T, dp) E.heavy.T = dp;
+  UP(h_frame, ip) E.heavy.frame = ip;
+  E.frames.n = (int)f_sid.size();
+  UP(f_sid, ip) E.frames.sid = ip;
+  UP(f_near, lp) E.frames.nearest = lp;
+  UP(f_n, dp) E.frames.normal = dp;
+  if ((st = dev_alloc(E, &E.frames.out, (size_t)kFrameLen * (f_sid.empty() ? 1 : f_sid.size())))) return st;
+  E.heavy.F = E.frames.out;
 #undef UP
   if ((st = build_tables(E))) return st;
   return RH3T_OK;

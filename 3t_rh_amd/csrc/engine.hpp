@@ -71,6 +71,21 @@ struct HeavyGhosts {
   const double* S;       // 979 per table row
   const double* W;       // 225 per table row
   const double* T;       // kSwzLen per table row: S and W swizzled for k_ghost_open
+  const int* frame;      // FarField / Outflow: the ghost's frame (GhostFrames)
+  const double* F;       // kFrameLen per frame, formed by k_ghost_frames each update
+};
+
+// The frames of the FarField / Outflow ghosts: ghosts of one solve that
+// share a nearest point and a normal share the eigensystem of
+// build_local_plain (src/bc1.rs:294-312), which k_ghost_frames forms once
+// per frame: rows of L (36), rows of R (36), eigenvalues (6).
+constexpr int kFrameLen = 78;
+struct GhostFrames {
+  int n;
+  const int* sid;
+  const long* nearest;   // slot of the nearest point
+  const double* normal;  // 2 per frame
+  double* out;           // kFrameLen per frame
 };
 
 // The swizzled row block of a heavy-ghost table (FarField / Outflow kernel):
@@ -127,6 +142,7 @@ struct Engine {
   Scalars* d_sc = nullptr;
   CheapGhosts cheap{};
   HeavyGhosts heavy{};
+  GhostFrames frames{};
   // host copies of the cheap-ghost lists (the adjoint transposes them)
   std::vector<int> h_cheap_sid, h_cheap_kind;
   std::vector<long> h_cheap_pad, h_cheap_src;
