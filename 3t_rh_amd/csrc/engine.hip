@@ -484,11 +484,10 @@ __global__ __launch_bounds__(64) void k_ghost_frames(const SolveDesc* __restrict
 // 760-928), so the values are the same as one ghost per wavefront.
 constexpr int kOpenPerWave = 2;
 constexpr int kOpenLG = 64 / kOpenPerWave;
-constexpr int kOpenLds = 336 + 150 + 30 + 30 + kFrameLen;  // doubles per ghost
+constexpr int kOpenLds = 336 + 90 + kFrameLen;  // doubles per ghost
 static_assert(kOpenLG == 32, "the swizzled table schedule is for 32-lane groups");
-// the (k, r) with k > r, whose v_k,r are zero
-__constant__ int kZeroK[10] = {1, 2, 3, 4, 2, 3, 4, 3, 4, 4};
-__constant__ int kZeroR[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
+// slot of v_k,r (k <= r; v_k,r = 0 for k > r is never stored)
+__device__ __forceinline__ int vk_slot(int k, int r) { return r * (r + 1) / 2 + k; }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ghost_open(
     const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
@@ -499,15 +498,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int hl = (xcd_tile(blockIdx.x, gridDim.x) * blockDim.y + w) * kOpenPerWave + grp;
   // per-ghost LDS (kOpenLds doubles): data[6][56] (L . u at the E_r points,
   // then, once the table rows have been applied, the products data * (S_r
-  // data) in the same slots; slot 55 keeps L . u_inf), vk[5][6][5],
-  // alpha[6][5], v[5][6], the frame (rows of L, rows of R, eigenvalues)
+  // data) in the same slots; slot 55 keeps L . u_inf), vk[15][6] (v_k,r for
+  // k <= r), the frame (rows of L, rows of R, eigenvalues).  Once read, the
+  // L rows hold alpha[6][5], and data slots that are no longer read hold the
+  // normalised weights om[6][5] and V^(k) v[5][6] (slot 55 of each row is
+  // kept).  504 doubles per ghost.
   extern __shared__ double s_dyn[];
   double* const s_data = s_dyn + ((long)w * kOpenPerWave + grp) * kOpenLds;
   double* const s_part = s_data;
   double* const s_vk = s_data + 336;
-  double* const s_alpha = s_vk + 150;
-  double* const s_v = s_alpha + 30;
-  double* const s_F = s_v + 30;
+  double* const s_F = s_vk + 90;
+  double* const s_alpha = s_F;
+  double* const s_v = s_data;
+  double* const s_om = s_data + 60;
   const bool active = hl < count;
   const int h = h0 + (active ? hl : 0);
   const long row = hg.tab[h];
@@ -596,9 +599,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
         for (int c = 0; c < 6; ++c) s_data[c * 56 + a] = s_data[c * 56 + a] * acc[c];
       } else {
-        const int k = l < 30 ? l - 25 : 0, r = l < 30 ? 4 : 0;
+        const int q = l < 30 ? vk_slot(l - 25, 4) : vk_slot(0, 0);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) s_vk[(k * 6 + c) * 5 + r] = acc[c];
+        for (int c = 0; c < 6; ++c) s_vk[q * 6 + c] = acc[c];
       }
     }
     {  // segment B: S_3 / w_3 rows (lanes 0..19) over 16 steps; S_2 / w_2
@@ -619,7 +622,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           for (int c = 0; c < 6; ++c) s_data[c * 56 + a] = s_data[c * 56 + a] * acc[c];
         } else {
 #pragma unroll
-          for (int c = 0; c < 6; ++c) s_vk[((l - 29) * 6 + c) * 5 + 2] = acc[c];
+          for (int c = 0; c < 6; ++c) s_vk[vk_slot(l - 29, 2) * 6 + c] = acc[c];
         }
 #pragma unroll
         for (int c = 0; c < 6; ++c) acc[c] = 0.0;
@@ -634,17 +637,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const int a = l < 16 ? 14 + l : 1 + (l - 20);
 #pragma unroll
         for (int c = 0; c < 6; ++c) s_data[c * 56 + a] = s_data[c * 56 + a] * acc[c];
-      } else if (l < 20 || l < 26) {
-        const int k = l < 20 ? l - 16 : l - 24, r = l < 20 ? 3 : 1;
+      } else if (l < 26) {
+        const int q = l < 20 ? vk_slot(l - 16, 3) : vk_slot(l - 24, 1);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) s_vk[(k * 6 + c) * 5 + r] = acc[c];
+        for (int c = 0; c < 6; ++c) s_vk[q * 6 + c] = acc[c];
       }
-    }
-    // v_k = 0 for k > r (weno_extrapolation_pre)
-    for (int z = l; z < 10; z += LG) {
-      const int kz = kZeroK[z], rz = kZeroR[z];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) s_vk[(kz * 6 + c) * 5 + rz] = 0.0;
     }
   }
   __syncthreads();
@@ -659,15 +656,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
   }
   __syncthreads();
-  if (active) {  // V^(k)_c = sum_r (alpha_r / sum alpha) v_k,r, task (k, c)
+  if (active) {  // om_r = alpha_r / sum alpha, task (r, c)
     for (int t = l; t < 30; t += LG) {
-      const int k = t / 6, c = t % 6;
+      const int r = t / 6, c = t % 6;
       double asum = 0.0;
 #pragma unroll
-      for (int r = 0; r < 5; ++r) asum += s_alpha[c * 5 + r];
+      for (int q = 0; q < 5; ++q) asum += s_alpha[c * 5 + q];
+      s_om[c * 5 + r] = s_alpha[c * 5 + r] / asum;
+    }
+  }
+  __syncthreads();
+  if (active) {  // V^(k)_c = sum_r om_r v_k,r, task (k, c); the terms of
+                 // r < k are +0.0 products added to the +0.0 start, which
+                 // leave it unchanged, so the sum starts at r = k
+    for (int t = l; t < 30; t += LG) {
+      const int k = t / 6, c = t % 6;
       double res = 0.0;
 #pragma unroll
-      for (int r = 0; r < 5; ++r) res += (s_alpha[c * 5 + r] / asum) * s_vk[(k * 6 + c) * 5 + r];
+      for (int r = 0; r < 5; ++r)
+        if (r >= k) res += s_om[c * 5 + r] * s_vk[vk_slot(k, r) * 6 + c];
       s_v[k * 6 + c] = res;
     }
   }
