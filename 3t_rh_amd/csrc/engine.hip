@@ -790,33 +790,21 @@ __global__ __launch_bounds__(256) void k_derived(const SolveDesc* __restrict__ d
 #ifndef RH3T_FLUX_WAVES
 #define RH3T_FLUX_WAVES 2
 #endif
+// Flux of interface (i, j) of direction DIR (DIR 0: i - 1/2 at column j,
+// DIR 1: j - 1/2 at row i) from buffer buf; 0.0 when neither neighbour is
+// a fluid cell.
 template <int DIR, bool FAST>
-__global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* __restrict__ descs,
-                                                               const int* __restrict__ tiles, int nsolve,
-                                                               const EigConst* __restrict__ Kp, double eps,
-                                                               int buf) {
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int s = find_solve(tiles, nsolve, t);
-  const SolveDesc& sd = descs[s];
+__device__ __forceinline__ void iface_flux(const SolveDesc& sd, const EigConst* __restrict__ Kp, double eps, int buf,
+                                           int i, int j, double out[6]) {
   const Geo g = sd.g;
-  const int ni = DIR == 0 ? g.nx + 1 : g.nx;
-  const int nj = DIR == 0 ? g.ny : g.ny + 1;
-  const int tiles_j = (nj + blockDim.x - 1) / blockDim.x;
-  const int lt = t - tiles[s];
-  const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
-  const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
-  if (i >= ni || j >= nj) return;
   const double* __restrict__ S = glb(sd.U[buf]);
   const double* __restrict__ D = glb(sd.D);
-  double* __restrict__ F = glb(DIR == 0 ? sd.FX : sd.FY);
   const uint8_t* __restrict__ pt = glb(sd.pt);
-  const long fpl = DIR == 0 ? (long)(g.nx + 1) * g.ny : (long)g.nx * (g.ny + 1);
-  const long fidx = DIR == 0 ? (long)i * g.ny + j : (long)i * (g.ny + 1) + j;
   const long c0 = pad_index(g, i, j);
   const long step = DIR == 0 ? g.PY : 1;
   if (pt[c0] != 1 && pt[c0 - step] != 1) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) F[f * fpl + fidx] = 0.0;
+    for (int f = 0; f < 6; ++f) out[f] = 0.0;
     return;
   }
   // 6-point stencils of U and of the physical flux (flux_from_derived,
@@ -848,9 +836,31 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
   }
   const double e3[3] = {at(D + D_EE * g.plane, 0), at(D + D_EI * g.plane, 0), at(D + D_ER * g.plane, 0)};
   const double e2[3] = {at(D + D_EE * g.plane, -step), at(D + D_EI * g.plane, -step), at(D + D_ER * g.plane, -step)};
-  double out[6];
   interface_flux<DIR, FAST>(Kp, eps, U, Fl, at(D + D_U * g.plane, 0), at(D + D_V * g.plane, 0), e3,
                             at(D + D_U * g.plane, -step), at(D + D_V * g.plane, -step), e2, out);
+}
+
+template <int DIR, bool FAST>
+__global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* __restrict__ descs,
+                                                               const int* __restrict__ tiles, int nsolve,
+                                                               const EigConst* __restrict__ Kp, double eps,
+                                                               int buf) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int s = find_solve(tiles, nsolve, t);
+  const SolveDesc& sd = descs[s];
+  const Geo g = sd.g;
+  const int ni = DIR == 0 ? g.nx + 1 : g.nx;
+  const int nj = DIR == 0 ? g.ny : g.ny + 1;
+  const int tiles_j = (nj + blockDim.x - 1) / blockDim.x;
+  const int lt = t - tiles[s];
+  const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
+  const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
+  if (i >= ni || j >= nj) return;
+  double* __restrict__ F = glb(DIR == 0 ? sd.FX : sd.FY);
+  const long fpl = DIR == 0 ? (long)(g.nx + 1) * g.ny : (long)g.nx * (g.ny + 1);
+  const long fidx = DIR == 0 ? (long)i * g.ny + j : (long)i * (g.ny + 1) + j;
+  double out[6];
+  iface_flux<DIR, FAST>(sd, Kp, eps, buf, i, j, out);
 #pragma unroll
   for (int f = 0; f < 6; ++f) F[f * fpl + fidx] = out[f];
 }
@@ -862,9 +872,11 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
 // (the dt rule of the next step), +inf otherwise.
 // FAST: the noncon divisions as products (noncon_fast); EXACT: the
 // reference's divisions.
-template <bool FAST>
-__device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys& ph, const StageSpec& sa, int i,
-                                                int j) {
+// FYB / FYT: the y fluxes at j - 1/2 and j + 1/2, read from the FY planes
+// (FROM_PLANES) or passed in by the fused y-flux kernel.
+template <bool FAST, bool FROM_PLANES>
+__device__ __forceinline__ double assemble_core(const SolveDesc& sd, const Phys& ph, const StageSpec& sa, int i, int j,
+                                                const double* fyb_in, const double* fyt_in) {
   const Geo g = sd.g;
   const long c0 = pad_index(g, i, j);
   const long lin = (long)i * g.ny + j;
@@ -878,17 +890,26 @@ __device__ __forceinline__ double assemble_cell(const SolveDesc& sd, const Phys&
   }
   const double* __restrict__ D = glb(sd.D);
   const double* __restrict__ FX = glb(sd.FX);
-  const double* __restrict__ FY = glb(sd.FY);
-  const long fxpl = (long)(g.nx + 1) * g.ny, fypl = (long)g.nx * (g.ny + 1);
+  const long fxpl = (long)(g.nx + 1) * g.ny;
   const long ixl = (long)i * g.ny + j, ixr = ixl + g.ny;
-  const long iyb = (long)i * (g.ny + 1) + j, iyt = iyb + 1;
   const double inv_dx = g.rdx, inv_dy = g.rdy;
   double r[6];
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
+    double fyb, fyt;
+    if (FROM_PLANES) {
+      const double* __restrict__ FY = glb(sd.FY);
+      const long fypl = (long)g.nx * (g.ny + 1);
+      const long iyb = (long)i * (g.ny + 1) + j, iyt = iyb + 1;
+      fyb = FY[f * fypl + iyb];
+      fyt = FY[f * fypl + iyt];
+    } else {
+      fyb = fyb_in[f];
+      fyt = fyt_in[f];
+    }
     const double fxt = -(FX[f * fxpl + ixr] - FX[f * fxpl + ixl]) * inv_dx;
-    const double fyt = -(FY[f * fypl + iyt] - FY[f * fypl + iyb]) * inv_dy;
-    r[f] = fxt + fyt;
+    const double fy_ = -(fyt - fyb) * inv_dy;
+    r[f] = fxt + fy_;
   }
   // non-conservative terms, src/noncon.rs:288-332
   double ncx[3], ncy[3];
@@ -1007,7 +1028,40 @@ __global__ __launch_bounds__(256) void k_assemble(const SolveDesc* __restrict__ 
   const int lt = t - tiles[s];
   const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
   const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
-  const double m = (i < sd.g.nx && j < sd.g.ny) ? assemble_cell<FAST>(sd, ph, sa, i, j) : INFINITY;
+  const double m =
+      (i < sd.g.nx && j < sd.g.ny) ? assemble_core<FAST, true>(sd, ph, sa, i, j, nullptr, nullptr) : INFINITY;
+  if (sa.mode == 3) put_group_min(m, partial, t);
+}
+
+// The y fluxes and the assembly in one pass, so that the FY planes are
+// neither written nor read back: a row of blockDim.x lanes (a power of two
+// up to 64, so a row lies within one wavefront) takes the y interfaces
+// j0 - 1/2 .. j0 + blockDim.x - 3/2 and assembles the blockDim.x - 1 cells
+// j0 .. j0 + blockDim.x - 2 (the top flux of a cell is the bottom flux of
+// the next lane, passed by a lane shuffle).  Same arithmetic as k_flux<1>
+// followed by k_assemble.
+template <bool FAST>
+__global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_fy_assemble(const SolveDesc* __restrict__ descs,
+                                                                      const int* __restrict__ tiles, int nsolve,
+                                                                      const EigConst* __restrict__ Kp, Phys ph,
+                                                                      int src, StageSpec sa,
+                                                                      double* __restrict__ partial) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int s = find_solve(tiles, nsolve, t);
+  const SolveDesc& sd = descs[s];
+  const int nx = sd.g.nx, ny = sd.g.ny;
+  const int cw = blockDim.x - 1;
+  const int tiles_j = (ny + cw - 1) / cw;
+  const int lt = t - tiles[s];
+  const int j = (lt % tiles_j) * cw + threadIdx.x;
+  const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
+  double fb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < nx && j <= ny) iface_flux<1, FAST>(sd, Kp, ph.eps, src, i, j, fb);
+  double ft[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) ft[f] = __shfl_down(fb[f], 1, blockDim.x);
+  const double m = ((int)threadIdx.x < cw && i < nx && j < ny) ? assemble_core<FAST, false>(sd, ph, sa, i, j, fb, ft)
+                                                                : INFINITY;
   if (sa.mode == 3) put_group_min(m, partial, t);
 }
 
@@ -1030,20 +1084,22 @@ __global__ void k_capture(const SolveDesc* __restrict__ descs, int nsolve, unsig
 // ------------------------------------------------------------- dt
 
 // get_local_dt (src/dt.rs:5-34) of the state in U0 on every fluid cell,
-// reduced per workgroup; same tiles and partial layout as k_assemble.  Used
-// when the state did not come out of a stage 3 (first step, host writes).
+// reduced per workgroup; same tiles and partial layout as the assembly
+// kernel in use (cw cells per tile row: blockDim.x for k_assemble,
+// blockDim.x - 1 for k_fy_assemble).  Used when the state did not come out
+// of a stage 3 (first step, host writes).
 __global__ __launch_bounds__(256) void k_dt_partial(const SolveDesc* __restrict__ descs,
                                                     const int* __restrict__ tiles, int nsolve, Phys ph,
-                                                    double* __restrict__ partial) {
+                                                    double* __restrict__ partial, int cw) {
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int s = find_solve(tiles, nsolve, t);
   const SolveDesc& sd = descs[s];
-  const int tiles_j = (sd.g.ny + blockDim.x - 1) / blockDim.x;
+  const int tiles_j = (sd.g.ny + cw - 1) / cw;
   const int lt = t - tiles[s];
-  const int j = (lt % tiles_j) * blockDim.x + threadIdx.x;
+  const int j = (lt % tiles_j) * cw + threadIdx.x;
   const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
   double m = INFINITY;
-  if (i < sd.g.nx && j < sd.g.ny) {
+  if ((int)threadIdx.x < cw && i < sd.g.nx && j < sd.g.ny) {
     const long p = pad_index(sd.g, i, j);
     if (glb(sd.pt)[p] == 1) {
       const double* S = glb(sd.U[0]);
@@ -1137,10 +1193,14 @@ __global__ __launch_bounds__(256) void k_gather_ghosts(SolveDesc sd, int n, doub
 namespace rh3t_eng {
 const char* const kKernelNames[K_COUNT] = {"ghost_cheap", "ghost_heavy", "derived", "flux_x", "flux_y",
                                            "assemble", "dt_partial", "dt_final", "capture", "adj_iface",
-                                           "adj_gather", "adj_ghost", "adj_param"};
+                                           "adj_gather", "adj_ghost", "adj_param", "flux_y_assemble"};
 }  // namespace rh3t_eng
 
 namespace {
+
+// The y fluxes are formed inside the assembly pass (k_fy_assemble) when a
+// row of the workgroup is a power of two of lanes within one wavefront.
+bool fused_fy(const Engine& E) { return E.bx == 16 || E.bx == 32 || E.bx == 64; }
 
 // prefix tables for the current launch shape
 int build_tables(Engine& E) {
@@ -1149,6 +1209,7 @@ int build_tables(Engine& E) {
   E.t_fy.assign(n + 1, 0);
   E.t_as.assign(n + 1, 0);
   E.t_der.assign(n + 1, 0);
+  E.t_fa.assign(n + 1, 0);
   for (int s = 0; s < n; ++s) {
     const Geo& g = E.solves[s].g;
     auto tiles = [&](int ni, int nj) { return ((nj + E.bx - 1) / E.bx) * ((ni + E.by - 1) / E.by); };
@@ -1156,16 +1217,17 @@ int build_tables(Engine& E) {
     E.t_fy[s + 1] = E.t_fy[s] + tiles(g.nx, g.ny + 1);
     E.t_as[s + 1] = E.t_as[s] + tiles(g.nx, g.ny);
     E.t_der[s + 1] = E.t_der[s] + (int)((g.plane + 255) / 256);
+    E.t_fa[s + 1] = E.t_fa[s] + ((g.ny + E.bx - 2) / (E.bx - 1)) * ((g.nx + E.by - 1) / E.by);
   }
   std::vector<int> all;
-  for (auto* t : {&E.t_fx, &E.t_fy, &E.t_as, &E.t_der}) all.insert(all.end(), t->begin(), t->end());
+  for (auto* t : {&E.t_fx, &E.t_fy, &E.t_as, &E.t_der, &E.t_fa}) all.insert(all.end(), t->begin(), t->end());
   if (!E.d_tabs) {
     int st = dev_alloc(E, &E.d_tabs, all.size());
     if (st) return st;
   }
   HIP_TRY(hipMemcpy(E.d_tabs, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
   // one dt partial per assembly workgroup
-  const long need = (long)total(E.t_as);
+  const long need = (long)std::max(total(E.t_as), total(E.t_fa));
   if (need > E.partial_cap) {
     int st = dev_alloc(E, &E.d_partial, (size_t)need);
     if (st) return st;
@@ -1223,14 +1285,23 @@ void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture) {
     else
       k_flux<0, false><<<total(E.t_fx), blk, 0, E.stream>>>(E.d_desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
   }
-  {
-    Launcher L(E, K_FLUX_Y);
+  if (fused_fy(E)) {
+    Launcher L(E, K_FY_ASSEMBLE);
     if (E.fast)
-      k_flux<1, true><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps, src);
+      k_fy_assemble<true><<<total(E.t_fa), blk, 0, E.stream>>>(E.d_desc, E.tab(4), E.nsolve, E.d_eig, E.phys, src,
+                                                               sa, E.d_partial);
     else
-      k_flux<1, false><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps, src);
-  }
-  {
+      k_fy_assemble<false><<<total(E.t_fa), blk, 0, E.stream>>>(E.d_desc, E.tab(4), E.nsolve, E.d_eig, E.phys, src,
+                                                                sa, E.d_partial);
+  } else {
+    {
+      Launcher L(E, K_FLUX_Y);
+      if (E.fast)
+        k_flux<1, true><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps, src);
+      else
+        k_flux<1, false><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps,
+                                                              src);
+    }
     Launcher L(E, K_ASSEMBLE);
     if (E.fast)
       k_assemble<true><<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
@@ -1243,16 +1314,21 @@ void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture) {
   }
 }
 
+// get_local_dt minima of U0 per assembly workgroup (the layout k_dt_final reads)
+void launch_dt_partial(Engine& E) {
+  Launcher L(E, K_DT_PARTIAL);
+  const bool fu = fused_fy(E);
+  k_dt_partial<<<total(fu ? E.t_fa : E.t_as), dim3(E.bx, E.by), 0, E.stream>>>(E.d_desc, E.tab(fu ? 4 : 2), E.nsolve,
+                                                                             E.phys, E.d_partial,
+                                                                             fu ? E.bx - 1 : E.bx);
+}
+
 // dt of every solve from the wavefront minima; fresh: recompute them from
 // U0 first (otherwise the last stage 3 left them for the current state)
 void launch_dt(Engine& E, int advance, bool fresh) {
-  if (fresh) {
-    Launcher L(E, K_DT_PARTIAL);
-    k_dt_partial<<<total(E.t_as), dim3(E.bx, E.by), 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys,
-                                                                   E.d_partial);
-  }
+  if (fresh) launch_dt_partial(E);
   Launcher L(E, K_DT_FINAL);
-  k_dt_final<<<E.nsolve, 1024, 0, E.stream>>>(E.d_desc, E.tab(2), E.d_partial, advance);
+  k_dt_final<<<E.nsolve, 1024, 0, E.stream>>>(E.d_desc, E.tab(fused_fy(E) ? 4 : 2), E.d_partial, advance);
 }
 
 // rk3_ssp, src/main.rs:351-373, with the state kept in U0:
@@ -1765,11 +1841,7 @@ int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
   }
   HIP_TRY(hipMemcpyAsync(E.d_sc, sc.data(), sizeof(Scalars) * E.nsolve, hipMemcpyHostToDevice, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
-  if (!E.dt_partials_valid) {
-    Launcher L(E, K_DT_PARTIAL);
-    k_dt_partial<<<total(E.t_as), dim3(E.bx, E.by), 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys,
-                                                                   E.d_partial);
-  }
+  if (!E.dt_partials_valid) launch_dt_partial(E);
   if (E.profiling) {
     for (int s = 0; s < nsteps; ++s) {
       launch_dt(E, 1, false);

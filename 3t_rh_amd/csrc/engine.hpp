@@ -108,7 +108,7 @@ struct StageSpec {
 
 enum KernelId {
   K_GHOST_CHEAP, K_GHOST_HEAVY, K_DERIVED, K_FLUX_X, K_FLUX_Y, K_ASSEMBLE, K_DT_PARTIAL, K_DT_FINAL, K_CAPTURE,
-  K_ADJ_IFACE, K_ADJ_GATHER, K_ADJ_GHOST, K_ADJ_PARAM, K_COUNT
+  K_ADJ_IFACE, K_ADJ_GATHER, K_ADJ_GHOST, K_ADJ_PARAM, K_FY_ASSEMBLE, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 
@@ -155,7 +155,7 @@ struct Engine {
   // launch shapes (autotuned): flux / assemble workgroup (bx, by), heavy
   // ghosts per workgroup
   int bx = 64, by = 4, ghosts_per_block = 1;
-  std::vector<int> t_fx, t_fy, t_as, t_der;  // host prefix tables
+  std::vector<int> t_fx, t_fy, t_as, t_der, t_fa;  // host prefix tables
   int* d_tabs = nullptr;  // device copy: fx | fy | as | der, each nsolve+1
   double* d_partial = nullptr;  // get_local_dt minimum per assembly workgroup
   long partial_cap = 0;
