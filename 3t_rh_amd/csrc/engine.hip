@@ -469,6 +469,22 @@ __global__ __launch_bounds__(64) void k_ghost_frames(const SolveDesc* __restrict
   for (int c = 0; c < 6; ++c) o[72 + c] = lamx(Q, c);
 }
 
+// One packed point per lane: its 6 state values, contiguous.
+__global__ __launch_bounds__(256) void k_pack_points(const SolveDesc* __restrict__ descs, PackedPoints pk, int buf) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pk.n) return;
+  const SolveDesc& sd = descs[pk.sid[e]];
+  const double* S = sd.U[buf];
+  const long q = pk.slot[e];
+  double v[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) v[f] = S[f * sd.g.plane + q];
+  double2* o = reinterpret_cast<double2*>(pk.out + 6L * e);
+  o[0] = make_double2(v[0], v[1]);
+  o[1] = make_double2(v[2], v[3]);
+  o[2] = make_double2(v[4], v[5]);
+}
+
 // FarField / Outflow ghosts, kOpenPerWave of them per 64-lane wavefront:
 // lane group g = lane / LG (LG = 64 / kOpenPerWave lanes) takes one ghost.
 // The eigensystem of the ghost's frame comes from k_ghost_frames (formed
@@ -520,7 +536,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
   const bool far = hg.kind[h] == RH3T_BC_FARFIELD;
   // The ghost's loads are issued together: the state gathers at its E_r
-  // points (behind the stencil slots of its table row), its frame's
+  // points (packed by k_pack_points: 3 x 16 bytes per point), its frame's
   // eigensystem (k_ghost_frames) and the table entries of the first
   // segment, so that their latencies overlap.
   const double* T = hg.T + (long)kSwzLen * row + l;
@@ -538,9 +554,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       double s6[6] = {0, 0, 0, 0, 0, 0};
       if (has[p]) {
         if (a < 55) {
-          const long q = hg.stencil[55L * row + a];
-#pragma unroll
-          for (int f = 0; f < 6; ++f) s6[f] = S[f * plane + q];
+          const double2* pp = reinterpret_cast<const double2*>(hg.P + 6L * hg.gpk[55L * h + a]);
+          const double2 x0 = pp[0], x1 = pp[1], x2 = pp[2];
+          s6[0] = x0.x;
+          s6[1] = x0.y;
+          s6[2] = x1.x;
+          s6[3] = x1.y;
+          s6[4] = x2.x;
+          s6[5] = x2.y;
         } else {
 #pragma unroll
           for (int f = 0; f < 6; ++f) s6[f] = hg.bcstate[6L * h + f];
@@ -1256,6 +1277,7 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, int gt, bool capture) {
     const size_t lds = (size_t)g * kGhostLds * sizeof(double);
     if (n_open > 0) {
       k_ghost_frames<<<(E.frames.n + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.frames, E.phys, buf);
+      k_pack_points<<<(E.packed.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.packed, buf);
       const int per = g * kOpenPerWave;  // ghosts per workgroup
       k_ghost_open<<<(n_open + per - 1) / per, dim3(64, g), (size_t)per * kOpenLds * sizeof(double), E.stream>>>(
           E.d_desc, E.heavy, E.phys, buf, phase, 0, n_open);
@@ -1737,6 +1759,24 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
       h_frame[k] = it->second;
     }
   }
+  // the distinct E_r points of the FarField / Outflow ghosts, numbered in
+  // the order the ghosts first read them
+  std::vector<int> h_gpk((size_t)E.heavy.n_open * 55), p_sid;
+  std::vector<long> p_slot;
+  {
+    std::map<std::pair<int, long>, int> pid;
+    for (int k = 0; k < E.heavy.n_open; ++k)
+      for (int a = 0; a < 55; ++a) {
+        const auto key = std::make_pair(h_sid[k], h_st[55 * (size_t)h_tab[k] + a]);
+        auto it = pid.find(key);
+        if (it == pid.end()) {
+          it = pid.emplace(key, (int)p_sid.size()).first;
+          p_sid.push_back(key.first);
+          p_slot.push_back(key.second);
+        }
+        h_gpk[55 * (size_t)k + a] = it->second;
+      }
+  }
   E.h_heavy_kind = h_kind;
   E.h_heavy_tab = h_tab;
   E.h_heavy_pad = h_pad;
@@ -1786,6 +1826,12 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   UP(f_n, dp) E.frames.normal = dp;
   if ((st = dev_alloc(E, &E.frames.out, (size_t)kFrameLen * (f_sid.empty() ? 1 : f_sid.size())))) return st;
   E.heavy.F = E.frames.out;
+  UP(h_gpk, ip) E.heavy.gpk = ip;
+  E.packed.n = (int)p_sid.size();
+  UP(p_sid, ip) E.packed.sid = ip;
+  UP(p_slot, lp) E.packed.slot = lp;
+  if ((st = dev_alloc(E, &E.packed.out, 6 * (p_sid.empty() ? 1 : p_sid.size())))) return st;
+  E.heavy.P = E.packed.out;
 #undef UP
   if ((st = build_tables(E))) return st;
   return RH3T_OK;

@@ -73,6 +73,19 @@ struct HeavyGhosts {
   const double* T;       // kSwzLen per table row: S and W swizzled for k_ghost_open
   const int* frame;      // FarField / Outflow: the ghost's frame (GhostFrames)
   const double* F;       // kFrameLen per frame, formed by k_ghost_frames each update
+  const int* gpk;        // FarField / Outflow: 55 per ghost, PackedPoints index of each E_r point
+  const double* P;       // PackedPoints values, formed by k_pack_points each update
+};
+
+// The E_r points read by the FarField / Outflow ghosts, each distinct
+// (solve, slot) once, their 6 state values contiguous (48 bytes, one cache
+// line at most two) instead of in 6 planes: k_pack_points copies them out of
+// the state buffer before k_ghost_open gathers them.
+struct PackedPoints {
+  int n;
+  const int* sid;
+  const long* slot;
+  double* out;  // 6 per point
 };
 
 // The frames of the FarField / Outflow ghosts: ghosts of one solve that
@@ -143,6 +156,7 @@ struct Engine {
   CheapGhosts cheap{};
   HeavyGhosts heavy{};
   GhostFrames frames{};
+  PackedPoints packed{};
   // host copies of the cheap-ghost lists (the adjoint transposes them)
   std::vector<int> h_cheap_sid, h_cheap_kind;
   std::vector<long> h_cheap_pad, h_cheap_src;
