@@ -431,14 +431,12 @@ __device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs,
     raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
 }
 
-// One frame per lane: build_local_plain (src/bc1.rs:294-312), the nearest
+// One frame: build_local_plain (src/bc1.rs:294-312), the nearest
 // state rotated to the (n, t) frame and the X-direction eigensystem of its
 // Roe average with itself -- the rows of L and R and the eigenvalues, the
 // values k_ghost_open's ghosts of this frame apply.
-__global__ __launch_bounds__(64) void k_ghost_frames(const SolveDesc* __restrict__ descs, GhostFrames fr, Phys ph,
-                                                     int buf) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= fr.n) return;
+__device__ __forceinline__ void ghost_frame(const SolveDesc* __restrict__ descs, const GhostFrames& fr,
+                                            const Phys& ph, int buf, int k) {
   const SolveDesc& sd = descs[fr.sid[k]];
   const double* S = sd.U[buf];
   const long plane = sd.g.plane;
@@ -469,10 +467,9 @@ __global__ __launch_bounds__(64) void k_ghost_frames(const SolveDesc* __restrict
   for (int c = 0; c < 6; ++c) o[72 + c] = lamx(Q, c);
 }
 
-// One packed point per lane: its 6 state values, contiguous.
-__global__ __launch_bounds__(256) void k_pack_points(const SolveDesc* __restrict__ descs, PackedPoints pk, int buf) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= pk.n) return;
+// One packed point: its 6 state values, contiguous.
+__device__ __forceinline__ void pack_point(const SolveDesc* __restrict__ descs, const PackedPoints& pk, int buf,
+                                           int e) {
   const SolveDesc& sd = descs[pk.sid[e]];
   const double* S = sd.U[buf];
   const long q = pk.slot[e];
@@ -485,9 +482,23 @@ __global__ __launch_bounds__(256) void k_pack_points(const SolveDesc* __restrict
   o[2] = make_double2(v[4], v[5]);
 }
 
+// The FarField / Outflow ghosts' inputs in one launch: workgroups [0, nbf)
+// form the frames, the others pack the E_r points.
+__global__ __launch_bounds__(64) void k_ghost_prep(const SolveDesc* __restrict__ descs, GhostFrames fr,
+                                                   PackedPoints pk, Phys ph, int buf, int nbf) {
+  const int b = blockIdx.x;
+  if (b < nbf) {
+    const int k = b * 64 + threadIdx.x;
+    if (k < fr.n) ghost_frame(descs, fr, ph, buf, k);
+  } else {
+    const int e = (b - nbf) * 64 + threadIdx.x;
+    if (e < pk.n) pack_point(descs, pk, buf, e);
+  }
+}
+
 // FarField / Outflow ghosts, kOpenPerWave of them per 64-lane wavefront:
 // lane group g = lane / LG (LG = 64 / kOpenPerWave lanes) takes one ghost.
-// The eigensystem of the ghost's frame comes from k_ghost_frames (formed
+// The eigensystem of the ghost's frame comes from k_ghost_prep (formed
 // once per nearest point and normal, about one in four ghosts); what else is
 // uniform over a ghost -- the Outflow relaxation, the Taylor sum -- is
 // evaluated once per lane group, so the wavefront's instructions serve
@@ -536,8 +547,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
   const bool far = hg.kind[h] == RH3T_BC_FARFIELD;
   // The ghost's loads are issued together: the state gathers at its E_r
-  // points (packed by k_pack_points: 3 x 16 bytes per point), its frame's
-  // eigensystem (k_ghost_frames) and the table entries of the first
+  // points (packed by k_ghost_prep: 3 x 16 bytes per point), its frame's
+  // eigensystem (k_ghost_prep) and the table entries of the first
   // segment, so that their latencies overlap.
   const double* T = hg.T + (long)kSwzLen * row + l;
   double tvA[kSwzA];
@@ -1274,10 +1285,13 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, int gt, bool capture) {
   if (E.heavy.n > 0) {
     Launcher L(E, K_GHOST_HEAVY);
     const int g = E.ghosts_per_block, n_open = E.heavy.n_open, n_wall = E.heavy.n - n_open;
+    if (n_open > 0) {
+      const int nbf = (E.frames.n + 63) / 64;
+      k_ghost_prep<<<nbf + (E.packed.n + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.frames, E.packed, E.phys, buf,
+                                                                       nbf);
+    }
     const size_t lds = (size_t)g * kGhostLds * sizeof(double);
     if (n_open > 0) {
-      k_ghost_frames<<<(E.frames.n + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.frames, E.phys, buf);
-      k_pack_points<<<(E.packed.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.packed, buf);
       const int per = g * kOpenPerWave;  // ghosts per workgroup
       k_ghost_open<<<(n_open + per - 1) / per, dim3(64, g), (size_t)per * kOpenLds * sizeof(double), E.stream>>>(
           E.d_desc, E.heavy, E.phys, buf, phase, 0, n_open);
@@ -1760,7 +1774,8 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
     }
   }
   // the distinct E_r points of the FarField / Outflow ghosts, numbered in
-  // the order the ghosts first read them
+  // the order the ghosts first read them (the Wall ghosts, one per
+  // wavefront, gather from the planes: packing them measured no faster)
   std::vector<int> h_gpk((size_t)E.heavy.n_open * 55), p_sid;
   std::vector<long> p_slot;
   {

@@ -72,14 +72,14 @@ struct HeavyGhosts {
   const double* W;       // 225 per table row
   const double* T;       // kSwzLen per table row: S and W swizzled for k_ghost_open
   const int* frame;      // FarField / Outflow: the ghost's frame (GhostFrames)
-  const double* F;       // kFrameLen per frame, formed by k_ghost_frames each update
+  const double* F;       // kFrameLen per frame, formed by k_ghost_prep each update
   const int* gpk;        // FarField / Outflow: 55 per ghost, PackedPoints index of each E_r point
-  const double* P;       // PackedPoints values, formed by k_pack_points each update
+  const double* P;       // PackedPoints values, formed by k_ghost_prep each update
 };
 
 // The E_r points read by the FarField / Outflow ghosts, each distinct
 // (solve, slot) once, their 6 state values contiguous (48 bytes, one cache
-// line at most two) instead of in 6 planes: k_pack_points copies them out of
+// line at most two) instead of in 6 planes: k_ghost_prep copies them out of
 // the state buffer before k_ghost_open gathers them.
 struct PackedPoints {
   int n;
@@ -90,7 +90,7 @@ struct PackedPoints {
 
 // The frames of the FarField / Outflow ghosts: ghosts of one solve that
 // share a nearest point and a normal share the eigensystem of
-// build_local_plain (src/bc1.rs:294-312), which k_ghost_frames forms once
+// build_local_plain (src/bc1.rs:294-312), which k_ghost_prep forms once
 // per frame: rows of L (36), rows of R (36), eigenvalues (6).
 constexpr int kFrameLen = 78;
 struct GhostFrames {
