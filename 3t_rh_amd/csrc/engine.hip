@@ -511,7 +511,10 @@ __global__ __launch_bounds__(64) void k_ghost_prep(const SolveDesc* __restrict__
 // 760-928), so the values are the same as one ghost per wavefront.
 constexpr int kOpenPerWave = 2;
 constexpr int kOpenLG = 64 / kOpenPerWave;
-constexpr int kOpenLds = 336 + 90 + kFrameLen;  // doubles per ghost
+// data rows of 57 doubles (55 E_r points, u_inf, one pad): a stride of 56
+// put rows c and c + 2 on the same LDS banks
+constexpr int kRow = 57;
+constexpr int kOpenLds = 6 * kRow + 90 + kFrameLen;  // doubles per ghost
 static_assert(kOpenLG == 32, "the swizzled table schedule is for 32-lane groups");
 // slot of v_k,r (k <= r; v_k,r = 0 for k > r is never stored)
 __device__ __forceinline__ int vk_slot(int k, int r) { return r * (r + 1) / 2 + k; }
@@ -523,7 +526,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int grp = lane / LG, l = lane % LG;
   const int w = threadIdx.y;
   const int hl = (xcd_tile(blockIdx.x, gridDim.x) * blockDim.y + w) * kOpenPerWave + grp;
-  // per-ghost LDS (kOpenLds doubles): data[6][56] (L . u at the E_r points,
+  // per-ghost LDS (kOpenLds doubles): data[6][kRow] (L . u at the E_r points,
   // then, once the table rows have been applied, the products data * (S_r
   // data) in the same slots; slot 55 keeps L . u_inf), vk[15][6] (v_k,r for
   // k <= r), the frame (rows of L, rows of R, eigenvalues).  Once read, the
@@ -533,7 +536,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   extern __shared__ double s_dyn[];
   double* const s_data = s_dyn + ((long)w * kOpenPerWave + grp) * kOpenLds;
   double* const s_part = s_data;
-  double* const s_vk = s_data + 336;
+  double* const s_vk = s_data + 6 * kRow;
   double* const s_F = s_vk + 90;
   double* const s_alpha = s_F;
   double* const s_v = s_data;
@@ -605,7 +608,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       for (int m = 0; m < 6; ++m) lr[m] = s_F[6 * c + m];
 #pragma unroll
       for (int p = 0; p < NP; ++p)
-        if (has[p]) s_data[c * 56 + l + p * LG] = dot6(lr, u6[p]);
+        if (has[p]) s_data[c * kRow + l + p * LG] = dot6(lr, u6[p]);
     }
   }
   __syncthreads();
@@ -625,11 +628,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       for (int b = 0; b < kSwzA; ++b)
         if (b < m)
 #pragma unroll
-          for (int c = 0; c < 6; ++c) acc[c] += tvA[b] * s_data[c * 56 + d0 + b];
+          for (int c = 0; c < 6; ++c) acc[c] += tvA[b] * s_data[c * kRow + d0 + b];
       if (l < 25 || l == 30) {
         const int a = l < 25 ? 30 + l : 0;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) s_data[c * 56 + a] = s_data[c * 56 + a] * acc[c];
+        for (int c = 0; c < 6; ++c) s_data[c * kRow + a] = s_data[c * kRow + a] * acc[c];
       } else {
         const int q = l < 30 ? vk_slot(l - 25, 4) : vk_slot(0, 0);
 #pragma unroll
@@ -646,12 +649,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
       for (int b = 0; b < 9; ++b)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) acc[c] += tv[b] * s_data[c * 56 + d0 + b];
+        for (int c = 0; c < 6; ++c) acc[c] += tv[b] * s_data[c * kRow + d0 + b];
       if (l >= 20) {
         if (l < 29) {
           const int a = 5 + (l - 20);
 #pragma unroll
-          for (int c = 0; c < 6; ++c) s_data[c * 56 + a] = s_data[c * 56 + a] * acc[c];
+          for (int c = 0; c < 6; ++c) s_data[c * kRow + a] = s_data[c * kRow + a] * acc[c];
         } else {
 #pragma unroll
           for (int c = 0; c < 6; ++c) s_vk[vk_slot(l - 29, 2) * 6 + c] = acc[c];
@@ -664,11 +667,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       for (int b = 9; b < kSwzB; ++b)
         if (b < m)
 #pragma unroll
-          for (int c = 0; c < 6; ++c) acc[c] += tv[b] * s_data[c * 56 + d1 + b];
+          for (int c = 0; c < 6; ++c) acc[c] += tv[b] * s_data[c * kRow + d1 + b];
       if (l < 16 || (l >= 20 && l < 24)) {
         const int a = l < 16 ? 14 + l : 1 + (l - 20);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) s_data[c * 56 + a] = s_data[c * 56 + a] * acc[c];
+        for (int c = 0; c < 6; ++c) s_data[c * kRow + a] = s_data[c * kRow + a] * acc[c];
       } else if (l < 26) {
         const int q = l < 20 ? vk_slot(l - 16, 3) : vk_slot(l - 24, 1);
 #pragma unroll
@@ -681,7 +684,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int t = l; t < 30; t += LG) {
       const int r = t / 6, c = t % 6;
       const int m = (r + 1) * (r + 1);
-      const double* p = &s_part[c * 56 + r * (r + 1) * (2 * r + 1) / 6];
+      const double* p = &s_part[c * kRow + r * (r + 1) * (2 * r + 1) / 6];
       double beta = 0.0;
       for (int a = 0; a < m; ++a) beta += p[a];
       s_alpha[c * 5 + r] = hg.d5[5L * row + r] / weno_pow(ph.eps + beta, ph.weno_q);
@@ -760,7 +763,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       if (far) {
 #pragma unroll
         for (int m = 0; m < 6; ++m)
-          if (s_F[72 + m] < 0.0) rhs[m] = k == 0 ? s_data[m * 56 + 55] : 0.0;
+          if (s_F[72 + m] < 0.0) rhs[m] = k == 0 ? s_data[m * kRow + 55] : 0.0;
       } else if (k == 1 && s_fix[0] != 0.0) {
         rhs[0] = s_fix[1];
       }
