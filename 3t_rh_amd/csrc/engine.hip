@@ -553,7 +553,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // points (packed by k_ghost_prep: 3 x 16 bytes per point), its frame's
   // eigensystem (k_ghost_prep) and the table entries of the first
   // segment, so that their latencies overlap.
-  const double* T = hg.T + (long)kSwzLen * row + l;
+  const double2* T = reinterpret_cast<const double2*>(hg.T + (long)kSwzLen * row) + l;
   double tvA[kSwzA];
   // L . u at the E_r points (a < 55) and, for a FarField ghost, L . u_inf
   // (src/bc1.rs:829-924) in the free slot 55: point a = l + p LG, p = 0, 1
@@ -595,7 +595,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       for (int q = 0; q < (kFrameLen + LG - 1) / LG; ++q)
         if (l + q * LG < kFrameLen) fv[q] = Fg[l + q * LG];
 #pragma unroll
-      for (int b = 0; b < kSwzA; ++b) tvA[b] = T[32 * b];
+      for (int q = 0; q < kSwzPairsA; ++q) {
+        const double2 t2 = T[32 * q];
+        tvA[2 * q] = t2.x;
+        if (2 * q + 1 < kSwzA) tvA[2 * q + 1] = t2.y;
+      }
 #pragma unroll
       for (int q = 0; q < (kFrameLen + LG - 1) / LG; ++q)
         if (l + q * LG < kFrameLen) s_F[l + q * LG] = fv[q];
@@ -615,8 +619,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   if (active) {
     // The 70 table rows on a fixed two-segment schedule (kSwz* in
     // engine.hpp): step s of the swizzled row block holds the entry each of
-    // the group's 32 lanes applies at that step, so every load of the block
-    // is 32 consecutive doubles.  Each row sums over b in increasing order
+    // the group's 32 lanes applies at that step, two steps per 16-byte load,
+    // so every load of the block is 512 consecutive bytes.  Each row sums over b in increasing order
     // from 0.0, as weno_extrapolation_pre.  An S row's product data[a] * (S
     // data)[a] goes straight into data slot a once no lane reads that E_r
     // segment any more (the group is one wavefront: LDS accesses complete in
@@ -643,7 +647,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
        // rows (20..31) over the first 9, then S_1 / w_1 rows (20..25) over 4
       double tv[kSwzB];
 #pragma unroll
-      for (int b = 0; b < kSwzB; ++b) tv[b] = T[32 * (kSwzA + b)];
+      for (int q = 0; q < kSwzPairsB; ++q) {
+        const double2 t2 = T[32 * (kSwzPairsA + q)];
+        tv[2 * q] = t2.x;
+        tv[2 * q + 1] = t2.y;
+      }
       const int d0 = l < 20 ? 14 : 5;
       double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1558,7 +1566,7 @@ static void swizzle_rows(const double* S, const double* W, std::vector<double>& 
         v = w_at(4, l - 25, b);
       else if (b == 0)
         v = l == 30 ? s_at(0, 0, 0) : w_at(0, 0, 0);
-      o[32 * b + l] = v;
+      o[(b / 2) * 64 + 2 * l + b % 2] = v;
     }
   for (int b = 0; b < kSwzB; ++b)
     for (int l = 0; l < 32; ++l) {
@@ -1571,7 +1579,7 @@ static void swizzle_rows(const double* S, const double* W, std::vector<double>& 
         v = l < 29 ? s_at(2, l - 20, b) : w_at(2, l - 29, b);
       else if (b < 13 && l < 26)
         v = l < 24 ? s_at(1, l - 20, b - 9) : w_at(1, l - 24, b - 9);
-      o[32 * (kSwzA + b) + l] = v;
+      o[(kSwzPairsA + b / 2) * 64 + 2 * l + b % 2] = v;
     }
 }
 

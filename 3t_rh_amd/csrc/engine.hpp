@@ -102,16 +102,21 @@ struct GhostFrames {
 };
 
 // The swizzled row block of a heavy-ghost table (FarField / Outflow kernel):
-// kSwzA + kSwzB steps of 32 lanes, step-major; entry (s, l) is what lane l
-// of a ghost's lane group multiplies at step s, 0.0 where it has none.
-//   steps 0..24:  lanes 0..24 row a = l of S_4, 25..29 row k = l - 25 of w_4
-//                 (entry b = s); lane 30 the one entry of S_0, 31 of w_0 (s = 0)
-//   steps 25..40: b = s - 25; lanes 0..15 row l of S_3, 16..19 row l - 16 of
-//                 w_3 (b < 16); lanes 20..28 row l - 20 of S_2, 29..31 row
-//                 l - 29 of w_2 (b < 9); then lanes 20..23 row l - 20 of S_1,
-//                 24..25 row l - 24 of w_1 (entry b - 9, 9 <= b < 13)
+// entry (s, l) is what lane l of a ghost's lane group multiplies at step s,
+// 0.0 where it has none.  Steps go in pairs, pair-major, lane-minor, the two
+// steps of a pair adjacent ((s / 2) * 64 + 2 l + s % 2): a lane reads two
+// steps with one 16-byte load, the group 512 consecutive bytes.
+//   steps 0..24 (segment A, padded to 26):  lanes 0..24 row a = l of S_4,
+//                 25..29 row k = l - 25 of w_4 (entry b = s); lane 30 the one
+//                 entry of S_0, 31 of w_0 (s = 0)
+//   steps 26..41 (segment B), b = s - 26: lanes 0..15 row l of S_3, 16..19
+//                 row l - 16 of w_3 (b < 16); lanes 20..28 row l - 20 of S_2,
+//                 29..31 row l - 29 of w_2 (b < 9); then lanes 20..23 row
+//                 l - 20 of S_1, 24..25 row l - 24 of w_1 (entry b - 9,
+//                 9 <= b < 13)
 constexpr int kSwzA = 25, kSwzB = 16;
-constexpr int kSwzLen = (kSwzA + kSwzB) * 32;
+constexpr int kSwzPairsA = (kSwzA + 1) / 2, kSwzPairsB = kSwzB / 2;
+constexpr int kSwzLen = (kSwzPairsA + kSwzPairsB) * 64;
 
 struct StageSpec {
   int mode;        // 0: write rhs, 1..3: SSP-RK3 stage
