@@ -11,7 +11,8 @@ production runs it (rh3t_advance replaying the captured hipGraph of one
 step); a second run of the same steps, launched eagerly with hipEvents
 recorded around every kernel on the solver's own stream, gives the
 per-kernel averages and the roofline entry for the dominant kernel (the
-x-direction WENO flux).  The CPU baseline is the oracle (C restatement of
+fused y-direction WENO flux + assembly pass; the x-direction flux kernel
+gets its own entry beside it).  The CPU baseline is the oracle (C restatement of
 the reference algorithm) timed on a bounded sample of the same workload:
 whole RK3 steps of the same 2048^2 box on the host's cores, about 10 s.
 """
@@ -35,6 +36,9 @@ PEAK_HBM = 8.0e12      # MI355X HBM3E, B/s
 # x/y, derived, assembly, mean stage combination, admissibility).
 FLOP_PER_INTERFACE = 2252
 FLOP_PER_CELL_STAGE = 5250
+# the assembly part of a cell-stage: noncon x and y (2 x 308), assembly 56,
+# stage combination 24, admissibility 12
+FLOP_PER_CELL_ASSEMBLY = 2 * 308 + 56 + 24 + 12
 BYTES_PER_CELL_STAGE = 128  # stage 1: read U^k, write U (96 B); stages 2-3: + U^n (144 B)
 # HBM traffic of one k_flux<0> launch on the 2048^2 box, from two rocprofv3
 # --pmc passes over this bench (FETCH_SIZE and WRITE_SIZE, KiB per dispatch,
@@ -47,6 +51,11 @@ COPY_FETCH_SCALE, COPY_WRITE_SCALE = 2.0, 1.0
 FLUX_X_TRAFFIC_BYTES = (FLUX_X_FETCH_KIB * COPY_FETCH_SCALE + FLUX_X_WRITE_KIB * COPY_WRITE_SCALE) * 1024
 TRAFFIC_SOURCE = ("not measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_flux<0> at 2048^2, "
                   "copy-calibrated (profiles/flux_x_counters_2048.md)")
+# the same for one k_fy_assemble<true> launch (profiles/hbm_per_stage_2048.md)
+FY_AS_FETCH_KIB, FY_AS_WRITE_KIB = 434816, 197220
+FY_AS_TRAFFIC_BYTES = (FY_AS_FETCH_KIB * COPY_FETCH_SCALE + FY_AS_WRITE_KIB * COPY_WRITE_SCALE) * 1024
+FY_AS_TRAFFIC_SOURCE = ("not measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_fy_assemble<true> "
+                        "at 2048^2, copy-calibrated (profiles/hbm_per_stage_2048.md)")
 
 
 def log(msg):
@@ -422,6 +431,32 @@ def main():
     fx_ms, fx_n = prof["flux_x"]
     interfaces = (n + 1) * n
     achieved = FLOP_PER_INTERFACE * interfaces / (fx_ms * 1e-3)
+    # the y fluxes with the assembly (one pass) when the launch shape fuses
+    # them, else the y flux kernel
+    if "flux_y_assemble" in prof and prof["flux_y_assemble"][1]:
+        dom_name, (dom_ms, dom_n) = "flux_y_assemble", prof["flux_y_assemble"]
+        dom_flop = FLOP_PER_INTERFACE * interfaces + FLOP_PER_CELL_ASSEMBLY * n_fluid
+        dom_traffic, dom_src = (FY_AS_TRAFFIC_BYTES, FY_AS_TRAFFIC_SOURCE) if (n == 2048 and head == "fast") \
+            else (None, None)
+    else:
+        dom_name, (dom_ms, dom_n) = "flux_y", prof["flux_y"]
+        dom_flop = FLOP_PER_INTERFACE * interfaces
+        dom_traffic, dom_src = None, None
+    roof_fx = {"bound": "valu", "pipe": "fp64 valu", "kernel": "flux_x", "achieved": achieved / 1e12,
+               "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+               "traffic": FLUX_X_TRAFFIC_BYTES if n == 2048 else None,
+               "traffic_source": TRAFFIC_SOURCE if n == 2048 else None,
+               "flop_per_interface": FLOP_PER_INTERFACE, "interfaces": interfaces,
+               "avg_launch_ms": fx_ms, "launches_timed": fx_n}
+    dom_ach = dom_flop / (dom_ms * 1e-3)
+    roof_dom = {"bound": "valu", "pipe": "fp64 valu", "kernel": dom_name, "achieved": dom_ach / 1e12,
+                "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": dom_ach / PEAK_FP64,
+                "traffic": dom_traffic, "traffic_source": dom_src,
+                "flop_per_launch": dom_flop, "flop_per_interface": FLOP_PER_INTERFACE,
+                "flop_per_cell_assembly": FLOP_PER_CELL_ASSEMBLY, "interfaces": interfaces, "cells": n_fluid,
+                "avg_launch_ms": dom_ms, "launches_timed": dom_n}
+    if fx_ms > dom_ms:
+        roof_dom, roof_fx = roof_fx, roof_dom
     line = {
         "metric": METRIC,
         "value": rate,
@@ -451,15 +486,12 @@ def main():
         other: {"value": legs[other]["rate"], "ms_per_step": legs[other]["wall"] / args.steps * 1e3,
                 "kernels_ms": {k: round(v[0], 4) for k, v in legs[other]["prof"].items() if v[1]},
                 "state_finite": legs[other]["finite"]},
-        # the dominant kernel is bound by FP64 vector (VALU) issue, priced
-        # against the dense FP64 peak (78.6 TFLOP/s: MI355X's vector and
-        # matrix FP64 peaks are equal)
-        "roofline": {"bound": "valu", "pipe": "fp64 valu", "kernel": "flux_x", "achieved": achieved / 1e12,
-                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                     "traffic": FLUX_X_TRAFFIC_BYTES if n == 2048 else None,
-                     "traffic_source": TRAFFIC_SOURCE if n == 2048 else None,
-                     "flop_per_interface": FLOP_PER_INTERFACE, "interfaces": interfaces,
-                     "avg_launch_ms": fx_ms, "launches_timed": fx_n},
+        # the dominant kernel (the longest launch) and the other flux pass,
+        # both bound by FP64 vector (VALU) issue, priced against the dense
+        # FP64 peak (78.6 TFLOP/s: MI355X's vector and matrix FP64 peaks
+        # are equal)
+        "roofline": roof_dom,
+        "roofline_other": roof_fx,
     }
     if not args.no_batched:
         log("batched leg")

@@ -394,6 +394,8 @@ def test_opcount_matches_bench_constants():
     bench = importlib.import_module("bench")
     assert counts["flux"] == bench.FLOP_PER_INTERFACE
     assert round(counts["cell_stage"]) == bench.FLOP_PER_CELL_STAGE
+    assembly = 2 * counts["noncon"] + counts["assembly"] + round(counts["stage_mean"]) + counts["admissible"]
+    assert assembly == bench.FLOP_PER_CELL_ASSEMBLY
 
 
 def test_cylinder_step_is_ill_conditioned(rh3t):
