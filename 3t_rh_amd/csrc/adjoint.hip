@@ -1331,7 +1331,7 @@ void launch_vjp(Engine& E, int buf, bool taped, const double* LAM, double rscale
   const SolveDesc& d = E.h_desc[0];
   const Geo& g = d.g;
   if (!taped) {
-    launch_ghosts(E, buf, 2u, GT_FIELD, false);
+    launch_ghosts(E, buf, 2u, GT_FIELD);
     launch_derived(E, buf);
   }
   AdjDesc A;
@@ -1555,14 +1555,14 @@ void recompute_segment(Engine& E, const rh3t_adjoint* A, int k) {
   for (int n = n0; n < n1; ++n) {
     k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, E.adj->dts, n, 0);
     OnStep on(E, A, n);
-    launch_stage(E, 0, kStage1, false);
+    launch_stage(E, 0, kStage1);
     on.at(1);
-    launch_stage(E, 1, kStage2, false);
+    launch_stage(E, 1, kStage2);
     on.at(2);
     if (n + 1 < n1) {
-      launch_stage(E, 2, kStage3, false);
+      launch_stage(E, 2, kStage3);
     } else {
-      launch_ghosts(E, 2, 6u, GT_STAGE3, false);
+      launch_ghosts(E, 2, 6u, GT_STAGE3);
       launch_derived(E, 2);
     }
   }
@@ -1600,6 +1600,7 @@ int rh3t_rk3_vjp(rh3t_ctx* ctx, double dt, const double* lam, double* ubar, doub
   // forward stages 1 and 2 (U[0] is left untouched)
   launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3});
   launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5});
+  launch_capture(E, 2, 5);
   if ((st = upload_cot(E, lam, W.lam[0]))) return st;
   launch_rk3_reverse(E, false, dt, W.lam[0], W.lam[1], W.lam[2], W.lam[3], [](int) {});
   HIP_TRY(hipGetLastError());
@@ -1720,11 +1721,16 @@ int rh3t_adjoint_forward(rh3t_adjoint* A, int32_t nsteps, const double* dt) {
   for (int n = 0; n < nsteps; ++n) {
     k_set_dt<<<1, 64, 0, E.stream>>>(E.d_sc, E.adj->dts, n, 1);
     OnStep on(E, A, n);
+    // each stage's own buffer descriptors: capture its phases before the
+    // next stage's descriptors replace them
     launch_stage(E, 0, kStage1);
+    launch_capture(E, 2, 3);
     on.at(1);
     launch_stage(E, 1, kStage2);
+    launch_capture(E, 4, 5);
     on.at(2);
     launch_stage(E, 2, kStage3);
+    launch_capture(E, 6, 7);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(A->t_fwd[1], E.stream));

@@ -1108,16 +1108,24 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_fy_assemble(const Solv
   if (sa.mode == 3) put_group_min(m, partial, t);
 }
 
-// One thread per solve: copies the state of the first failure of `phase`
-// into that solve's scalars (runs right after the kernels that can fail).
-__global__ void k_capture(const SolveDesc* __restrict__ descs, int nsolve, unsigned phase, int buf) {
+// One thread per solve: copies the state of the current step's first
+// failure, if its phase is in [lo, hi], into that solve's scalars.  Runs
+// after the stages whose phases it covers (once per step in the time loop):
+// no stage writes a buffer that an earlier phase of the same step failed in
+// -- phase 2k (the ghosts of stage k) fails in its source buffer U[k - 1]
+// (U[0] for an l()), phase 2k + 1 (stage k) in its destination, U[1], U[2]
+// and U[3] (the alias of U[0], or the tape buffer the adjoint's forward
+// points it at) -- so the failing values are still there.
+__global__ void k_capture(const SolveDesc* __restrict__ descs, int nsolve, unsigned lo, unsigned hi) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsolve) return;
   const SolveDesc& sd = descs[s];
   Scalars* sc = sd.sc;
   unsigned long long key = sc->err;
   if (key == ~0ULL) return;
-  if (((key >> 44) & 0xFFFFF) != (sc->step & 0xFFFFF) || ((key >> 40) & 0xF) != phase) return;
+  const unsigned phase = (unsigned)((key >> 40) & 0xF);
+  if (((key >> 44) & 0xFFFFF) != (sc->step & 0xFFFFF) || phase < lo || phase > hi || phase < 2 || phase > 7) return;
+  const int buf = phase == 7 ? 3 : (int)(phase / 2) - (phase % 2 == 0 ? 1 : 0);
   unsigned idx = (unsigned)(key & 0xFFFFFFFFULL);
   long p = (phase % 2 == 0) ? sd.gpad[idx] : pad_index(sd.g, idx / sd.g.ny, idx % sd.g.ny);
   const double* b = sd.U[buf];
@@ -1288,7 +1296,7 @@ int build_tables(Engine& E) {
 
 namespace rh3t_eng {
 
-void launch_ghosts(Engine& E, int buf, unsigned phase, int gt, bool capture) {
+void launch_ghosts(Engine& E, int buf, unsigned phase, int gt) {
   if (E.cheap.n > 0) {
     Launcher L(E, K_GHOST_CHEAP);
     k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.cheap, buf, phase, gt);
@@ -1311,9 +1319,11 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, int gt, bool capture) {
       k_ghost_wall<<<(n_wall + g - 1) / g, dim3(64, g), lds, E.stream>>>(E.d_desc, E.heavy, E.phys, buf,
                                                                                  phase, n_open, n_wall);
   }
-  if (!capture) return;
+}
+
+void launch_capture(Engine& E, unsigned lo, unsigned hi) {
   Launcher L(E, K_CAPTURE);
-  k_capture<<<(E.nsolve + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.nsolve, phase, buf);
+  k_capture<<<(E.nsolve + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.nsolve, lo, hi);
 }
 
 void launch_derived(Engine& E, int buf) {
@@ -1321,8 +1331,8 @@ void launch_derived(Engine& E, int buf) {
   k_derived<<<total(E.t_der), 256, 0, E.stream>>>(E.d_desc, E.tab(3), E.nsolve, E.phys, buf, E.nplanes);
 }
 
-void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture) {
-  launch_ghosts(E, src, sa.mode == 0 ? 2u : (unsigned)(2 * sa.mode), sa.mode, capture);
+void launch_stage(Engine& E, int src, const StageSpec& sa) {
+  launch_ghosts(E, src, sa.mode == 0 ? 2u : (unsigned)(2 * sa.mode), sa.mode);
   launch_derived(E, src);
   dim3 blk(E.bx, E.by);
   {
@@ -1355,10 +1365,6 @@ void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture) {
     else
       k_assemble<false><<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
   }
-  if (sa.mode > 0 && capture) {
-    Launcher L(E, K_CAPTURE);
-    k_capture<<<(E.nsolve + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.nsolve, sa.phase, sa.dst);
-  }
 }
 
 // get_local_dt minima of U0 per assembly workgroup (the layout k_dt_final reads)
@@ -1387,6 +1393,7 @@ void launch_rk3(Engine& E) {
   launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3});
   launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5});
   launch_stage(E, 2, StageSpec{3, 0, 2, 0, 7});
+  launch_capture(E, 2, 7);
 }
 
 // Reads every solve's first-failure key; returns the first failing status.
@@ -2071,6 +2078,7 @@ int rh3t_l(rh3t_ctx* ctx, double* rhs_aos, double* ghost_vals) {
   int st = reset_errors(E);
   if (st) return st;
   launch_stage(E, 0, StageSpec{0, 0, 0, 0, 2});
+  launch_capture(E, 2, 2);
   HIP_TRY(hipGetLastError());
   const SolveDesc& d = E.h_desc[0];
   const long ncell = (long)d.g.nx * d.g.ny;

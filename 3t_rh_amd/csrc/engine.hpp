@@ -249,17 +249,18 @@ struct Launcher {
 
 inline int total(const std::vector<int>& t) { return t.back(); }
 
-// ghosts of every solve on buffer buf (+ the first-failure state capture
-// unless capture is false; the failure key is raised either way), with
-// TimeDependent ghosts at time gt
-void launch_ghosts(Engine& E, int buf, unsigned phase, int gt, bool capture = true);
+// ghosts of every solve on buffer buf (a failure raises the solve's
+// first-failure key), with TimeDependent ghosts at time gt
+void launch_ghosts(Engine& E, int buf, unsigned phase, int gt);
 // Derived::from_state on every fluid cell and ghost of buffer buf
 void launch_derived(Engine& E, int buf);
 // ghosts -> derived -> x/y fluxes -> assemble (one l() + stage) on buffer src
-// capture: record the failing point's state of a stage error (a recompute
-// of stages that already ran without error skips it)
-void launch_stage(Engine& E, int src, const StageSpec& sa, bool capture = true);
-// one SSP-RK3 step with the dt held in the solves' scalars
+void launch_stage(Engine& E, int src, const StageSpec& sa);
+// the failing point's state of the current step's first failure, if its
+// phase is in [lo, hi], into the solves' scalars (after the stages of those
+// phases; a recompute of stages that already ran without error skips it)
+void launch_capture(Engine& E, unsigned lo, unsigned hi);
+// one SSP-RK3 step with the dt held in the solves' scalars, then the capture
 void launch_rk3(Engine& E);
 int check_errors(Engine& E);
 int reset_errors(Engine& E);

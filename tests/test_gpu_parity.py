@@ -283,6 +283,8 @@ def test_nonphysical_state_error(rh3t, box64):
     print("gpu", g, "oracle", o)
     assert ei.value.status == rh3t.ERR_NONPHYSICAL
     assert (g["stage"], g["i"], g["j"]) == (o["stage"], o["i"], o["j"])
+    # the reported state is the failing cell's stage-1 output
+    assert np.allclose(g["state"], o["state"], rtol=1e-12, atol=0.0, equal_nan=True)
     s.close()
 
 
@@ -511,6 +513,9 @@ def test_nonfinite_ghost_error(rh3t, cyl):
     assert ei.value.status == rh3t.ERR_GHOST and eo.value.status == 4
     assert g["ghost_id"] == o["ghost_id"]
     assert g["stage"] == 0
+    # the reported state is the failing ghost's value, as the oracle's
+    assert np.array_equal(np.isfinite(g["state"]), np.isfinite(o["state"]))
+    assert not np.isfinite(g["state"]).all()
     # the context stays usable
     s.set_state(u)
     assert np.isfinite(s.l()).all()
