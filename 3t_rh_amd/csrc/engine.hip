@@ -80,10 +80,8 @@ __device__ __forceinline__ void raise_error(Scalars* sc, unsigned phase, unsigne
 
 // ------------------------------------------------------------ ghosts
 
-__global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict__ descs, CheapGhosts cg, int buf,
-                                                     unsigned phase, int gt) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cg.n) return;
+__device__ __forceinline__ void ghost_cheap(const SolveDesc* __restrict__ descs, const CheapGhosts& cg, int buf,
+                                            unsigned phase, int gt, int k) {
   const SolveDesc& sd = descs[cg.sid[k]];
   double* S = sd.U[buf];
   const long plane = sd.g.plane;
@@ -124,6 +122,12 @@ __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict
     ok = ok && isfinite(v[f]);
   }
   if (!ok) raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)cg.gid[k]);
+}
+
+__global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict__ descs, CheapGhosts cg, int buf,
+                                                     unsigned phase, int gt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < cg.n) ghost_cheap(descs, cg, buf, phase, gt, k);
 }
 
 
@@ -482,17 +486,22 @@ __device__ __forceinline__ void pack_point(const SolveDesc* __restrict__ descs, 
   o[2] = make_double2(v[4], v[5]);
 }
 
-// The FarField / Outflow ghosts' inputs in one launch: workgroups [0, nbf)
-// form the frames, the others pack the E_r points.
+// The cheap ghosts and the FarField / Outflow ghosts' inputs in one launch:
+// workgroups [0, nbf) form the frames, [nbf, nbp) pack the E_r points, the
+// others take the cheap ghosts (which no frame or E_r point reads).
 __global__ __launch_bounds__(64) void k_ghost_prep(const SolveDesc* __restrict__ descs, GhostFrames fr,
-                                                   PackedPoints pk, Phys ph, int buf, int nbf) {
+                                                   PackedPoints pk, CheapGhosts cg, Phys ph, int buf,
+                                                   unsigned phase, int gt, int nbf, int nbp) {
   const int b = blockIdx.x;
   if (b < nbf) {
     const int k = b * 64 + threadIdx.x;
     if (k < fr.n) ghost_frame(descs, fr, ph, buf, k);
-  } else {
+  } else if (b < nbp) {
     const int e = (b - nbf) * 64 + threadIdx.x;
     if (e < pk.n) pack_point(descs, pk, buf, e);
+  } else {
+    const int k = (b - nbp) * 64 + threadIdx.x;
+    if (k < cg.n) ghost_cheap(descs, cg, buf, phase, gt, k);
   }
 }
 
@@ -1297,18 +1306,24 @@ int build_tables(Engine& E) {
 namespace rh3t_eng {
 
 void launch_ghosts(Engine& E, int buf, unsigned phase, int gt) {
-  if (E.cheap.n > 0) {
+  const int n_open = E.heavy.n_open;
+  const bool with_prep = n_open > 0 && E.cheap_with_prep;
+  if (E.cheap.n > 0 && !with_prep) {
     Launcher L(E, K_GHOST_CHEAP);
     k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.cheap, buf, phase, gt);
   }
+  if (n_open > 0) {
+    // the frames and packed points, with the cheap ghosts riding along
+    // when nothing the first two read is a ghost point
+    Launcher L(E, K_GHOST_HEAVY);
+    const int nbf = (E.frames.n + 63) / 64, nbp = nbf + (E.packed.n + 63) / 64;
+    const int nbc = with_prep ? (E.cheap.n + 63) / 64 : 0;
+    k_ghost_prep<<<nbp + nbc, 64, 0, E.stream>>>(E.d_desc, E.frames, E.packed, E.cheap, E.phys, buf, phase, gt, nbf,
+                                                 nbp);
+  }
   if (E.heavy.n > 0) {
     Launcher L(E, K_GHOST_HEAVY);
-    const int g = E.ghosts_per_block, n_open = E.heavy.n_open, n_wall = E.heavy.n - n_open;
-    if (n_open > 0) {
-      const int nbf = (E.frames.n + 63) / 64;
-      k_ghost_prep<<<nbf + (E.packed.n + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.frames, E.packed, E.phys, buf,
-                                                                       nbf);
-    }
+    const int g = E.ghosts_per_block, n_wall = E.heavy.n - n_open;
     const size_t lds = (size_t)g * kGhostLds * sizeof(double);
     if (n_open > 0) {
       const int per = g * kOpenPerWave;  // ghosts per workgroup
@@ -1865,6 +1880,16 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   UP(p_slot, lp) E.packed.slot = lp;
   if ((st = dev_alloc(E, &E.packed.out, 6 * (p_sid.empty() ? 1 : p_sid.size())))) return st;
   E.heavy.P = E.packed.out;
+  // the cheap ghosts may share the launch of the frames and packed points
+  // when none of those reads a ghost point
+  E.cheap_with_prep = true;
+  auto fluid_slot = [&](int s, long q) {
+    const Geo& g = E.solves[s].g;
+    const long i = q / g.PY - 4, j = q % g.PY - 4;
+    return i >= 0 && i < g.nx && j >= 0 && j < g.ny && E.solves[s].fluid[i * g.ny + j];
+  };
+  for (size_t e = 0; e < p_sid.size() && E.cheap_with_prep; ++e) E.cheap_with_prep = fluid_slot(p_sid[e], p_slot[e]);
+  for (size_t k = 0; k < f_sid.size() && E.cheap_with_prep; ++k) E.cheap_with_prep = fluid_slot(f_sid[k], f_near[k]);
 #undef UP
   if ((st = build_tables(E))) return st;
   return RH3T_OK;

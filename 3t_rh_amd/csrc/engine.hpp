@@ -179,6 +179,7 @@ struct Engine {
   double* d_partial = nullptr;  // get_local_dt minimum per assembly workgroup
   long partial_cap = 0;
   bool dt_partials_valid = false;  // d_partial holds the minima of the current U0
+  bool cheap_with_prep = false;    // every frame and packed point is a fluid cell
   double* d_aos = nullptr;   // staging (largest solve)
   double* d_gvals = nullptr;
   std::vector<void*> allocs;
