@@ -6,20 +6,28 @@
 // (src/main.rs:135-305):
 //   k_ghost_cheap   Periodic / Reflective / Zeroth / Constant ghosts
 //                   (src/bc1.rs:471-636, 930-1037): gather from a static
-//                   source cell into the ghost's padded slot.
-//   k_ghost_heavy   FarField / Outflow / Wall ghosts (src/bc1.rs:609-928,
-//                   193-255): one wavefront per ghost; the E_r gather, the
-//                   characteristic projection and the S_r quadratic forms are
-//                   spread over the 64 lanes, reductions keep the reference's
-//                   summation order.
+//                   source cell into the ghost's padded slot (it rides in
+//                   k_ghost_prep's launch when a problem has heavy ghosts).
+//   k_ghost_prep    the heavy ghosts' frames (the Roe eigensystem of each
+//                   distinct nearest point and normal, src/bc1.rs:294-312)
+//                   and their E_r points packed 6 doubles per point.
+//   k_ghost_open /  FarField / Outflow and Wall ghosts (src/bc1.rs:609-928,
+//   k_ghost_wall    193-255): two ghosts per wavefront; the E_r projection
+//                   and the S_r / w_r rows spread over a 32-lane group,
+//                   sums in the reference's order; the BC branch (Wall: the
+//                   two ILW systems) and the Taylor sum.
 //   k_derived       Derived::from_state on every fluid cell and ghost
 //                   (src/state.rs:241-275, src/main.rs:149-153).
-//   k_flux<0|1>     characteristic WENO5 interface flux in x / y
+//   k_flux<0>       characteristic WENO5 interface flux in x
 //                   (src/main.rs:165-204 -> src/weno.rs:685-802), one
 //                   interface per lane (flux.hpp).
-//   k_assemble      flux differences + noncon x/y + source + diffusion
-//                   (src/main.rs:251-304) fused with the RK stage combination
-//                   and the admissibility check (src/main.rs:307-349).
+//   k_fy_assemble   the y interface fluxes (as k_flux<1>) with flux
+//                   differences + noncon x/y + source + diffusion
+//                   (src/main.rs:251-304), the RK stage combination and the
+//                   admissibility check (src/main.rs:307-349) in one pass
+//                   (k_flux<1> + k_assemble for launch shapes it does not
+//                   take).
+//   k_capture       the state of a step's first failure (one per step).
 //   k_dt_final      calc_global_dt (src/main.rs:375-408, src/dt.rs:5-34) as
 //                   wavefront fmin reductions: the last RK stage's assembly
 //                   leaves one minimum of get_local_dt of the new state per
@@ -47,10 +55,6 @@ using namespace rh3t_dev;
 using namespace rh3t_eng;
 
 namespace {
-
-constexpr int kStOff[6] = {0, 1, 5, 14, 30, 55};
-constexpr int kSOffD[6] = {0, 1, 17, 98, 354, 979};
-constexpr int kWOffD[6] = {0, 1, 9, 36, 100, 225};
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; give
 // each XCD a contiguous run of tiles so that halo rows shared by
@@ -133,307 +137,8 @@ __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict
 
 // y = R x of the X eigensystem with the entries formed on the fly (one dot6
 // per row, rows in order).
-__device__ __forceinline__ void rx_apply(const Phys& ph, const RoeX& Q, const double x[6], double y[6]) {
-#pragma unroll
-  for (int a = 0; a < 6; ++a) {
-    double r[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) r[c] = rx_entry(ph, Q, a, c);
-    y[a] = dot6(r, x);
-  }
-}
-
-// One 64-lane wavefront per heavy ghost (blockDim = 64 x G ghosts).
-// set_ghost_point_value's fast path (src/bc1.rs:663-928) evaluates
-// weno_extrapolation_pre five times (k = 0..4) on identical data; the
-// projected data and the smoothness indicators beta_r do not depend on k, so
-// they are formed once here (same values, a fifth of the work).
-// WALL = false serves the FarField / Outflow ghosts, true the Wall ghosts
-// (whose ILW solves need the full matrices and would otherwise set the
-// register budget of the common case); the engine keeps the two kinds in
-// separate ranges [h0, h0 + count) of one list.
+// Heavy ghosts per workgroup: at most this many wavefronts (two ghosts each)
 constexpr int kMaxGhostsPerBlock = 4;
-constexpr int kGhostLds = 336 + 336 + 150 + 30 + 30 + 20;  // doubles per ghost
-static_assert(sizeof(RoeX) <= 20 * sizeof(double), "RoeX slot of the ghost LDS");
-
-template <bool WALL>
-__device__ __forceinline__ void ghost_heavy(const SolveDesc* __restrict__ descs, const HeavyGhosts& hg,
-                                            const Phys& ph, int buf, unsigned phase, int h0, int count) {
-  // the ghost index is uniform over the wavefront: per-ghost scalars come
-  // through scalar loads
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int hl = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * blockDim.y) + w;
-  const int lane = threadIdx.x;
-  // per-ghost LDS (dynamic, kGhostLds doubles per wavefront):
-  //   data[6][56]  L . u6 at the 55 E_r points
-  //   part[6][56]  data[c][a] * (S_r data[c])[a], a by E_r row
-  //   vk[5][6][5]  [k][c][r];  alpha[6][5]  [c][r];  v[5][6]  V^(k) per characteristic
-  //   Roe scalars of the nearest state
-  extern __shared__ double s_dyn[];
-  double* const s_data = s_dyn + (long)w * kGhostLds;
-  double* const s_part = s_data + 336;
-  double* const s_vk = s_part + 336;
-  double* const s_alpha = s_vk + 150;
-  double* const s_v = s_alpha + 30;
-  RoeX* const s_q = (RoeX*)(s_v + 30);
-  const bool active = hl < count;
-  const int h = h0 + (active ? hl : 0);
-  const long row = hg.tab[h];
-  const double d5r = lane < 30 ? hg.d5[5L * row + lane / 6] : 0.0;
-  const SolveDesc& sd = descs[hg.sid[h]];
-  double* S = sd.U[buf];  // flat: measured faster here than global
-  const long plane = sd.g.plane;
-  const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
-  // build_local_plain, src/bc1.rs:294-312: rotate the nearest state to the
-  // (n, t) frame; X-direction eigenvectors of the Roe average of that state
-  // with itself.
-  RoeX Q;
-  {
-    double val[6];
-    const long q = hg.nearest[h];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) val[f] = S[f * plane + q];
-    double s1[6] = {val[0], val[1] * nx + val[2] * ny, -val[1] * ny + val[2] * nx, val[3], val[4], val[5]};
-    double d1[D_N];
-    derived_from_state(ph, s1, d1, false);
-    roe_x(ph, s1[0], s1[0], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI], d1[D_ER], d1[D_U], d1[D_V], d1[D_EE], d1[D_EI],
-          d1[D_ER], Q);
-  }
-  // lanes 0..54: L . u at the E_r points; lane 55 of a FarField ghost:
-  // L . u_inf (src/bc1.rs:829-924) into the free slot 55 of each row
-  const bool far = !WALL && hg.kind[h] == RH3T_BC_FARFIELD;
-  if (active && (lane < 55 || (lane == 55 && far))) {
-    double s[6];
-    if (lane < 55) {
-      const long q = hg.stencil[55L * row + lane];
-#pragma unroll
-      for (int f = 0; f < 6; ++f) s[f] = S[f * plane + q];
-    } else {
-#pragma unroll
-      for (int f = 0; f < 6; ++f) s[f] = hg.bcstate[6L * h + f];
-    }
-    double u6[6] = {s[0], s[1] * nx + s[2] * ny, -s[1] * ny + s[2] * nx, s[3], s[4], s[5]};
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      double lr[6];
-      lx_row(ph, Q, c, lr);
-      s_data[(c) * 56 + lane] = dot6(lr, u6);
-    }
-  }
-  if (active && lane == 0) *s_q = Q;
-  __syncthreads();
-  if (active) {
-    // One lane per table row applies it to all six characteristic data
-    // vectors: lanes 0..54 row a of S_r (the 55 E_r points), lanes 55..63
-    // row k of w_r for r = 4 (k = 0..4) and r = 3 (k = 0..3), all in one
-    // loop; then lanes 0..5 the six rows of w_r, r <= 2.  Every table entry
-    // is read once per ghost.  Sums run over b / a in increasing order, as
-    // in weno_extrapolation_pre.
-    const double* Sh = hg.S + 979L * row;
-    const double* Wh = hg.W + 225L * row;
-    auto apply_row = [&](const double* trow, int m, const double* dat, double acc[6]) {
-      // chunks of 5 loads in flight; the sum still runs in order
-      for (int b0 = 0; b0 < m; b0 += 5) {
-        double tv[5];
-#pragma unroll
-        for (int t = 0; t < 5; ++t) tv[t] = b0 + t < m ? trow[b0 + t] : 0.0;
-#pragma unroll
-        for (int t = 0; t < 5; ++t)
-          if (b0 + t < m)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) acc[c] += tv[t] * dat[c * 56 + b0 + t];
-      }
-    };
-    {
-      int r, k = 0;
-      const double* trow;
-      if (lane < 55) {
-        r = 0;
-        while (lane >= kStOff[r + 1]) ++r;
-        trow = Sh + kSOffD[r] + (lane - kStOff[r]) * (r + 1) * (r + 1);
-      } else {
-        const int q = lane - 55;
-        r = q < 5 ? 4 : 3;
-        k = q < 5 ? q : q - 5;
-        trow = Wh + kWOffD[r] + k * (r + 1) * (r + 1);
-      }
-      double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      apply_row(trow, (r + 1) * (r + 1), s_data + kStOff[r], acc);
-      if (lane < 55) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) s_part[(c) * 56 + lane] = s_data[(c) * 56 + lane] * acc[c];
-      } else {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) s_vk[((k) * 6 + c) * 5 + r] = acc[c];
-      }
-    }
-    if (lane < 6) {  // (r, k) = (2,0) (2,1) (2,2) (1,0) (1,1) (0,0)
-      const int r = lane < 3 ? 2 : (lane < 5 ? 1 : 0);
-      const int k = lane < 3 ? lane : (lane < 5 ? lane - 3 : 0);
-      double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-      apply_row(Wh + kWOffD[r] + k * (r + 1) * (r + 1), (r + 1) * (r + 1), s_data + kStOff[r], acc);
-#pragma unroll
-      for (int c = 0; c < 6; ++c) s_vk[((k) * 6 + c) * 5 + r] = acc[c];
-    }
-    // v_k = 0 for k > r (weno_extrapolation_pre)
-    if (lane >= 6 && lane < 16) {
-      const int kr[10][2] = {{1, 0}, {2, 0}, {3, 0}, {4, 0}, {2, 1}, {3, 1}, {4, 1}, {3, 2}, {4, 2}, {4, 3}};
-      const int kz = kr[lane - 6][0], rz = kr[lane - 6][1];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) s_vk[((kz) * 6 + c) * 5 + rz] = 0.0;
-    }
-  }
-  __syncthreads();
-  if (active && lane < 30) {  // beta_r and alpha_r = d_r / (eps + beta_r)^q
-    int r = lane / 6, c = lane % 6;
-    int m = (r + 1) * (r + 1);
-    const double* p = &s_part[(c) * 56 + kStOff[r]];
-    double beta = 0.0;
-    for (int a = 0; a < m; ++a) beta += p[a];
-    s_alpha[(c) * 5 + r] = d5r / weno_pow(ph.eps + beta, ph.weno_q);
-  }
-  __syncthreads();
-  if (active && lane < 30) {  // V^(k)_c = sum_r (alpha_r / sum alpha) v_k,r
-    int k = lane / 6, c = lane % 6;
-    double asum = 0.0;
-#pragma unroll
-    for (int r = 0; r < 5; ++r) asum += s_alpha[(c) * 5 + r];
-    double res = 0.0;
-#pragma unroll
-    for (int r = 0; r < 5; ++r) res += (s_alpha[(c) * 5 + r] / asum) * s_vk[((k) * 6 + c) * 5 + r];
-    s_v[(k) * 6 + c] = res;
-  }
-  __syncthreads();
-  const RoeX Qs = *s_q;
-  const double dist = hg.dist[h];
-  if (!WALL) {
-    // FarField, src/bc1.rs:829-924: incoming characteristics of u^(0) from
-    // u_inf, of u^(k >= 1) zero.  Outflow, src/bc1.rs:760-828: the u - c
-    // characteristic of u^(1) from the pressure relaxation when incoming.
-    // u^(k) = R rhs_k one lane per (k, component), then the Taylor
-    // expansion to the ghost point, sum_k dist^k / k! u^(k), one lane per
-    // component (the LDS of vk is free again).
-    double* const s_uk = s_vk;       // [k][i]
-    double* const s_ug = s_vk + 30;  // Taylor sums
-    double* const s_fix = s_vk + 42; // Outflow: flag, rhs_1[0]
-    // FarField: L u_inf is slot 55 of the data rows (formed with the E_r
-    // data); Outflow: u^(0) = R V^(0), one lane per component
-    if (active && lane < 6 && !far) {
-      double r[6], x[6];
-      rx_row_rt(ph, Qs, lane, r);
-#pragma unroll
-      for (int c = 0; c < 6; ++c) x[c] = s_v[c];
-      s_uk[lane] = dot6(r, x);
-    }
-    __syncthreads();
-    if (active && lane == 0 && !far) {
-      const double* uk = s_uk;
-      double rho0 = uk[0];
-      double un0 = uk[1] / rho0, ut0 = uk[2] / rho0;
-      double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
-      double ee1 = uk[3] / rho0 - w2 / 6.0;
-      double ei1 = uk[4] / rho0 - w2 / 6.0;
-      double er1 = uk[5] / rho0 - w2 / 6.0;
-      double p0 = rho0 * (ph.ge * ee1 + ph.gi * ei1 + ph.gr * er1);
-      double a2 = ph.gamma_e * ph.ge * ee1 + ph.gamma_i * ph.gi * ei1 + ph.gamma_r * ph.gr * er1;
-      double a = sqrt(a2);
-      double lambda1 = un0 - a;
-      s_fix[0] = 0.0;
-      if (lambda1 < 0.0) {
-        const double p_inf = hg.outflow[3L * h], sigma = hg.outflow[3L * h + 1], l_dom = hg.outflow[3L * h + 2];
-        double ml = fabs(un0 / a);
-        double l1 = sigma * (1.0 - sq(ml)) * a / l_dom * (p0 - p_inf);
-        s_fix[0] = 1.0;
-        s_fix[1] = l1 / lambda1;
-      }
-    }
-    __syncthreads();
-    if (active && lane < 30 && (far || lane >= 6)) {
-      const int k = lane / 6, i = lane % 6;
-      double rhs[6], r[6];
-#pragma unroll
-      for (int m = 0; m < 6; ++m) rhs[m] = s_v[(k) * 6 + m];
-      if (far) {
-#pragma unroll
-        for (int m = 0; m < 6; ++m)
-          if (lamx(Qs, m) < 0.0) rhs[m] = k == 0 ? s_data[(m) * 56 + 55] : 0.0;
-      } else if (k == 1 && s_fix[0] != 0.0) {
-        rhs[0] = s_fix[1];
-      }
-      rx_row_rt(ph, Qs, i, r);
-      s_uk[(k) * 6 + i] = dot6(r, rhs);
-    }
-    __syncthreads();
-    if (active && lane < 6) {
-      double ug = s_uk[lane];
-      double coef = 1.0;
-      for (int k = 1; k <= 4; ++k) {
-        coef *= dist / (double)k;
-        ug = ug + coef * s_uk[(k) * 6 + lane];
-      }
-      s_ug[lane] = ug;
-    }
-    __syncthreads();
-    if (active && lane < 6) {  // rotation back to (x, y)
-      const int f = lane;
-      const double out = f == 1 ? s_ug[1] * nx - s_ug[2] * ny : (f == 2 ? s_ug[1] * ny + s_ug[2] * nx : s_ug[f]);
-      S[f * plane + hg.pad[h]] = out;
-      if (!isfinite(out)) raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
-    }
-    return;
-  }
-  if (!active || lane != 0) return;
-  // Taylor expansion to the ghost point, sum_k dist^k / k! u^(k)
-  double ug[6];
-  bool ok = true, solved = true;
-  {  // Wall-ILW, src/bc1.rs:691-756
-    double uk[5][6];
-    double A[6][6], rhs[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) lx_row(ph, Qs, c, A[c]);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) A[0][c] = c == 1 ? 1.0 : 0.0;
-#pragma unroll
-    for (int m = 0; m < 6; ++m) rhs[m] = s_v[(0) * 6 + m];
-    rhs[0] = 0.0;
-    solved = solve6(A, rhs, uk[0]);
-    double u1 = uk[0][0], u3 = uk[0][2];
-    A[0][0] = -ph.gt * sq(u3) / (6.0 * sq(u1));  // build_ilw_row1, src/bc1.rs:448-465
-    A[0][1] = 0.0;
-    A[0][2] = ph.gt * u3 / (6.0 * u1);
-    A[0][3] = -ph.ge;
-    A[0][4] = -ph.gi;
-    A[0][5] = -ph.gr;
-#pragma unroll
-    for (int m = 0; m < 6; ++m) rhs[m] = s_v[(1) * 6 + m];
-    rhs[0] = 0.0;
-    solved = solved && solve6(A, rhs, uk[1]);
-    // u^(k) = R V^(k), k >= 2
-#pragma unroll
-    for (int k = 2; k <= 4; ++k) rx_apply(ph, Qs, s_v + (k) * 6, uk[k]);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) ug[c] = uk[0][c];
-    double coef = 1.0;
-#pragma unroll
-    for (int k = 1; k <= 4; ++k) {
-      coef *= dist / (double)k;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) ug[c] = ug[c] + coef * uk[k][c];
-    }
-  }
-  // rotation back to (x, y)
-  const long dst = hg.pad[h];
-  double out[6] = {ug[0], ug[1] * nx - ug[2] * ny, ug[1] * ny + ug[2] * nx, ug[3], ug[4], ug[5]};
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    S[f * plane + dst] = out[f];
-    ok = ok && isfinite(out[f]);
-  }
-  if (!solved)
-    raise_error(sd.sc, phase, RH3T_ERR_SINGULAR, (unsigned)hg.gid[h]);
-  else if (!ok)
-    raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
-}
 
 // One frame: build_local_plain (src/bc1.rs:294-312), the nearest
 // state rotated to the (n, t) frame and the X-direction eigensystem of its
@@ -524,12 +229,15 @@ constexpr int kOpenLG = 64 / kOpenPerWave;
 // put rows c and c + 2 on the same LDS banks
 constexpr int kRow = 57;
 constexpr int kOpenLds = 6 * kRow + 90 + kFrameLen;  // doubles per ghost
+// Wall ghosts keep the L rows for their ILW systems: alpha gets its own 30
+constexpr int grp_lds(bool wall) { return kOpenLds + (wall ? 30 : 0); }
 static_assert(kOpenLG == 32, "the swizzled table schedule is for 32-lane groups");
 // slot of v_k,r (k <= r; v_k,r = 0 for k > r is never stored)
 __device__ __forceinline__ int vk_slot(int k, int r) { return r * (r + 1) / 2 + k; }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ghost_open(
-    const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
+template <bool WALL>
+__device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, const HeavyGhosts& hg,
+                                          const Phys& ph, int buf, unsigned phase, int h0, int count) {
   constexpr int LG = kOpenLG;
   const int lane = threadIdx.x;
   const int grp = lane / LG, l = lane % LG;
@@ -543,11 +251,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // normalised weights om[6][5] and V^(k) v[5][6] (slot 55 of each row is
   // kept).  504 doubles per ghost.
   extern __shared__ double s_dyn[];
-  double* const s_data = s_dyn + ((long)w * kOpenPerWave + grp) * kOpenLds;
+  double* const s_data = s_dyn + ((long)w * kOpenPerWave + grp) * grp_lds(WALL);
   double* const s_part = s_data;
   double* const s_vk = s_data + 6 * kRow;
   double* const s_F = s_vk + 90;
-  double* const s_alpha = s_F;
+  double* const s_alpha = WALL ? s_F + kFrameLen : s_F;  // Wall keeps the L rows
   double* const s_v = s_data;
   double* const s_om = s_data + 60;
   const bool active = hl < count;
@@ -557,7 +265,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   double* S = sd.U[buf];
   const long plane = sd.g.plane;
   const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
-  const bool far = hg.kind[h] == RH3T_BC_FARFIELD;
+  const bool far = !WALL && hg.kind[h] == RH3T_BC_FARFIELD;
   // The ghost's loads are issued together: the state gathers at its E_r
   // points (packed by k_ghost_prep: 3 x 16 bytes per point), its frame's
   // eigensystem (k_ghost_prep) and the table entries of the first
@@ -732,64 +440,112 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   }
   __syncthreads();
   const double dist = hg.dist[h];
-  // FarField, src/bc1.rs:829-924: incoming characteristics of u^(0) from
-  // u_inf, of u^(k >= 1) zero.  Outflow, src/bc1.rs:760-828: the u - c
-  // characteristic of u^(1) from the pressure relaxation when incoming.
   double* const s_uk = s_vk;        // [k][i]
   double* const s_ug = s_vk + 30;   // Taylor sums
-  double* const s_fix = s_vk + 42;  // Outflow: flag, rhs_1[0]
-  if (active && l < 6 && !far) {    // Outflow: u^(0) = R V^(0)
-    double r[6], x[6];
+  double* const s_fix = s_vk + 42;  // Outflow: flag, rhs_1[0]; Wall: solved
+  if constexpr (WALL) {
+    // Wall-ILW, src/bc1.rs:691-756: u^(0) from L with row 0 replaced by the
+    // zero normal momentum, u^(1) from L with row 0 the ILW row of u^(0)
+    // (build_ilw_row1, src/bc1.rs:448-465), both right-hand sides V^(k)
+    // with entry 0 set to 0 (lane 0 of the group, two solve6); u^(k) = R
+    // V^(k) for k >= 2 on the other lanes
+    if (active && l == 0) {
+      double A[6][6], rhs[6], u0[6], u1[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      r[c] = s_F[36 + 6 * l + c];
-      x[c] = s_v[c];
-    }
-    s_uk[l] = dot6(r, x);
-  }
-  __syncthreads();
-  if (active && l == 0 && !far) {
-    const double* uk = s_uk;
-    double rho0 = uk[0];
-    double un0 = uk[1] / rho0, ut0 = uk[2] / rho0;
-    double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
-    double ee1 = uk[3] / rho0 - w2 / 6.0;
-    double ei1 = uk[4] / rho0 - w2 / 6.0;
-    double er1 = uk[5] / rho0 - w2 / 6.0;
-    double p0 = rho0 * (ph.ge * ee1 + ph.gi * ei1 + ph.gr * er1);
-    double a2 = ph.gamma_e * ph.ge * ee1 + ph.gamma_i * ph.gi * ei1 + ph.gamma_r * ph.gr * er1;
-    double a = sqrt(a2);
-    double lambda1 = un0 - a;
-    s_fix[0] = 0.0;
-    if (lambda1 < 0.0) {
-      const double p_inf = hg.outflow[3L * h], sigma = hg.outflow[3L * h + 1], l_dom = hg.outflow[3L * h + 2];
-      double ml = fabs(un0 / a);
-      double l1 = sigma * (1.0 - sq(ml)) * a / l_dom * (p0 - p_inf);
-      s_fix[0] = 1.0;
-      s_fix[1] = l1 / lambda1;
-    }
-  }
-  __syncthreads();
-  if (active) {  // u^(k) = R rhs_k, task (k, i) (u^(0) of Outflow done)
-    for (int t = l; t < 30; t += LG) {
-      if (!far && t < 6) continue;
-      const int k = t / 6, i = t % 6;
-      double rhs[6], r[6];
+      for (int c = 0; c < 6; ++c)
 #pragma unroll
-      for (int m = 0; m < 6; ++m) rhs[m] = s_v[k * 6 + m];
-      if (far) {
+        for (int m = 0; m < 6; ++m) A[c][m] = s_F[6 * c + m];
 #pragma unroll
-        for (int m = 0; m < 6; ++m)
-          if (s_F[72 + m] < 0.0) rhs[m] = k == 0 ? s_data[m * kRow + 55] : 0.0;
-      } else if (k == 1 && s_fix[0] != 0.0) {
-        rhs[0] = s_fix[1];
+      for (int c = 0; c < 6; ++c) A[0][c] = c == 1 ? 1.0 : 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) rhs[m] = s_v[m];
+      rhs[0] = 0.0;
+      bool solved = solve6(A, rhs, u0);
+      const double r0 = u0[0], m3 = u0[2];
+      A[0][0] = -ph.gt * sq(m3) / (6.0 * sq(r0));
+      A[0][1] = 0.0;
+      A[0][2] = ph.gt * m3 / (6.0 * r0);
+      A[0][3] = -ph.ge;
+      A[0][4] = -ph.gi;
+      A[0][5] = -ph.gr;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) rhs[m] = s_v[6 + m];
+      rhs[0] = 0.0;
+      solved = solved && solve6(A, rhs, u1);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) {
+        s_uk[m] = u0[m];
+        s_uk[6 + m] = u1[m];
       }
-#pragma unroll
-      for (int m = 0; m < 6; ++m) r[m] = s_F[36 + 6 * i + m];
-      s_uk[k * 6 + i] = dot6(r, rhs);
+      s_fix[0] = solved ? 1.0 : 0.0;
     }
+    if (active) {
+      for (int t = 12 + l; t < 30; t += LG) {
+        const int k = t / 6, i = t % 6;
+        double r[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) r[m] = s_F[36 + 6 * i + m];
+        s_uk[k * 6 + i] = dot6(r, s_v + k * 6);
+      }
+    }
+    __syncthreads();
+  } else {
+    // FarField, src/bc1.rs:829-924: incoming characteristics of u^(0) from
+    // u_inf, of u^(k >= 1) zero.  Outflow, src/bc1.rs:760-828: the u - c
+    // characteristic of u^(1) from the pressure relaxation when incoming.
+    if (active && l < 6 && !far) {    // Outflow: u^(0) = R V^(0)
+      double r[6], x[6];
+  #pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        r[c] = s_F[36 + 6 * l + c];
+        x[c] = s_v[c];
+      }
+      s_uk[l] = dot6(r, x);
+    }
+    __syncthreads();
+    if (active && l == 0 && !far) {
+      const double* uk = s_uk;
+      double rho0 = uk[0];
+      double un0 = uk[1] / rho0, ut0 = uk[2] / rho0;
+      double w2 = un0 * un0 + ut0 * ut0;  // local_pressure_and_a2, src/bc1.rs:424-440
+      double ee1 = uk[3] / rho0 - w2 / 6.0;
+      double ei1 = uk[4] / rho0 - w2 / 6.0;
+      double er1 = uk[5] / rho0 - w2 / 6.0;
+      double p0 = rho0 * (ph.ge * ee1 + ph.gi * ei1 + ph.gr * er1);
+      double a2 = ph.gamma_e * ph.ge * ee1 + ph.gamma_i * ph.gi * ei1 + ph.gamma_r * ph.gr * er1;
+      double a = sqrt(a2);
+      double lambda1 = un0 - a;
+      s_fix[0] = 0.0;
+      if (lambda1 < 0.0) {
+        const double p_inf = hg.outflow[3L * h], sigma = hg.outflow[3L * h + 1], l_dom = hg.outflow[3L * h + 2];
+        double ml = fabs(un0 / a);
+        double l1 = sigma * (1.0 - sq(ml)) * a / l_dom * (p0 - p_inf);
+        s_fix[0] = 1.0;
+        s_fix[1] = l1 / lambda1;
+      }
+    }
+    __syncthreads();
+    if (active) {  // u^(k) = R rhs_k, task (k, i) (u^(0) of Outflow done)
+      for (int t = l; t < 30; t += LG) {
+        if (!far && t < 6) continue;
+        const int k = t / 6, i = t % 6;
+        double rhs[6], r[6];
+  #pragma unroll
+        for (int m = 0; m < 6; ++m) rhs[m] = s_v[k * 6 + m];
+        if (far) {
+  #pragma unroll
+          for (int m = 0; m < 6; ++m)
+            if (s_F[72 + m] < 0.0) rhs[m] = k == 0 ? s_data[m * kRow + 55] : 0.0;
+        } else if (k == 1 && s_fix[0] != 0.0) {
+          rhs[0] = s_fix[1];
+        }
+  #pragma unroll
+        for (int m = 0; m < 6; ++m) r[m] = s_F[36 + 6 * i + m];
+        s_uk[k * 6 + i] = dot6(r, rhs);
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
   if (active && l < 6) {  // Taylor expansion to the ghost point
     double ug = s_uk[l];
     double coef = 1.0;
@@ -804,14 +560,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int f = l;
     const double out = f == 1 ? s_ug[1] * nx - s_ug[2] * ny : (f == 2 ? s_ug[1] * ny + s_ug[2] * nx : s_ug[f]);
     S[f * plane + hg.pad[h]] = out;
-    if (!isfinite(out)) raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
+    if (WALL && s_fix[0] == 0.0) {
+      if (f == 0) raise_error(sd.sc, phase, RH3T_ERR_SINGULAR, (unsigned)hg.gid[h]);
+    } else if (!isfinite(out)) {
+      raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
+    }
   }
 }
 
-// Wall: three wavefronts per SIMD (168 VGPRs, 4 spilled; faster than two)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ghost_open(
+    const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
+  ghost_grp<false>(descs, hg, ph, buf, phase, h0, count);
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ghost_wall(
     const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
-  ghost_heavy<true>(descs, hg, ph, buf, phase, h0, count);
+  ghost_grp<true>(descs, hg, ph, buf, phase, h0, count);
 }
 
 // ----------------------------------------------------------- derived
@@ -1307,12 +1071,12 @@ namespace rh3t_eng {
 
 void launch_ghosts(Engine& E, int buf, unsigned phase, int gt) {
   const int n_open = E.heavy.n_open;
-  const bool with_prep = n_open > 0 && E.cheap_with_prep;
+  const bool with_prep = E.heavy.n > 0 && E.cheap_with_prep;
   if (E.cheap.n > 0 && !with_prep) {
     Launcher L(E, K_GHOST_CHEAP);
     k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.cheap, buf, phase, gt);
   }
-  if (n_open > 0) {
+  if (E.heavy.n > 0) {
     // the frames and packed points, with the cheap ghosts riding along
     // when nothing the first two read is a ghost point
     Launcher L(E, K_GHOST_HEAVY);
@@ -1324,15 +1088,16 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, int gt) {
   if (E.heavy.n > 0) {
     Launcher L(E, K_GHOST_HEAVY);
     const int g = E.ghosts_per_block, n_wall = E.heavy.n - n_open;
-    const size_t lds = (size_t)g * kGhostLds * sizeof(double);
     if (n_open > 0) {
       const int per = g * kOpenPerWave;  // ghosts per workgroup
       k_ghost_open<<<(n_open + per - 1) / per, dim3(64, g), (size_t)per * kOpenLds * sizeof(double), E.stream>>>(
           E.d_desc, E.heavy, E.phys, buf, phase, 0, n_open);
     }
-    if (n_wall > 0)
-      k_ghost_wall<<<(n_wall + g - 1) / g, dim3(64, g), lds, E.stream>>>(E.d_desc, E.heavy, E.phys, buf,
-                                                                                 phase, n_open, n_wall);
+    if (n_wall > 0) {
+      const int per = g * kOpenPerWave;
+      k_ghost_wall<<<(n_wall + per - 1) / per, dim3(64, g), (size_t)per * grp_lds(true) * sizeof(double),
+                     E.stream>>>(E.d_desc, E.heavy, E.phys, buf, phase, n_open, n_wall);
+    }
   }
 }
 
@@ -1783,14 +1548,14 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
     permute(h_state, 6);
     permute(h_of, 3);
   }
-  // FarField / Outflow ghosts of one solve that share a nearest point and a
-  // normal share a frame
+  // heavy ghosts of one solve that share a nearest point and a normal share
+  // a frame
   std::vector<int> h_frame(E.heavy.n, 0), f_sid;
   std::vector<long> f_near;
   std::vector<double> f_n;
   {
     std::map<std::tuple<int, long, uint64_t, uint64_t>, int> fid;
-    for (int k = 0; k < E.heavy.n_open; ++k) {
+    for (int k = 0; k < E.heavy.n; ++k) {
       uint64_t bx, by;
       std::memcpy(&bx, &h_n[2 * (size_t)k], sizeof(bx));
       std::memcpy(&by, &h_n[2 * (size_t)k + 1], sizeof(by));
@@ -1806,14 +1571,13 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
       h_frame[k] = it->second;
     }
   }
-  // the distinct E_r points of the FarField / Outflow ghosts, numbered in
-  // the order the ghosts first read them (the Wall ghosts, one per
-  // wavefront, gather from the planes: packing them measured no faster)
-  std::vector<int> h_gpk((size_t)E.heavy.n_open * 55), p_sid;
+  // the distinct E_r points of the heavy ghosts, numbered in the order the
+  // ghosts first read them
+  std::vector<int> h_gpk((size_t)E.heavy.n * 55), p_sid;
   std::vector<long> p_slot;
   {
     std::map<std::pair<int, long>, int> pid;
-    for (int k = 0; k < E.heavy.n_open; ++k)
+    for (int k = 0; k < E.heavy.n; ++k)
       for (int a = 0; a < 55; ++a) {
         const auto key = std::make_pair(h_sid[k], h_st[55 * (size_t)h_tab[k] + a]);
         auto it = pid.find(key);
