@@ -342,6 +342,14 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
     // data)[a] goes straight into data slot a once no lane reads that E_r
     // segment any more (the group is one wavefront: LDS accesses complete in
     // program order), a w row's sum into vk.
+    // segment B's table entries in flight while segment A runs
+    double tv[kSwzB];
+#pragma unroll
+    for (int q = 0; q < kSwzPairsB; ++q) {
+      const double2 t2 = T[32 * (kSwzPairsA + q)];
+      tv[2 * q] = t2.x;
+      tv[2 * q + 1] = t2.y;
+    }
     {  // segment A: S_4 rows (lanes 0..24), w_4 rows (25..29), S_0 / w_0 (30, 31)
       const int m = l < 30 ? 25 : 1, d0 = l < 30 ? 30 : 0;
       double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -362,13 +370,6 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
     }
     {  // segment B: S_3 / w_3 rows (lanes 0..19) over 16 steps; S_2 / w_2
        // rows (20..31) over the first 9, then S_1 / w_1 rows (20..25) over 4
-      double tv[kSwzB];
-#pragma unroll
-      for (int q = 0; q < kSwzPairsB; ++q) {
-        const double2 t2 = T[32 * (kSwzPairsA + q)];
-        tv[2 * q] = t2.x;
-        tv[2 * q + 1] = t2.y;
-      }
       const int d0 = l < 20 ? 14 : 5;
       double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
