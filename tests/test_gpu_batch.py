@@ -48,6 +48,37 @@ def test_batch_matches_plain_bitwise(rh3t, members, arith):
     b.close()
 
 
+def test_batch_with_wall_ghosts_matches_plain_bitwise(rh3t, workloads):
+    """A batch mixing Wall (ILW) ghosts with FarField ones -- two double Mach
+    problems (h = 1/40; the second a clone, so the two share their tables)
+    and a 121x481 cylinder -- evolves every member exactly as its plain
+    solve (EXACT, 3 steps): the grouped Wall kernel indexes its solves,
+    frames and packed points per member."""
+    _say("batch of 3 (double Mach h=1/40 x2, cylinder 121), 3 steps, exact")
+    dm = rh3t.Field.init_double_mach(1.0 / 40.0)
+    dm2 = dm.clone()
+    cyl = rh3t.Field.init_cylinder(121, 3.0)
+    fields = [dm, cyl, dm2]
+    u_dm = dm.double_mach_state()
+    u_dm2 = u_dm + 0.01 * workloads.smooth_direction(dm, u_dm, 11)
+    u_dm2[~dm.fluid] = u_dm[~dm.fluid]
+    states = [u_dm, cyl.cylinder_state(3.0), u_dm2]
+    b = rh3t.Batch(fields)
+    for k, u in enumerate(states):
+        b.set_state(k, u, 0.0)
+    times = b.advance(3, 0.05)
+    for k, (f, u) in enumerate(zip(fields, states)):
+        s = f.solver()
+        s.set_state(u, 0.0)
+        t = s.advance(3, 0.05)
+        ref, _ = s.get_state()
+        got, tb = b.get_state(k)
+        assert tb == t == times[k]
+        assert np.array_equal(got, ref), (k, np.abs(got - ref).max())
+        s.close()
+    b.close()
+
+
 @pytest.mark.parametrize("shape", [(32, 8, 2), (128, 2, 4), (16, 16, 1)])
 def test_batch_launch_shape_invariance(rh3t, members, shape):
     fields, states = members
