@@ -313,34 +313,56 @@ __device__ __forceinline__ void noncon(const double vel[9], const double pe[9], 
   out[2] = v3 * der_er + div_rn(cl * jump(ner) + cr * jump(ner + 1), ds, yds);
 }
 
-// noncon with the divisions as products: by 3 through 1/3, by ds and 60 ds
-// through ids = 1/ds (the caller's) and ids / 60.  Every sum keeps the
-// order of noncon; each product differs from the quotient by at most an
-// ulp or two of that term.
+// noncon in FAST arithmetic, on the linear structure of its terms (the same
+// algebra; the divisions by 3, ds and 60 ds as products with 1/3 and the
+// caller's ids = 1/ds).
+// * jump(a) = q_pulse(a7..a1) - q_pulse(a0..a6) = sum_k J_k a_k with
+//   antisymmetric weights J_{7-k} = -J_k: J_0 = -Q_0, J_1 = Q_6 - Q_1,
+//   J_2 = Q_5 - Q_2, J_3 = Q_4 - Q_3 (Q the q_pulse coefficients), so the
+//   upwind term cl jump(x) + cr jump(x + 1) of a field x is one combination
+//   sum_m W_m x_m over the 9 points, W_m = cl J_m + cr J_{m-1} (J_{-1} =
+//   J_8 = 0), formed once per cell and line.
+// * jump and the sixth-order derivative are linear, so the combinations
+//   2 p_e - p_i - p_r (and its rotations) are taken after them:
+//   2 y_e - y_i - y_r = 3 y_e - (y_e + y_i + y_r).
+// * the sixth-order derivative (q7 - q1) - 9 (q6 - q2) + 45 (q5 - q3) keeps
+//   its factor 1 / (60 ds) in the velocity factor.
 __device__ __forceinline__ void noncon_fast(const double vel[9], const double pe[9], const double pi[9],
                                             const double pr[9], double ids, double out[3]) {
-  const double third = 1.0 / 3.0, i60 = ids * (1.0 / 60.0);
+  constexpr double Q0 = -0.00714285714285723, Q1 = 0.0595238095238118, Q2 = -0.240476190476197,
+                   Q3 = 0.759523809523819, Q4 = 0.509523809523803, Q5 = -0.0904761904761891,
+                   Q6 = 0.00952380952380977;
+  constexpr double J0 = -Q0, J1 = Q6 - Q1, J2 = Q5 - Q2, J3 = Q4 - Q3;
   const double v = vel[4];
-  auto sixth_f = [&](const double q[9]) {
-    return (q[7] - 9.0 * q[6] + 45.0 * q[5] - 45.0 * q[3] + 9.0 * q[2] - q[1]) * i60;
-  };
-  const double dpe = sixth_f(pe), dpi = sixth_f(pi), dpr = sixth_f(pr);
-  const double der_ee = 2.0 * dpe - dpi - dpr;
-  const double der_ei = 2.0 * dpi - dpe - dpr;
-  const double der_er = 2.0 * dpr - dpe - dpi;
-  double nee[9], nei[9], ner[9];
+  const double i3 = ids * (1.0 / 3.0);
+  const double cl = fmax(fmax(v, vel[3]), 0.0) * i3;  // cl / ds
+  const double cr = fmin(fmin(v, vel[5]), 0.0) * i3;  // cr / ds
+  const double vs = v * (ids * (1.0 / 180.0));         // (v / 3) / (60 ds)
+  double W[9];
+  W[0] = cl * J0;
+  W[1] = fma(cl, J1, cr * J0);
+  W[2] = fma(cl, J2, cr * J1);
+  W[3] = fma(cl, J3, cr * J2);
+  W[4] = (cr - cl) * J3;
+  W[5] = fma(-cl, J2, -cr * J3);
+  W[6] = fma(-cl, J1, -cr * J2);
+  W[7] = fma(-cl, J0, -cr * J1);
+  W[8] = -cr * J0;
+  auto upwind = [&](const double x[9]) {
+    double acc = W[0] * x[0];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    nee[k] = 2.0 * pe[k] - pi[k] - pr[k];
-    nei[k] = 2.0 * pi[k] - pe[k] - pr[k];
-    ner[k] = 2.0 * pr[k] - pe[k] - pi[k];
-  }
-  const double cl = fmax(fmax(v, vel[3]), 0.0) * third;
-  const double cr = fmin(fmin(v, vel[5]), 0.0) * third;
-  const double v3 = v * third;
-  out[0] = v3 * der_ee + (cl * jump(nee) + cr * jump(nee + 1)) * ids;
-  out[1] = v3 * der_ei + (cl * jump(nei) + cr * jump(nei + 1)) * ids;
-  out[2] = v3 * der_er + (cl * jump(ner) + cr * jump(ner + 1)) * ids;
+    for (int m = 1; m < 9; ++m) acc = fma(W[m], x[m], acc);
+    return acc;
+  };
+  auto sixth_u = [](const double q[9]) {
+    return fma(45.0, q[5] - q[3], fma(-9.0, q[6] - q[2], q[7] - q[1]));
+  };
+  const double ye = upwind(pe), yi = upwind(pi), yr = upwind(pr);
+  const double se = sixth_u(pe), si = sixth_u(pi), sr = sixth_u(pr);
+  const double S = ye + yi + yr, D = se + si + sr;
+  out[0] = fma(vs, fma(3.0, se, -D), fma(3.0, ye, -S));
+  out[1] = fma(vs, fma(3.0, si, -D), fma(3.0, yi, -S));
+  out[2] = fma(vs, fma(3.0, sr, -D), fma(3.0, yr, -S));
 }
 
 // build_diffusion_from_derived, src/diffusion.rs:27-50: t[0..6) are the

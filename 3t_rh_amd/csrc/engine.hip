@@ -624,8 +624,9 @@ __device__ __forceinline__ void iface_flux(const SolveDesc& sd, const EigConst* 
     for (int f = 0; f < 6; ++f) out[f] = 0.0;
     return;
   }
-  // 6-point stencils of U and of the physical flux (flux_from_derived,
-  // src/state.rs:56-85).  In x, each value is read as a workgroup-uniform
+  // 6-point stencil of U with each point's physical flux (EXACT,
+  // flux_from_derived, src/state.rs:56-85) or its normal velocity and
+  // pressures (FAST; see Stencil).  In x, each value is read as a workgroup-uniform
   // base (plane and stencil row, formed in scalar registers) plus this
   // lane's 32-bit byte offset: one per-lane address for all the loads.  In
   // y, plain indexing lets the compiler merge neighbouring points into
@@ -634,26 +635,35 @@ __device__ __forceinline__ void iface_flux(const SolveDesc& sd, const EigConst* 
   auto at = [&](const double* plane, long q_off) {
     return DIR == 0 ? plane_at(glb(plane + q_off), boff) : plane[c0 + q_off];
   };
-  double U[6][6], Fl[6][6];
+  Stencil<FAST> P;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     const long qo = (q - 3) * step;
+    double* Uq = P.U[q];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) U[q][f] = at(S + f * g.plane, qo);
+    for (int f = 0; f < 6; ++f) Uq[f] = at(S + f * g.plane, qo);
     const double u = at(D + D_U * g.plane, qo), v = at(D + D_V * g.plane, qo);
     const double pe = at(D + D_PE * g.plane, qo), pi = at(D + D_PI * g.plane, qo), pr = at(D + D_PR * g.plane, qo);
     const double ptot = pe + pi + pr;
     const double w = DIR == 0 ? u : v;
-    Fl[q][0] = U[q][0] * w;
-    Fl[q][1] = DIR == 0 ? U[q][0] * u * u + ptot : U[q][0] * u * v;
-    Fl[q][2] = DIR == 0 ? U[q][0] * u * v : U[q][0] * v * v + ptot;
-    Fl[q][3] = (U[q][3] + pe) * w;
-    Fl[q][4] = (U[q][4] + pi) * w;
-    Fl[q][5] = (U[q][5] + pr) * w;
+    if constexpr (FAST) {
+      P.w[q] = w;
+      P.pe[q] = pe;
+      P.pi[q] = pi;
+      P.pr[q] = pr;
+      P.p[q] = ptot;
+    } else {
+      P.F[q][0] = Uq[0] * w;
+      P.F[q][1] = DIR == 0 ? Uq[0] * u * u + ptot : Uq[0] * u * v;
+      P.F[q][2] = DIR == 0 ? Uq[0] * u * v : Uq[0] * v * v + ptot;
+      P.F[q][3] = (Uq[3] + pe) * w;
+      P.F[q][4] = (Uq[4] + pi) * w;
+      P.F[q][5] = (Uq[5] + pr) * w;
+    }
   }
   const double e3[3] = {at(D + D_EE * g.plane, 0), at(D + D_EI * g.plane, 0), at(D + D_ER * g.plane, 0)};
   const double e2[3] = {at(D + D_EE * g.plane, -step), at(D + D_EI * g.plane, -step), at(D + D_ER * g.plane, -step)};
-  interface_flux<DIR, FAST>(Kp, eps, U, Fl, at(D + D_U * g.plane, 0), at(D + D_V * g.plane, 0), e3,
+  interface_flux<DIR, FAST>(Kp, eps, P, at(D + D_U * g.plane, 0), at(D + D_V * g.plane, 0), e3,
                             at(D + D_U * g.plane, -step), at(D + D_V * g.plane, -step), e2, out);
 }
 

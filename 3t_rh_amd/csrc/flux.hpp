@@ -1,8 +1,9 @@
 // Characteristic WENO5 interface flux, Stencil6::reconstruction_fast of
 // colinhu123/3T_RH (src/weno.rs:685-802), for one interface per lane.
 //
-// Register plan: the 6-point stencils of U and of the physical flux F stay
-// in VGPRs (72 doubles); the Roe state is reduced to ~16 scalars and each row
+// Register plan: the 6-point stencil of U and each point's physical flux F
+// (EXACT) or normal velocity and pressures (FAST) stay in VGPRs (72 or 66
+// doubles); the Roe state is reduced to ~16 scalars and each row
 // of L / column of R (src/weno.rs:28-548) is generated when its
 // characteristic field is processed, so the 6x6 matrices never exist as a
 // whole.  Uniform subexpressions of the physics constants are evaluated once
@@ -12,8 +13,9 @@
 //   Exact  (FAST = false): the reference's rounding sequence -- literal
 //          divisions, left-to-right dot products, no FMA contraction.
 //   Fast   (FAST = true): one reciprocal per shared denominator, FMA
-//          chains, and a single division per WENO5 evaluation.  Same
-//          algebra, different rounding.
+//          chains, WENO5 on first differences with one refined reciprocal,
+//          and the flux projections formed from the state projections.
+//          Same algebra, different rounding.
 #pragma once
 
 #include "device_math.hpp"
@@ -244,28 +246,48 @@ __device__ __forceinline__ double r_entry(const EigConst& K, const Roe<DIR, FAST
   return (j == (k + 1) % 3) ? K.g[m] : -K.g[m];
 }
 
-// weno5, src/weno.rs:846-886
+// 1/x from v_rcp_f64 (relative error up to 2^-24.4 on gfx950) refined by
+// RH3T_RCP_NEWTON Newton steps: one step leaves at most 2.3e-15 (about
+// 10 ulp), two give the correctly rounded reciprocal.  FAST only.
+#ifndef RH3T_RCP_NEWTON
+#define RH3T_RCP_NEWTON 1
+#endif
+__device__ __forceinline__ double rcp_refined(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int k = 0; k < RH3T_RCP_NEWTON; ++k) r = fma(fma(-x, r, 1.0), r, r);
+  return r;
+}
+
+// weno5, src/weno.rs:846-886.
+// EXACT: the reference's expression (weno5 of device_math.hpp).
+// FAST: the same weights and candidate values written on the first
+// differences a = u1 - u0, b = u2 - u1, c = u3 - u2, dd = u4 - u3.  The
+// smoothness indicators 13/12 d_k^2 + 1/4 e_k^2 (d_k the second difference,
+// e_k the one-sided first difference of candidate k) are the quadratic forms
+//   3 beta_0 = 4a^2 - 11ab + 10b^2,
+//   3 beta_1 = 4b^2 - 5bc + 4c^2,
+//   3 beta_2 = 10c^2 - 11c dd + 4dd^2,
+// and the candidates are p_k = u2 + r_k / 6 with r = (5b - 2a, b + 2c,
+// 4c - dd), so weno5 = u2 + sum_k t_k r_k / (6 sum_k t_k) with
+// t_k = d_k prod_{j!=k} s_j and s_k = (eps3 + 3 beta_k)^2: the caller passes
+// eps3 = 3 eps, which leaves the normalised weights as they are.
 template <bool FAST>
 __device__ __forceinline__ double weno5v(double u0, double u1, double u2, double u3, double u4, double eps) {
   if (!FAST) return weno5(u0, u1, u2, u3, u4, eps);
-  const double d2 = u2 - 2.0 * u3 + u4, e2 = fma(3.0, u2, fma(-4.0, u3, u4));
-  const double d1 = u1 - 2.0 * u2 + u3, e1 = u1 - u3;
-  const double d0 = u0 - 2.0 * u1 + u2, e0 = fma(-4.0, u1, u0) + 3.0 * u2;
-  const double c13 = 13.0 / 12.0;
-  const double beta2 = fma(c13 * d2, d2, 0.25 * e2 * e2);
-  const double beta1 = fma(c13 * d1, d1, 0.25 * e1 * e1);
-  const double beta0 = fma(c13 * d0, d0, 0.25 * e0 * e0);
-  double s0 = eps + beta0, s1 = eps + beta1, s2 = eps + beta2;
+  const double a = u1 - u0, b = u2 - u1, c = u3 - u2, dd = u4 - u3;
+  const double bb = b * b, cc = c * c;
+  double s0 = eps + fma(a, fma(4.0, a, -11.0 * b), 10.0 * bb);
+  double s1 = eps + fma(-5.0 * b, c, 4.0 * (bb + cc));
+  double s2 = eps + fma(dd, fma(4.0, dd, -11.0 * c), 10.0 * cc);
   s0 *= s0;
   s1 *= s1;
   s2 *= s2;
-  // w_k = (d_k / s_k) / sum_m (d_m / s_m), with all three over one division
-  const double t0 = 0.1 * s1 * s2, t1 = 0.6 * s0 * s2, t2 = 0.3 * s0 * s1;
-  const double q0 = fma(2.0, u0, fma(-7.0, u1, 11.0 * u2));
-  const double q1 = fma(-1.0, u1, fma(5.0, u2, 2.0 * u3));
-  const double q2 = fma(2.0, u2, fma(5.0, u3, -u4));
-  const double num = fma(t0, q0, fma(t1, q1, t2 * q2));
-  return num / (6.0 * (t0 + t1 + t2));
+  // w_k = (d_k / s_k) / sum_m (d_m / s_m), with all three over one quotient
+  const double t0 = 0.1 * (s1 * s2), t1 = 0.6 * (s0 * s2), t2 = 0.3 * (s0 * s1);
+  const double r0 = fma(5.0, b, -2.0 * a), r1 = fma(2.0, c, b), r2 = fma(4.0, c, -dd);
+  const double num = fma(t0, r0, fma(t1, r1, t2 * r2));
+  return fma(num, rcp_refined(6.0 * (t0 + t1 + t2)), u2);
 }
 
 template <bool FAST>
@@ -277,15 +299,44 @@ __device__ __forceinline__ double dot6v(const double r[6], const double x[6]) {
   return acc;
 }
 
-// One interface: U[q], Fl[q] for stencil points q = 0..5 (interface between
-// points 2 and 3); e2/e3 the specific energies of points 2 and 3.
+// The 6-point stencil of one interface (points q = 0..5, the interface
+// between points 2 and 3).  EXACT keeps each point's physical flux F
+// (flux_from_derived, src/state.rs:56-85) and projects it as the reference
+// does.  FAST keeps the normal velocity w and the pressures instead: with
+// F = w U + (0, p n_x, p n_y, w p_e, w p_i, w p_r), the projection of the
+// flux on a row l of L follows from that of the state,
+//   l . F = w (l . U + l_3 p_e + l_4 p_i + l_5 p_r) + l_n p,
+// five operations where the dot product takes six, and five values per
+// point held instead of six.
+template <bool FAST>
+struct Stencil;
+template <>
+struct Stencil<false> {
+  double U[6][6], F[6][6];
+};
+template <>
+struct Stencil<true> {
+  double U[6][6], w[6], pe[6], pi[6], pr[6], p[6];
+};
+
+// One interface from its stencil; e2/e3 the specific energies, (u2, v2) and
+// (u3, v3) the velocities of points 2 and 3.
 template <int DIR, bool FAST>
-__device__ __forceinline__ void interface_flux(const EigConst* Kp, double eps, const double U[6][6],
-                                               const double Fl[6][6], double u3, double v3, const double e3[3],
-                                               double u2, double v2, const double e2[3], double out[6]) {
+__device__ __forceinline__ void interface_flux(const EigConst* Kp, double eps, const Stencil<FAST>& P, double u3,
+                                               double v3, const double e3[3], double u2, double v2,
+                                               const double e2[3], double out[6]) {
+  const auto& U = P.U;
   Roe<DIR, FAST> R;
   roe_state<DIR, FAST>(*fresh(Kp), U[3][0], U[2][0], u3, v3, e3, u2, v2, e2, R);
   const int tr = DIR == 0 ? 4 : 1;
+  // FAST forms the split fluxes f+- = (cf +- |lam| cu) / 2 without the
+  // factor 1/2 and halves the sum of the two reconstructions instead:
+  // weno5(2x, 4 eps) = 2 weno5(x, eps) while no value leaves the normal
+  // range (every operation commutes with scaling by a power of two), and its
+  // weno5v takes 3 x that eps.  EXACT halves each split flux, as the
+  // reference does: near-uniform flow gives subnormal split fluxes, where
+  // the scaling is not exact.
+  const double ew = FAST ? 12.0 * eps : eps;
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     const EigConst& K = *fresh(Kp);
@@ -298,23 +349,33 @@ __device__ __forceinline__ void interface_flux(const EigConst* Kp, double eps, c
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
       double cf, cu;
-      if (c == tr) {  // row (-w, e_t): two terms, exact in both variants
-        cf = row[0] * Fl[q][0] + (DIR == 0 ? Fl[q][2] : Fl[q][1]);
+      if constexpr (FAST) {
+        if (c == tr) {  // row (-w_t, e_t): l . F = w (l . U)
+          cu = fma(row[0], U[q][0], DIR == 0 ? U[q][2] : U[q][1]);
+          cf = P.w[q] * cu;
+        } else {
+          cu = dot6v<FAST>(row, U[q]);
+          const double lp = fma(row[5], P.pr[q], fma(row[4], P.pi[q], fma(row[3], P.pe[q], cu)));
+          cf = fma(P.w[q], lp, (DIR == 0 ? row[1] : row[2]) * P.p[q]);
+        }
+      } else if (c == tr) {  // row (-w, e_t): two terms
+        cf = row[0] * P.F[q][0] + (DIR == 0 ? P.F[q][2] : P.F[q][1]);
         cu = row[0] * U[q][0] + (DIR == 0 ? U[q][2] : U[q][1]);
       } else {
-        cf = dot6v<FAST>(row, Fl[q]);
+        cf = dot6v<FAST>(row, P.F[q]);
         cu = dot6v<FAST>(row, U[q]);
       }
       if (FAST) {
-        fp[q] = 0.5 * fma(a, cu, cf);
-        fm[q] = 0.5 * fma(-a, cu, cf);
+        fp[q] = fma(a, cu, cf);
+        fm[q] = fma(-a, cu, cf);
       } else {
         fp[q] = 0.5 * (cf + a * cu);
         fm[q] = 0.5 * (cf - a * cu);
       }
     }
-    const double fc = weno5v<FAST>(fp[0], fp[1], fp[2], fp[3], fp[4], eps) +
-                      weno5v<FAST>(fm[5], fm[4], fm[3], fm[2], fm[1], eps);
+    const double fsum = weno5v<FAST>(fp[0], fp[1], fp[2], fp[3], fp[4], ew) +
+                        weno5v<FAST>(fm[5], fm[4], fm[3], fm[2], fm[1], ew);
+    const double fc = FAST ? 0.5 * fsum : fsum;
     // r_dot accumulated over c left to right
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
