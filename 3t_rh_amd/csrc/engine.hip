@@ -619,11 +619,14 @@ __device__ __forceinline__ void iface_flux(const SolveDesc& sd, const EigConst* 
   const uint8_t* __restrict__ pt = glb(sd.pt);
   const long c0 = pad_index(g, i, j);
   const long step = DIR == 0 ? g.PY : 1;
-  if (pt[c0] != 1 && pt[c0 - step] != 1) {
-#pragma unroll
-    for (int f = 0; f < 6; ++f) out[f] = 0.0;
-    return;
-  }
+  // Both point types are read together and the flux is formed whether or
+  // not the interface is active, then replaced by 0.0 if it is not: the
+  // stencil loads need not wait for the point types (every stencil point
+  // lies within the 4-point halo of the padded planes, so they are in
+  // bounds either way), one memory round trip before the arithmetic instead
+  // of three.  Values at points that are neither fluid nor ghost are never
+  // used.
+  const bool active = (pt[c0] == 1) | (pt[c0 - step] == 1);
   // 6-point stencil of U with each point's physical flux (EXACT,
   // flux_from_derived, src/state.rs:56-85) or its normal velocity and
   // pressures (FAST; see Stencil).  In x, each value is read as a workgroup-uniform
@@ -663,8 +666,13 @@ __device__ __forceinline__ void iface_flux(const SolveDesc& sd, const EigConst* 
   }
   const double e3[3] = {at(D + D_EE * g.plane, 0), at(D + D_EI * g.plane, 0), at(D + D_ER * g.plane, 0)};
   const double e2[3] = {at(D + D_EE * g.plane, -step), at(D + D_EI * g.plane, -step), at(D + D_ER * g.plane, -step)};
-  interface_flux<DIR, FAST>(Kp, eps, P, at(D + D_U * g.plane, 0), at(D + D_V * g.plane, 0), e3,
-                            at(D + D_U * g.plane, -step), at(D + D_V * g.plane, -step), e2, out);
+  const double u3 = at(D + D_U * g.plane, 0), v3 = at(D + D_V * g.plane, 0);
+  const double u2 = at(D + D_U * g.plane, -step), v2 = at(D + D_V * g.plane, -step);
+  interface_flux<DIR, FAST>(Kp, eps, P, u3, v3, e3, u2, v2, e2, out);
+  if (!active) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) out[f] = 0.0;
+  }
 }
 
 template <int DIR, bool FAST>
