@@ -1313,6 +1313,7 @@ int init_adjoint(Engine& E) {
   // the gather needs the temperatures of every stage state
   if (E.nplanes != D_N) {
     E.nplanes = D_N;
+    E.dvalid = 0;  // the kept Derived planes lack the temperatures
     if (E.graph) {
       (void)hipGraphExecDestroy(E.graph);
       E.graph = nullptr;
@@ -1395,6 +1396,7 @@ int begin_adjoint_call(Engine& E) {
   if (st) return st;
   HIP_TRY(hipSetDevice(E.device));
   if ((st = reset_errors(E))) return st;
+  E.dvalid = 0;  // adjoint stages write the state buffers without the kept planes
   AdjWork& W = *E.adj;
   const long npg = (long)kNParam * W.pg_blocks;
   k_fill<<<(unsigned)((npg + 255) / 256), 256, 0, E.stream>>>(W.PG, npg, 0.0);

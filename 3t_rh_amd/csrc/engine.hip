@@ -40,6 +40,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -84,8 +85,20 @@ __device__ __forceinline__ void raise_error(Scalars* sc, unsigned phase, unsigne
 
 // ------------------------------------------------------------ ghosts
 
+// Derived::from_state of the state s written to slot p of solve sd: its
+// first wd planes into sd.D (the planes of the buffer the slot belongs to)
+__device__ __forceinline__ void put_derived(const SolveDesc& sd, const Phys& ph, const double s[6], long p, int wd) {
+  double d[D_N];
+  derived_from_state(ph, s, d, wd > 8);
+  double* __restrict__ Dp = sd.D;  // per lane in the ghost kernels: no scalar base
+#pragma unroll
+  for (int f = 0; f < D_N; ++f)
+    if (f < wd) Dp[f * sd.g.plane + p] = d[f];
+}
+
+// wd > 0: also Derived of the ghost's new state, its first wd planes
 __device__ __forceinline__ void ghost_cheap(const SolveDesc* __restrict__ descs, const CheapGhosts& cg, int buf,
-                                            unsigned phase, int gt, int k) {
+                                            unsigned phase, int gt, int k, const Phys& ph, int wd) {
   const SolveDesc& sd = descs[cg.sid[k]];
   double* S = sd.U[buf];
   const long plane = sd.g.plane;
@@ -126,12 +139,30 @@ __device__ __forceinline__ void ghost_cheap(const SolveDesc* __restrict__ descs,
     ok = ok && isfinite(v[f]);
   }
   if (!ok) raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)cg.gid[k]);
+  if (wd > 0) put_derived(sd, ph, v, dst, wd);
 }
 
 __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict__ descs, CheapGhosts cg, int buf,
-                                                     unsigned phase, int gt) {
+                                                     unsigned phase, int gt, Phys ph, int wd) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < cg.n) ghost_cheap(descs, cg, buf, phase, gt, k);
+  if (k < cg.n) ghost_cheap(descs, cg, buf, phase, gt, k, ph, wd);
+}
+
+// Derived::from_state of every ghost slot of buffer buf (the cheap ghosts,
+// then the heavy ones), its first wd planes: after a ghost update whose
+// fluid cells' Derived planes are current.
+__global__ __launch_bounds__(256) void k_derived_ghosts(const SolveDesc* __restrict__ descs, CheapGhosts cg,
+                                                        HeavyGhosts hg, Phys ph, int buf, int wd) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cg.n + hg.n) return;
+  const bool ch = k < cg.n;
+  const SolveDesc& sd = descs[ch ? cg.sid[k] : hg.sid[k - cg.n]];
+  const long p = ch ? cg.pad[k] : hg.pad[k - cg.n];
+  const double* __restrict__ S = sd.U[buf];
+  double st[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) st[f] = S[f * sd.g.plane + p];
+  put_derived(sd, ph, st, p, wd);
 }
 
 
@@ -206,7 +237,7 @@ __global__ __launch_bounds__(64) void k_ghost_prep(const SolveDesc* __restrict__
     if (e < pk.n) pack_point(descs, pk, buf, e);
   } else {
     const int k = (b - nbp) * 64 + threadIdx.x;
-    if (k < cg.n) ghost_cheap(descs, cg, buf, phase, gt, k);
+    if (k < cg.n) ghost_cheap(descs, cg, buf, phase, gt, k, ph, 0);
   }
 }
 
@@ -827,6 +858,14 @@ __device__ __forceinline__ double assemble_core(const SolveDesc& sd, const Phys&
   }
 #pragma unroll
   for (int f = 0; f < 6; ++f) dst[f * g.plane + c0] = v[f];
+  if (sa.wd > 0) {  // Derived of the new state, for the next stage's kernels
+    double d[D_N];
+    derived_from_state(ph, v, d, sa.wd > 8);
+    double* __restrict__ Dn = glb(sd.Dn);
+#pragma unroll
+    for (int f = 0; f < D_N; ++f)
+      if (f < sa.wd) Dn[f * g.plane + c0] = d[f];
+  }
   if (!admissible(v)) raise_error(sd.sc, sa.phase, RH3T_ERR_NONPHYSICAL, (unsigned)lin);
   return sa.mode == 3 ? local_dt(ph, v, g.dx, g.dy) : INFINITY;
 }
@@ -1088,12 +1127,16 @@ int build_tables(Engine& E) {
 
 namespace rh3t_eng {
 
-void launch_ghosts(Engine& E, int buf, unsigned phase, int gt) {
+// The ghost update of buffer buf with descriptors desc; wd > 0: also the
+// ghost slots' Derived planes (their first wd), in k_ghost_cheap when there
+// are no heavy ghosts, else in one pass over every ghost slot afterwards.
+void ghosts_on(Engine& E, const SolveDesc* desc, int buf, unsigned phase, int gt, int wd) {
   const int n_open = E.heavy.n_open;
   const bool with_prep = E.heavy.n > 0 && E.cheap_with_prep;
   if (E.cheap.n > 0 && !with_prep) {
     Launcher L(E, K_GHOST_CHEAP);
-    k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(E.d_desc, E.cheap, buf, phase, gt);
+    k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(desc, E.cheap, buf, phase, gt, E.phys,
+                                                                 E.heavy.n > 0 ? 0 : wd);
   }
   if (E.heavy.n > 0) {
     // the frames and packed points, with the cheap ghosts riding along
@@ -1101,7 +1144,7 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, int gt) {
     Launcher L(E, K_GHOST_HEAVY);
     const int nbf = (E.frames.n + 63) / 64, nbp = nbf + (E.packed.n + 63) / 64;
     const int nbc = with_prep ? (E.cheap.n + 63) / 64 : 0;
-    k_ghost_prep<<<nbp + nbc, 64, 0, E.stream>>>(E.d_desc, E.frames, E.packed, E.cheap, E.phys, buf, phase, gt, nbf,
+    k_ghost_prep<<<nbp + nbc, 64, 0, E.stream>>>(desc, E.frames, E.packed, E.cheap, E.phys, buf, phase, gt, nbf,
                                                  nbp);
   }
   if (E.heavy.n > 0) {
@@ -1110,59 +1153,102 @@ void launch_ghosts(Engine& E, int buf, unsigned phase, int gt) {
     if (n_open > 0) {
       const int per = g * kOpenPerWave;  // ghosts per workgroup
       k_ghost_open<<<(n_open + per - 1) / per, dim3(64, g), (size_t)per * kOpenLds * sizeof(double), E.stream>>>(
-          E.d_desc, E.heavy, E.phys, buf, phase, 0, n_open);
+          desc, E.heavy, E.phys, buf, phase, 0, n_open);
     }
     if (n_wall > 0) {
       const int per = g * kOpenPerWave;
       k_ghost_wall<<<(n_wall + per - 1) / per, dim3(64, g), (size_t)per * grp_lds(true) * sizeof(double),
-                     E.stream>>>(E.d_desc, E.heavy, E.phys, buf, phase, n_open, n_wall);
+                     E.stream>>>(desc, E.heavy, E.phys, buf, phase, n_open, n_wall);
     }
   }
+  if (wd > 0 && E.heavy.n > 0) {
+    Launcher L(E, K_DERIVED);
+    const int n = E.cheap.n + E.heavy.n;
+    k_derived_ghosts<<<(n + 255) / 256, 256, 0, E.stream>>>(desc, E.cheap, E.heavy, E.phys, buf, wd);
+  }
 }
+
+void launch_ghosts(Engine& E, int buf, unsigned phase, int gt) { ghosts_on(E, E.d_desc, buf, phase, gt, 0); }
 
 void launch_capture(Engine& E, unsigned lo, unsigned hi) {
   Launcher L(E, K_CAPTURE);
   k_capture<<<(E.nsolve + 63) / 64, 64, 0, E.stream>>>(E.d_desc, E.nsolve, lo, hi);
 }
 
-void launch_derived(Engine& E, int buf) {
+void derived_on(Engine& E, const SolveDesc* desc, int buf) {
   Launcher L(E, K_DERIVED);
-  k_derived<<<total(E.t_der), 256, 0, E.stream>>>(E.d_desc, E.tab(3), E.nsolve, E.phys, buf, E.nplanes);
+  k_derived<<<total(E.t_der), 256, 0, E.stream>>>(desc, E.tab(3), E.nsolve, E.phys, buf, E.nplanes);
 }
 
-void launch_stage(Engine& E, int src, const StageSpec& sa) {
-  launch_ghosts(E, src, sa.mode == 0 ? 2u : (unsigned)(2 * sa.mode), sa.mode);
-  launch_derived(E, src);
+void launch_derived(Engine& E, int buf) {
+  derived_on(E, E.d_desc, buf);
+  E.dvalid = 0;  // the planes of U[0] now hold another state's (or U[0]'s ghosts only)
+}
+
+// Derived of every point of U[buf] into its own planes, if not current
+void ensure_derived(Engine& E, int buf) {
+  if (E.dvalid & (1u << buf)) return;
+  derived_on(E, E.d_desc3 + (size_t)buf * E.nsolve, buf);
+  E.dvalid |= 1u << buf;
+}
+
+void launch_stage(Engine& E, int src, const StageSpec& sa_in, bool kept) {
+  StageSpec sa = sa_in;
+  const SolveDesc* desc = E.d_desc;
+  const unsigned phase = sa.mode == 0 ? 2u : (unsigned)(2 * sa.mode);
+  if (kept) {
+    desc = E.d_desc3 + (size_t)src * E.nsolve;
+    if (E.dvalid & (1u << src)) {
+      ghosts_on(E, desc, src, phase, sa.mode, E.nplanes);
+    } else {
+      ghosts_on(E, desc, src, phase, sa.mode, 0);
+      derived_on(E, desc, src);
+      E.dvalid |= 1u << src;
+    }
+    // the assembly fills the planes of the new state, U[(src + 1) % 3]
+    if (sa.mode >= 1 && sa.dst == (src + 1) % 3) {
+      sa.wd = E.nplanes;
+      E.dvalid |= 1u << sa.dst;
+    } else {
+      sa.wd = 0;
+      if (sa.mode >= 1) E.dvalid &= ~(1u << sa.dst);
+    }
+  } else {
+    sa.wd = 0;
+    ghosts_on(E, desc, src, phase, sa.mode, 0);
+    derived_on(E, desc, src);
+    E.dvalid = 0;
+  }
   dim3 blk(E.bx, E.by);
   {
     Launcher L(E, K_FLUX_X);
     if (E.fast)
-      k_flux<0, true><<<total(E.t_fx), blk, 0, E.stream>>>(E.d_desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
+      k_flux<0, true><<<total(E.t_fx), blk, 0, E.stream>>>(desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
     else
-      k_flux<0, false><<<total(E.t_fx), blk, 0, E.stream>>>(E.d_desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
+      k_flux<0, false><<<total(E.t_fx), blk, 0, E.stream>>>(desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
   }
   if (fused_fy(E)) {
     Launcher L(E, K_FY_ASSEMBLE);
     if (E.fast)
-      k_fy_assemble<true><<<total(E.t_fa), blk, 0, E.stream>>>(E.d_desc, E.tab(4), E.nsolve, E.d_eig, E.phys, src,
+      k_fy_assemble<true><<<total(E.t_fa), blk, 0, E.stream>>>(desc, E.tab(4), E.nsolve, E.d_eig, E.phys, src,
                                                                sa, E.d_partial);
     else
-      k_fy_assemble<false><<<total(E.t_fa), blk, 0, E.stream>>>(E.d_desc, E.tab(4), E.nsolve, E.d_eig, E.phys, src,
+      k_fy_assemble<false><<<total(E.t_fa), blk, 0, E.stream>>>(desc, E.tab(4), E.nsolve, E.d_eig, E.phys, src,
                                                                 sa, E.d_partial);
   } else {
     {
       Launcher L(E, K_FLUX_Y);
       if (E.fast)
-        k_flux<1, true><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps, src);
+        k_flux<1, true><<<total(E.t_fy), blk, 0, E.stream>>>(desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps, src);
       else
-        k_flux<1, false><<<total(E.t_fy), blk, 0, E.stream>>>(E.d_desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps,
+        k_flux<1, false><<<total(E.t_fy), blk, 0, E.stream>>>(desc, E.tab(1), E.nsolve, E.d_eig, E.phys.eps,
                                                               src);
     }
     Launcher L(E, K_ASSEMBLE);
     if (E.fast)
-      k_assemble<true><<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
+      k_assemble<true><<<total(E.t_as), blk, 0, E.stream>>>(desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
     else
-      k_assemble<false><<<total(E.t_as), blk, 0, E.stream>>>(E.d_desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
+      k_assemble<false><<<total(E.t_as), blk, 0, E.stream>>>(desc, E.tab(2), E.nsolve, E.phys, sa, E.d_partial);
   }
 }
 
@@ -1189,9 +1275,9 @@ void launch_dt(Engine& E, int advance, bool fresh) {
 //   stage 3: U0 = 1/3 U0 + 2/3 U2 + 2dt/3 L(U2)   (in place: each cell
 //            reads its own U0 value before writing it)
 void launch_rk3(Engine& E) {
-  launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3});
-  launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5});
-  launch_stage(E, 2, StageSpec{3, 0, 2, 0, 7});
+  launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3}, true);
+  launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5}, true);
+  launch_stage(E, 2, StageSpec{3, 0, 2, 0, 7}, true);
   launch_capture(E, 2, 7);
 }
 
@@ -1320,8 +1406,10 @@ int apply_physics(Engine& E, const rh3t_physics& p) {
   E.eig = make_eig(E.phys);
   HIP_TRY(hipMemcpy(E.d_eig, &E.eig, sizeof(EigConst), hipMemcpyHostToDevice));
   E.nplanes = (E.adj || E.phys.diffusion || E.phys.source) ? D_N : 8;
-  // the stored get_local_dt minima were formed with the old constants
+  // the stored get_local_dt minima and Derived planes were formed with the
+  // old constants
   E.dt_partials_valid = false;
+  E.dvalid = 0;
   if (E.graph) {
     (void)hipGraphExecDestroy(E.graph);
     E.graph = nullptr;
@@ -1400,6 +1488,7 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   E.nplanes = (E.phys.diffusion || E.phys.source) ? D_N : 8;
   E.solves.resize(n);
   E.h_desc.resize(n);
+  std::vector<std::array<double*, 3>> dplanes(n);  // Derived planes of U0, U1, U2 per solve
   E.last.assign(n, rh3t_error{});
   int st;
   if ((st = dev_alloc(E, &E.d_sc, (size_t)n))) return st;
@@ -1519,14 +1608,30 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
       HIP_TRY(hipMemset(d.U[b], 0, 6 * (size_t)g.plane * sizeof(double)));
     }
     d.U[3] = d.U[0];
-    if ((st = dev_alloc(E, &d.D, (size_t)D_N * g.plane))) return st;
-    HIP_TRY(hipMemset(d.D, 0, (size_t)D_N * g.plane * sizeof(double)));
+    for (int b = 0; b < 3; ++b) {
+      if ((st = dev_alloc(E, &dplanes[s][b], (size_t)D_N * g.plane))) return st;
+      HIP_TRY(hipMemset(dplanes[s][b], 0, (size_t)D_N * g.plane * sizeof(double)));
+    }
+    d.D = dplanes[s][0];
+    d.Dn = dplanes[s][1];
     if ((st = dev_alloc(E, &d.FX, 6 * (size_t)(g.nx + 1) * g.ny))) return st;
     if ((st = dev_alloc(E, &d.FY, 6 * (size_t)g.nx * (g.ny + 1)))) return st;
     if ((st = dev_alloc(E, &d.RHS, 6 * (size_t)ncell))) return st;
     d.sc = E.d_sc + s;
   }
-  if ((st = upload(E, &E.d_desc, E.h_desc))) return st;
+  {
+    std::vector<SolveDesc> d3;
+    for (int b = 0; b < 3; ++b)
+      for (int s = 0; s < n; ++s) {
+        SolveDesc d = E.h_desc[s];
+        d.D = dplanes[s][b];
+        d.Dn = dplanes[s][(b + 1) % 3];
+        d3.push_back(d);
+      }
+    if ((st = upload(E, &E.d_desc3, d3))) return st;
+    E.d_desc = E.d_desc3;
+    E.dvalid = 0;
+  }
   if ((st = dev_alloc(E, &E.d_aos, 6 * (size_t)max_cells))) return st;
   if ((st = dev_alloc(E, &E.d_gvals, 6 * (size_t)max_ghost))) return st;
 
@@ -1692,6 +1797,7 @@ int set_state(Engine& E, int s, const double* u, double time) {
   HIP_TRY(hipMemcpyAsync(&E.d_sc[s].time, &time, sizeof(double), hipMemcpyHostToDevice, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
   E.dt_partials_valid = false;
+  E.dvalid = 0;
   return RH3T_OK;
 }
 
@@ -1729,6 +1835,9 @@ int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
   HIP_TRY(hipMemcpyAsync(E.d_sc, sc.data(), sizeof(Scalars) * E.nsolve, hipMemcpyHostToDevice, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
   if (!E.dt_partials_valid) launch_dt_partial(E);
+  // the captured step starts from current Derived planes of U0 (as every
+  // step leaves them), so that its first stage forms only the ghosts'
+  ensure_derived(E, 0);
   if (E.profiling) {
     for (int s = 0; s < nsteps; ++s) {
       launch_dt(E, 1, false);
@@ -1885,7 +1994,7 @@ int rh3t_l(rh3t_ctx* ctx, double* rhs_aos, double* ghost_vals) {
   HIP_TRY(hipSetDevice(E.device));
   int st = reset_errors(E);
   if (st) return st;
-  launch_stage(E, 0, StageSpec{0, 0, 0, 0, 2});
+  launch_stage(E, 0, StageSpec{0, 0, 0, 0, 2}, true);
   launch_capture(E, 2, 2);
   HIP_TRY(hipGetLastError());
   const SolveDesc& d = E.h_desc[0];

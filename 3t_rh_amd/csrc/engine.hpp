@@ -25,6 +25,8 @@ struct SolveDesc {
   double* D;            // Derived planes of the state a kernel reads (on an
                         // adjoint tape every step and stage state has its
                         // own descriptor and planes)
+  double* Dn;           // Derived planes of the state a stage writes, filled
+                        // by its assembly when StageSpec::wd > 0
   double* FX;
   double* FY;
   double* RHS;          // compact SoA nx*ny (l() output)
@@ -122,6 +124,7 @@ struct StageSpec {
   int mode;        // 0: write rhs, 1..3: SSP-RK3 stage
   int base, add, dst;  // buffer indices into SolveDesc::U
   unsigned phase;
+  int wd = 0;      // Derived planes the assembly writes for the new state (0: none)
 };
 
 enum KernelId {
@@ -157,6 +160,13 @@ struct Engine {
   std::vector<SolveHost> solves;
   std::vector<SolveDesc> h_desc;
   SolveDesc* d_desc = nullptr;
+  // Derived planes kept per state buffer (the plain and batched forward):
+  // descriptor array k (d_desc3 + k * nsolve) reads the planes of U[k] and
+  // its stage writes those of U[(k + 1) % 3], the next RK3 stage state;
+  // array 0 is d_desc.  Bit k of dvalid: the fluid cells' planes of U[k]
+  // hold Derived of its current state.
+  SolveDesc* d_desc3 = nullptr;
+  unsigned dvalid = 0;
   Scalars* d_sc = nullptr;
   CheapGhosts cheap{};
   HeavyGhosts heavy{};
@@ -255,8 +265,13 @@ inline int total(const std::vector<int>& t) { return t.back(); }
 void launch_ghosts(Engine& E, int buf, unsigned phase, int gt);
 // Derived::from_state on every fluid cell and ghost of buffer buf
 void launch_derived(Engine& E, int buf);
-// ghosts -> derived -> x/y fluxes -> assemble (one l() + stage) on buffer src
-void launch_stage(Engine& E, int src, const StageSpec& sa);
+// ghosts -> derived -> x/y fluxes -> assemble (one l() + stage) on buffer src,
+// with the descriptors in E.d_desc (the adjoint swaps in its own).  kept:
+// the engine's per-buffer Derived planes (d_desc3): Derived is formed only
+// where the planes of U[src] are not already current (the ghost slots, or
+// every point after a host write), and a stage's assembly fills the planes
+// of its new state.
+void launch_stage(Engine& E, int src, const StageSpec& sa, bool kept = false);
 // the failing point's state of the current step's first failure, if its
 // phase is in [lo, hi], into the solves' scalars (after the stages of those
 // phases; a recompute of stages that already ran without error skips it)
