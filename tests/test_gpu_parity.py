@@ -556,3 +556,66 @@ def test_set_physics_recomputes_dt(rh3t, box64):
     assert dt_o < 0.5 * ob.global_dt(p, u2)
     assert abs((t3 - t2) - 0.05 * dt_o) <= 1e-13 * dt_o
     s.close()
+
+
+@pytest.mark.parametrize("case", ["box64", "cylinder"])
+def test_kept_derived_planes_across_calls(rh3t, box64, cyl, case):
+    """Each state buffer keeps its own Derived planes between stages and
+    steps; they are formed anew after a host write, a physics change and an
+    adjoint call, which overwrite or invalidate them.  A sequence that mixes
+    those calls (steps, l() right after a step, set_state mid-run, new
+    physics with the temperature planes, a VJP between steps) must match the
+    oracle at every point: bitwise in EXACT at default physics.  box64 forms
+    the ghosts' Derived values in k_ghost_cheap, the cylinder in the pass
+    after its heavy ghosts."""
+    f, u, p = box64 if case == "box64" else cyl
+    if case == "cylinder":
+        u = _developed(f, u, p, 1)
+    m = f.fluid
+    _say(f"kept Derived planes across calls: {case}")
+    s = f.solver(arith=0)
+
+    def same_state(want):
+        got, _ = s.get_state()
+        assert np.array_equal(got[m], want[m]), np.abs(got[m] - want[m]).max()
+
+    def same_l(state, phys=None, tol=0.0):
+        rhs, gv = s.l(with_ghosts=True)
+        ref, gref = ob.l(p, state, phys=phys, with_ghosts=True)
+        if tol == 0.0:
+            assert np.array_equal(gv, gref), np.abs(gv - gref).max()
+            assert np.array_equal(rhs[m], ref[m]), np.abs(rhs[m] - ref[m]).max()
+        else:
+            assert ob.relerr(rhs, ref, m).max() <= tol
+
+    s.set_state(u, 0.0)
+    dt = 0.05 * ob.global_dt(p, u)
+    s.rk3_ssp(dt)                       # planes of U0 formed from scratch, stages keep theirs
+    u1, _ = ob.rk3_step(p, u, dt)
+    same_state(u1)
+    same_l(u1)                          # l() reads the planes stage 3 wrote
+    s.advance(2, 0.05, dt)              # the step graph, starting from kept planes
+    u3, _ = ob.rk3_step(p, ob.rk3_step(p, u1, dt)[0], dt)
+    same_state(u3)
+    v = u3.copy()
+    v[m, 4] *= 1.01                     # a host write between steps
+    s.set_state(v, 0.0)
+    same_l(v)
+    s.advance(1, 0.05, dt)
+    v1, _ = ob.rk3_step(p, v, dt)
+    same_state(v1)
+    rbar = np.random.default_rng(5).standard_normal(v1.shape)
+    s.l_vjp(rbar)                       # adjoint stages overwrite the planes of U0
+    s.advance(1, 0.05, dt)
+    v2, _ = ob.rk3_step(p, v1, dt)
+    same_state(v2)
+    same_l(v2)
+    kw = dict(omega_ei=1.0, omega_er=0.1, kappa_e=1e-3, kappa_i=1e-3, kappa_r=1e-3)
+    s.set_physics(rh3t.Physics(**kw))   # temperature planes from here on
+    ph = ob.physics(**kw)
+    s.advance(1, 0.05, dt)
+    v3, _ = ob.rk3_step(p, v2, dt, phys=ph)
+    got, _ = s.get_state()
+    assert ob.relerr(got, v3, m).max() <= TOL
+    same_l(got, phys=ph, tol=TOL)
+    s.close()
