@@ -106,6 +106,9 @@ constexpr int kCbPair = 36;
 template <int DIR>
 __device__ __forceinline__ void adj_iface(const AdjDesc& A, const EigConst* __restrict__ Kp, const Phys& ph,
                                           int tile, double (*s_cb)[kIfBx]) {
+  // not held to the forward's rounding sequence (unlike k_adj_heavy, which
+  // recomputes the forward's values): products and sums contract to FMA
+#pragma clang fp contract(fast)
   const Geo g = A.g;
   const int ni = DIR == 0 ? g.nx + 1 : g.nx;
   const int nj = DIR == 0 ? g.ny : g.ny + 1;
@@ -329,6 +332,7 @@ struct PlaneLine {
 template <int DIR, class Line>
 __device__ __forceinline__ void line_gather(const Line& T, const Geo& g, const Phys& ph, int i, int j,
                                             double db[D_N], double tb[3]) {
+#pragma clang fp contract(fast)
   const double ds = DIR == 0 ? g.dx : g.dy;
   const double ids = 1.0 / ds;
   const int VP = DIR == 0 ? D_U : D_V;
@@ -417,6 +421,7 @@ __device__ __forceinline__ void line_gather(const Line& T, const Geo& g, const P
 // sum_t lap_t T(c + (t-3) st) / (180 ds^2) for the three temperatures
 template <int DIR>
 __device__ __forceinline__ void laplacian(const AdjDesc& A, long c, double lap[3]) {
+#pragma clang fp contract(fast)
   const Geo& g = A.g;
   const long P = g.plane;
   const long st = DIR == 0 ? g.PY : 1;
@@ -441,6 +446,8 @@ __device__ __forceinline__ void laplacian(const AdjDesc& A, long c, double lap[3
 template <class LX, class LY>
 __device__ __forceinline__ void gather_point(const AdjDesc& A, const Phys& ph, int ip, int jp, int kind,
                                              const LX& lx, const LY& ly, double pg[kNParam]) {
+  // a pullback with no forward values to reproduce: FMA contraction
+#pragma clang fp contract(fast)
   const Geo& g = A.g;
   const long P = g.plane;
   const long p = (long)ip * g.PY + jp;

@@ -25,7 +25,8 @@ namespace rh3t_dev {
 // ----------------------------------------------------------- WENO5
 
 // Value of weno5(u) (src/weno.rs:846-886: alpha_k = d_k / (eps + beta_k)^2)
-// and g = s * d weno5 / d u.
+// and g = s * d weno5 / d u; the four reciprocals from v_rcp_f64 and a
+// Newton step (within 2.3e-15 relative).
 __device__ __forceinline__ double weno5_vjp(const double u[5], double eps, double s, double g[5]) {
   const double A0 = u[0] - 2.0 * u[1] + u[2], B0 = u[0] - 4.0 * u[1] + 3.0 * u[2];
   const double A1 = u[1] - 2.0 * u[2] + u[3], B1 = u[1] - u[3];
@@ -36,11 +37,11 @@ __device__ __forceinline__ double weno5_vjp(const double u[5], double eps, doubl
   double a[3], ie[3], S = 0.0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    ie[k] = 1.0 / (eps + beta[k]);
+    ie[k] = rcp_refined(eps + beta[k]);
     a[k] = dk[k] * ie[k] * ie[k];
     S += a[k];
   }
-  const double iS = 1.0 / S;
+  const double iS = rcp_refined(S);
   const double p[3] = {u[0] * (1.0 / 3.0) - 7.0 / 6.0 * u[1] + 11.0 / 6.0 * u[2],
                        -u[1] * (1.0 / 6.0) + 5.0 / 6.0 * u[2] + u[3] * (1.0 / 3.0),
                        u[2] * (1.0 / 3.0) + 5.0 / 6.0 * u[3] - u[4] * (1.0 / 6.0)};
