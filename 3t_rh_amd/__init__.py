@@ -131,6 +131,7 @@ SIGNATURES = {
     "rh3t_advance": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _DP]),
     "rh3t_ctx_set_arith": (ctypes.c_int, [_P, ctypes.c_int32]),
     "rh3t_ctx_set_launch": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "rh3t_ctx_set_launch_x": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "rh3t_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.POINTER(c_physics),
                                          ctypes.c_int32, ctypes.POINTER(_P)]),
     "rh3t_batch_destroy": (None, [_P]),
@@ -138,6 +139,7 @@ SIGNATURES = {
     "rh3t_batch_get_state": (ctypes.c_int, [_P, ctypes.c_int32, _DP, _DP]),
     "rh3t_batch_advance": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, _DP]),
     "rh3t_batch_set_launch": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "rh3t_batch_set_launch_x": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "rh3t_batch_set_arith": (ctypes.c_int, [_P, ctypes.c_int32]),
     "rh3t_batch_last_error": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(c_error)]),
     "rh3t_batch_profile": (ctypes.c_int, [_P, ctypes.c_int32]),
@@ -438,8 +440,13 @@ class Solver:
     def set_arith(self, mode):
         _check(lib().rh3t_ctx_set_arith(self._h, mode))
 
-    def set_launch(self, bx, by, ghosts_per_block=1):
+    def set_launch(self, bx, by, ghosts_per_block=1, x_shape=None):
+        """Workgroup shape of the flux / assembly kernels and heavy ghosts per
+        workgroup; x_shape = (bx, by) gives the x-flux kernel its own shape
+        ((0, 0): the same as the others)."""
         _check(lib().rh3t_ctx_set_launch(self._h, bx, by, ghosts_per_block), "set_launch")
+        if x_shape is not None:
+            _check(lib().rh3t_ctx_set_launch_x(self._h, *x_shape), "set_launch x")
 
     def set_state(self, u, time=0.0):
         u = np.ascontiguousarray(u, dtype=np.float64)
@@ -645,8 +652,10 @@ class Batch:
             raise RH3TError(st, f"batch advance: {errs[:3]}", errs)
         return times
 
-    def set_launch(self, bx, by, ghosts_per_block=1):
+    def set_launch(self, bx, by, ghosts_per_block=1, x_shape=None):
         _check(lib().rh3t_batch_set_launch(self._h, bx, by, ghosts_per_block), "batch set_launch")
+        if x_shape is not None:
+            _check(lib().rh3t_batch_set_launch_x(self._h, *x_shape), "batch set_launch x")
 
     def set_arith(self, mode):
         _check(lib().rh3t_batch_set_arith(self._h, mode), "batch set_arith")

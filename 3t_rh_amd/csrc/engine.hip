@@ -1003,8 +1003,20 @@ __global__ __launch_bounds__(1024) void k_dt_final(const SolveDesc* __restrict__
                                                    const double* __restrict__ partial, int advance) {
   __shared__ double red[16];
   const int s = blockIdx.x;
-  double m = INFINITY;
-  for (int k = tiles[s] + threadIdx.x; k < tiles[s + 1]; k += 1024) m = fmin(m, partial[k]);
+  // eight independent loads in flight per thread (fmin is exact and order
+  // independent: the same minimum in any grouping)
+  const int k1 = tiles[s + 1];
+  double mm[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) mm[u] = INFINITY;
+  int k = tiles[s] + threadIdx.x;
+  for (; k + 7 * 1024 < k1; k += 8 * 1024)
+#pragma unroll
+    for (int u = 0; u < 8; ++u) mm[u] = fmin(mm[u], partial[k + u * 1024]);
+  for (; k < k1; k += 1024) mm[0] = fmin(mm[0], partial[k]);
+  double m = mm[0];
+#pragma unroll
+  for (int u = 1; u < 8; ++u) m = fmin(m, mm[u]);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -1095,7 +1107,8 @@ int build_tables(Engine& E) {
   for (int s = 0; s < n; ++s) {
     const Geo& g = E.solves[s].g;
     auto tiles = [&](int ni, int nj) { return ((nj + E.bx - 1) / E.bx) * ((ni + E.by - 1) / E.by); };
-    E.t_fx[s + 1] = E.t_fx[s] + tiles(g.nx + 1, g.ny);
+    const int xbx = E.fx_bx(), xby = E.fx_by();
+    E.t_fx[s + 1] = E.t_fx[s] + ((g.ny + xbx - 1) / xbx) * ((g.nx + 1 + xby - 1) / xby);
     E.t_fy[s + 1] = E.t_fy[s] + tiles(g.nx, g.ny + 1);
     E.t_as[s + 1] = E.t_as[s] + tiles(g.nx, g.ny);
     E.t_der[s + 1] = E.t_der[s] + (int)((g.plane + 255) / 256);
@@ -1222,10 +1235,11 @@ void launch_stage(Engine& E, int src, const StageSpec& sa_in, bool kept) {
   dim3 blk(E.bx, E.by);
   {
     Launcher L(E, K_FLUX_X);
+    const dim3 xblk(E.fx_bx(), E.fx_by());
     if (E.fast)
-      k_flux<0, true><<<total(E.t_fx), blk, 0, E.stream>>>(desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
+      k_flux<0, true><<<total(E.t_fx), xblk, 0, E.stream>>>(desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
     else
-      k_flux<0, false><<<total(E.t_fx), blk, 0, E.stream>>>(desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
+      k_flux<0, false><<<total(E.t_fx), xblk, 0, E.stream>>>(desc, E.tab(0), E.nsolve, E.d_eig, E.phys.eps, src);
   }
   if (fused_fy(E)) {
     Launcher L(E, K_FY_ASSEMBLE);
@@ -1323,12 +1337,13 @@ int reset_errors(Engine& E) {
   return RH3T_OK;
 }
 
-void set_arith(Engine& E, bool fast) {
+int set_arith(Engine& E, bool fast) {
   if (fast != E.fast && E.graph) {
     (void)hipGraphExecDestroy(E.graph);
     E.graph = nullptr;
   }
   E.fast = fast;
+  return RH3T_OK;
 }
 
 }  // namespace rh3t_eng
@@ -1913,13 +1928,23 @@ int profile_read(Engine& E, int max_kernels, char* names, int names_cap, double*
   return RH3T_OK;
 }
 
+bool valid_shape(int bx, int by) {
+  return bx > 0 && by > 0 && bx * by <= 256 && bx % 16 == 0 && (bx * by) % 64 == 0;
+}
+
 int set_launch(Engine& E, int bx, int by, int gpb) {
-  if (bx <= 0 || by <= 0 || bx * by > 256 || bx % 16 != 0 || (bx * by) % 64 != 0 || gpb < 1 ||
-      gpb > kMaxGhostsPerBlock)
-    return RH3T_ERR_ARG;
+  if (!valid_shape(bx, by) || gpb < 1 || gpb > kMaxGhostsPerBlock) return RH3T_ERR_ARG;
   E.bx = bx;
   E.by = by;
   E.ghosts_per_block = gpb;
+  return build_tables(E);
+}
+
+// (0, 0): the x-flux kernel follows the assembly shape again
+int set_launch_x(Engine& E, int bx, int by) {
+  if (!(bx == 0 && by == 0) && !valid_shape(bx, by)) return RH3T_ERR_ARG;
+  E.xbx = bx;
+  E.xby = by;
   return build_tables(E);
 }
 
@@ -2055,8 +2080,7 @@ int rh3t_advance(rh3t_ctx* ctx, int32_t nsteps, double cfl_factor, double dt_fix
 
 int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode) {
   if (!ctx || (mode != RH3T_ARITH_EXACT && mode != RH3T_ARITH_FAST)) return RH3T_ERR_ARG;
-  set_arith(ctx->e, mode == RH3T_ARITH_FAST);
-  return RH3T_OK;
+  return set_arith(ctx->e, mode == RH3T_ARITH_FAST);
 }
 
 int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys) {
@@ -2070,6 +2094,11 @@ int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys) {
 int rh3t_ctx_set_launch(rh3t_ctx* ctx, int32_t bx, int32_t by, int32_t ghosts_per_block) {
   if (!ctx) return RH3T_ERR_ARG;
   return set_launch(ctx->e, bx, by, ghosts_per_block);
+}
+
+int rh3t_ctx_set_launch_x(rh3t_ctx* ctx, int32_t bx, int32_t by) {
+  if (!ctx) return RH3T_ERR_ARG;
+  return set_launch_x(ctx->e, bx, by);
 }
 
 int rh3t_last_error(const rh3t_ctx* ctx, rh3t_error* err) {
@@ -2143,13 +2172,17 @@ int rh3t_batch_advance(rh3t_batch* b, int32_t nsteps, double cfl_factor, double*
 
 int rh3t_batch_set_arith(rh3t_batch* b, int32_t mode) {
   if (!b || (mode != RH3T_ARITH_EXACT && mode != RH3T_ARITH_FAST)) return RH3T_ERR_ARG;
-  set_arith(b->e, mode == RH3T_ARITH_FAST);
-  return RH3T_OK;
+  return set_arith(b->e, mode == RH3T_ARITH_FAST);
 }
 
 int rh3t_batch_set_launch(rh3t_batch* b, int32_t bx, int32_t by, int32_t ghosts_per_block) {
   if (!b) return RH3T_ERR_ARG;
   return set_launch(b->e, bx, by, ghosts_per_block);
+}
+
+int rh3t_batch_set_launch_x(rh3t_batch* b, int32_t bx, int32_t by) {
+  if (!b) return RH3T_ERR_ARG;
+  return set_launch_x(b->e, bx, by);
 }
 
 int rh3t_batch_last_error(const rh3t_batch* b, int32_t solve, rh3t_error* err) {

@@ -183,7 +183,15 @@ struct Engine {
 
   // launch shapes (autotuned): flux / assemble workgroup (bx, by), heavy
   // ghosts per workgroup
-  int bx = 64, by = 4, ghosts_per_block = 1;
+  // (64, 1) by default: a one-wavefront workgroup starts as soon as one SIMD
+  // has a free slot.  Measured against (64, 4): 2048^2 box FAST 2.171 /
+  // 2.336 ms per step, EXACT 3.675 / 3.658; cylinder 121x481 EXACT 0.1825 /
+  // 0.1871; double Mach h = 1/320 FAST 0.681 / 0.727, EXACT 1.050 / 1.060.
+  int bx = 64, by = 1, ghosts_per_block = 1;
+  // the x-flux kernel's own workgroup shape (0: the same as (bx, by))
+  int xbx = 0, xby = 0;
+  int fx_bx() const { return xbx ? xbx : bx; }
+  int fx_by() const { return xby ? xby : by; }
   std::vector<int> t_fx, t_fy, t_as, t_der, t_fa;  // host prefix tables
   int* d_tabs = nullptr;  // device copy: fx | fy | as | der, each nsolve+1
   double* d_partial = nullptr;  // get_local_dt minimum per assembly workgroup
@@ -289,7 +297,7 @@ int apply_physics(Engine& E, const rh3t_physics& p);
 // the device constants (gamma - 1, gates, ...) of a caller's physics struct
 Phys physics_of(const rh3t_physics& p);
 // selects the flux / assembly arithmetic (drops the step graph on change)
-void set_arith(Engine& E, bool fast);
+int set_arith(Engine& E, bool fast);
 
 }  // namespace rh3t_eng
 

@@ -180,7 +180,8 @@ def batched_leg(rh3t, n_solves=64, steps=100, seed=4, tune=True, arith=0):
             "batched_s": t_batch, "plain_back_to_back_s": t_plain, "batched_vs_plain": t_plain / t_batch,
             "batched_cell_stage_per_s": cells * 3 * steps / t_batch,
             "plain_cell_stage_per_s": cells * 3 * steps / t_plain,
-            "autotune": None if tuned is None else {k: tuned[k] for k in ("bx", "by", "ghosts_per_block",
+            "autotune": None if tuned is None else {k: tuned[k] for k in ("bx", "by", "x_bx", "x_by",
+                                                                          "ghosts_per_block",
                                                                           "ms_per_step")},
             "kernels_ms_per_step": kern}
 

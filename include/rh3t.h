@@ -273,12 +273,13 @@ int rh3t_advance(rh3t_ctx* ctx, int32_t nsteps, double cfl_factor,
  * default of every context and batch) keeps the reference's rounding
  * sequence (literal divisions, left-to-right sums, no FMA) and agrees with
  * the reference's l() and rk3_ssp() to 1e-12 on every problem.  FAST uses
- * one reciprocal per shared denominator and FMA chains: the same algebra,
- * agreement with EXACT to roundoff, within 1e-12 on smooth periodic boxes
- * (6.9e-13 at 2048^2) but not on the cylinder's full RK3 step (2.3e-12),
- * where the boundary WENO weights (eps + beta)^-10 (src/bc1.rs:241) amplify
- * last-bit differences.  No counterpart in the reference (which has one
- * arithmetic). */
+ * the same algebra with fewer instructions (one reciprocal per shared
+ * denominator, FMA chains, WENO5 on first differences, flux projections
+ * from state projections): agreement with EXACT to roundoff, within 1e-12
+ * on smooth periodic boxes (6.4e-13 at 2048^2) but not on the cylinder's
+ * first full RK3 step (1.4e-12), where the boundary WENO weights
+ * (eps + beta)^-10 (src/bc1.rs:241) amplify last-bit differences.  No
+ * counterpart in the reference (which has one arithmetic). */
 #define RH3T_ARITH_EXACT 0
 #define RH3T_ARITH_FAST 1
 int rh3t_ctx_set_arith(rh3t_ctx* ctx, int32_t mode);
@@ -292,10 +293,16 @@ int rh3t_ctx_set_physics(rh3t_ctx* ctx, const rh3t_physics* phys);
 /* Workgroup shape of the WENO flux / assembly kernels (bx lanes along y,
  * by rows along x; bx a multiple of 16, bx*by <= 256 and a multiple of 64)
  * and heavy ghosts per workgroup (1..4, one wavefront each).  Other values
- * return RH3T_ERR_ARG and keep the current shape.  Chosen by the autotuner
- * in 3t_rh_amd/autotune.py; no counterpart in the reference. */
+ * return RH3T_ERR_ARG and keep the current shape.  Default 64 x 1 with one
+ * ghost per workgroup; the batched runs' shapes are chosen by the autotuner
+ * in 3t_rh_amd/autotune.py.  No counterpart in the reference. */
 int rh3t_ctx_set_launch(rh3t_ctx* ctx, int32_t bx, int32_t by,
                         int32_t ghosts_per_block);
+/* Workgroup shape of the x-direction flux kernel alone (same limits), which
+ * otherwise follows rh3t_ctx_set_launch; (0, 0) returns it to that.  The
+ * fused y-flux + assembly pass needs rows of 16, 32 or 64 lanes, the x flux
+ * does not.  No counterpart in the reference. */
+int rh3t_ctx_set_launch_x(rh3t_ctx* ctx, int32_t bx, int32_t by);
 
 /* First failure of the last call (main.rs:24-33 / ghost.rs:229-263). */
 int rh3t_last_error(const rh3t_ctx* ctx, rh3t_error* err);
@@ -355,6 +362,7 @@ int rh3t_batch_advance(rh3t_batch* b, int32_t nsteps, double cfl_factor,
 int rh3t_batch_set_arith(rh3t_batch* b, int32_t mode);
 int rh3t_batch_set_launch(rh3t_batch* b, int32_t bx, int32_t by,
                           int32_t ghosts_per_block);
+int rh3t_batch_set_launch_x(rh3t_batch* b, int32_t bx, int32_t by);
 int rh3t_batch_last_error(const rh3t_batch* b, int32_t solve,
                           rh3t_error* err);
 int rh3t_batch_profile(rh3t_batch* b, int32_t enable);

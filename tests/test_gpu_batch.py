@@ -79,8 +79,11 @@ def test_batch_with_wall_ghosts_matches_plain_bitwise(rh3t, workloads):
     b.close()
 
 
-@pytest.mark.parametrize("shape", [(32, 8, 2), (128, 2, 4), (16, 16, 1)])
+@pytest.mark.parametrize("shape", [(32, 8, 2), (128, 2, 4), (16, 16, 1), (64, 4, 1, (16, 16)),
+                                   (32, 8, 1, (128, 2)), (64, 1, 2, (64, 4))])
 def test_batch_launch_shape_invariance(rh3t, members, shape):
+    """Launch shapes (the fused pass's, the x flux's own, heavy ghosts per
+    workgroup) change no bit of the results."""
     fields, states = members
     _say(f"batch launch shape {shape}")
     ref = []
@@ -110,6 +113,10 @@ def test_batch_launch_shape_limits(rh3t, members):
         with pytest.raises(rh3t.RH3TError) as ei:
             b.set_launch(*shape)
         assert ei.value.status == rh3t.ERR_ARG, shape
+    for xs in [(256, 2), (24, 8), (16, 2), (0, 4)]:
+        with pytest.raises(rh3t.RH3TError) as ei:
+            b.set_launch(64, 4, 1, xs)
+        assert ei.value.status == rh3t.ERR_ARG, xs
     for k, u in enumerate(states):
         b.set_state(k, u, 0.0)
     b.advance(1)
