@@ -48,14 +48,14 @@ BYTES_PER_CELL_STAGE = 128  # stage 1: read U^k, write U (96 B); stages 2-3: + U
 # same counter reads for a plain 1 GiB device-to-device copy
 # (`--copy-calibration`: FETCH_SIZE reads 524,304 KiB -> x2.0, WRITE_SIZE
 # 1,048,576 KiB -> x1.0).
-FLUX_X_FETCH_KIB, FLUX_X_WRITE_KIB = 236969, 196704
+FLUX_X_FETCH_KIB, FLUX_X_WRITE_KIB = 236946, 196704
 COPY_FETCH_SCALE, COPY_WRITE_SCALE = 2.0, 1.0
 FLUX_X_TRAFFIC_BYTES = (FLUX_X_FETCH_KIB * COPY_FETCH_SCALE + FLUX_X_WRITE_KIB * COPY_WRITE_SCALE) * 1024
 TRAFFIC_SOURCE = ("not measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_flux<0> at 2048^2, "
                   "copy-calibrated (profiles/flux_x_counters_2048.md)")
 # the same for one k_fy_assemble<true> launch (profiles/hbm_per_stage_2048.md);
 # it writes the new state's 8 Derived planes beside the state
-FY_AS_FETCH_KIB, FY_AS_WRITE_KIB = 464010, 462250
+FY_AS_FETCH_KIB, FY_AS_WRITE_KIB = 460304, 460009
 FY_AS_TRAFFIC_BYTES = (FY_AS_FETCH_KIB * COPY_FETCH_SCALE + FY_AS_WRITE_KIB * COPY_WRITE_SCALE) * 1024
 FY_AS_TRAFFIC_SOURCE = ("not measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_fy_assemble<true> "
                         "at 2048^2, copy-calibrated (profiles/hbm_per_stage_2048.md)")
