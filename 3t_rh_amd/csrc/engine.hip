@@ -640,10 +640,10 @@ __global__ __launch_bounds__(256) void k_derived(const SolveDesc* __restrict__ d
 #endif
 // Flux of interface (i, j) of direction DIR (DIR 0: i - 1/2 at column j,
 // DIR 1: j - 1/2 at row i) from buffer buf; 0.0 when neither neighbour is
-// a fluid cell.
+// a fluid cell.  P receives the stencil (the caller may read it afterwards).
 template <int DIR, bool FAST>
 __device__ __forceinline__ void iface_flux(const SolveDesc& sd, const EigConst* __restrict__ Kp, double eps, int buf,
-                                           int i, int j, double out[6]) {
+                                           int i, int j, double out[6], Stencil<FAST>& P) {
   const Geo g = sd.g;
   const double* __restrict__ S = glb(sd.U[buf]);
   const double* __restrict__ D = glb(sd.D);
@@ -669,7 +669,6 @@ __device__ __forceinline__ void iface_flux(const SolveDesc& sd, const EigConst* 
   auto at = [&](const double* plane, long q_off) {
     return DIR == 0 ? plane_at(glb(plane + q_off), boff) : plane[c0 + q_off];
   };
-  Stencil<FAST> P;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     const long qo = (q - 3) * step;
@@ -726,7 +725,8 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
   const long fpl = DIR == 0 ? (long)(g.nx + 1) * g.ny : (long)g.nx * (g.ny + 1);
   const long fidx = DIR == 0 ? (long)i * g.ny + j : (long)i * (g.ny + 1) + j;
   double out[6];
-  iface_flux<DIR, FAST>(sd, Kp, eps, buf, i, j, out);
+  Stencil<FAST> P;
+  iface_flux<DIR, FAST>(sd, Kp, eps, buf, i, j, out, P);
 #pragma unroll
   for (int f = 0; f < 6; ++f) F[f * fpl + fidx] = out[f];
 }
@@ -739,10 +739,16 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_flux(const SolveDesc* 
 // FAST: the noncon divisions as products (noncon_fast); EXACT: the
 // reference's divisions.
 // FYB / FYT: the y fluxes at j - 1/2 and j + 1/2, read from the FY planes
-// (FROM_PLANES) or passed in by the fused y-flux kernel.
+// (FROM_PLANES) or passed in by the fused y-flux kernel, which also passes
+// what it already holds from the y stencil of interface j - 1/2: the state
+// of the cell in buffer src (uc_in, used for a stage's base or added state
+// when that is buffer src) and, in FAST, the y noncon terms (ncy_in).
 template <bool FAST, bool FROM_PLANES>
 __device__ __forceinline__ double assemble_core(const SolveDesc& sd, const Phys& ph, const StageSpec& sa, int i, int j,
-                                                const double* fyb_in, const double* fyt_in) {
+                                                const double* fyb_in, const double* fyt_in,
+                                                const double* ncy_in = nullptr, const double* uc_in = nullptr,
+                                                int src = -1) {
+  constexpr bool NCY_IN = FAST && !FROM_PLANES;
   const Geo g = sd.g;
   const long c0 = pad_index(g, i, j);
   const long lin = (long)i * g.ny + j;
@@ -793,18 +799,23 @@ __device__ __forceinline__ double assemble_core(const SolveDesc& sd, const Phys&
       noncon_fast(vel, pe, pi, pr, inv_dx, ncx);
     else
       noncon(vel, pe, pi, pr, g.dx, g.rdx, g.r60dx, ncx);
+    if constexpr (NCY_IN) {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const long p = c0 + (k - 4);
-      vel[k] = D[D_V * g.plane + p];
-      pe[k] = D[D_PE * g.plane + p];
-      pi[k] = D[D_PI * g.plane + p];
-      pr[k] = D[D_PR * g.plane + p];
+      for (int e = 0; e < 3; ++e) ncy[e] = ncy_in[e];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        const long p = c0 + (k - 4);
+        vel[k] = D[D_V * g.plane + p];
+        pe[k] = D[D_PE * g.plane + p];
+        pi[k] = D[D_PI * g.plane + p];
+        pr[k] = D[D_PR * g.plane + p];
+      }
+      if (FAST)
+        noncon_fast(vel, pe, pi, pr, inv_dy, ncy);
+      else
+        noncon(vel, pe, pi, pr, g.dy, g.rdy, g.r60dy, ncy);
     }
-    if (FAST)
-      noncon_fast(vel, pe, pi, pr, inv_dy, ncy);
-    else
-      noncon(vel, pe, pi, pr, g.dy, g.rdy, g.r60dy, ncy);
   }
 #pragma unroll
   for (int e = 0; e < 3; ++e) r[3 + e] = r[3 + e] + ncx[e] + ncy[e];
@@ -842,19 +853,34 @@ __device__ __forceinline__ double assemble_core(const SolveDesc& sd, const Phys&
   const double* __restrict__ base = glb(sd.U[sa.base]);
   const double* __restrict__ add = glb(sd.U[sa.add]);
   double* dst = glb(sd.U[sa.dst]);
-  double v[6];
+  double v[6], bv[6], av[6];
+  if (!FROM_PLANES && sa.base == src) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) bv[f] = uc_in[f];
+  } else {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) bv[f] = base[f * g.plane + c0];
+  }
+  if (sa.mode >= 2) {
+    if (!FROM_PLANES && sa.add == src) {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) av[f] = uc_in[f];
+    } else {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) av[f] = add[f * g.plane + c0];
+    }
+  }
   if (sa.mode == 1) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) v[f] = base[f * g.plane + c0] + r[f] * dt;
+    for (int f = 0; f < 6; ++f) v[f] = bv[f] + r[f] * dt;
   } else if (sa.mode == 2) {
     const double coef = dt / 4.0;
 #pragma unroll
-    for (int f = 0; f < 6; ++f) v[f] = base[f * g.plane + c0] * 0.75 + add[f * g.plane + c0] * 0.25 + r[f] * coef;
+    for (int f = 0; f < 6; ++f) v[f] = bv[f] * 0.75 + av[f] * 0.25 + r[f] * coef;
   } else {
     const double coef = 2.0 * dt / 3.0;
 #pragma unroll
-    for (int f = 0; f < 6; ++f)
-      v[f] = base[f * g.plane + c0] * (1.0 / 3.0) + add[f * g.plane + c0] * (2.0 / 3.0) + r[f] * coef;
+    for (int f = 0; f < 6; ++f) v[f] = bv[f] * (1.0 / 3.0) + av[f] * (2.0 / 3.0) + r[f] * coef;
   }
 #pragma unroll
   for (int f = 0; f < 6; ++f) dst[f * g.plane + c0] = v[f];
@@ -930,12 +956,46 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_fy_assemble(const Solv
   const int j = (lt % tiles_j) * cw + threadIdx.x;
   const int i = (lt / tiles_j) * blockDim.y + threadIdx.y;
   double fb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (i < nx && j <= ny) iface_flux<1, FAST>(sd, Kp, ph.eps, src, i, j, fb);
+  double uc[6], ncy[3];
+  if (i < nx && j <= ny) {
+    Stencil<FAST> P;
+    iface_flux<1, FAST>(sd, Kp, ph.eps, src, i, j, fb, P);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) uc[f] = P.U[3][f];  // point j of the stencil: the cell
+    if constexpr (FAST) {
+      // The y noncon line of cell j (points j-4 .. j+4, src/noncon.rs:288-332):
+      // the stencil holds points j-3 .. j+2 of v, p_e, p_i, p_r; the other
+      // three are read here (for j = ny, not a cell, they may fall past the
+      // halo into the next padded row: in bounds, not used).
+      const Geo g = sd.g;
+      const double* __restrict__ D = glb(sd.D);
+      const long c0 = pad_index(g, i, j);
+      double vel[9], pe[9], pi[9], pr[9];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        vel[q + 1] = P.w[q];
+        pe[q + 1] = P.pe[q];
+        pi[q + 1] = P.pi[q];
+        pr[q + 1] = P.pr[q];
+      }
+      vel[0] = vel[7] = vel[8] = 0.0;  // noncon_fast reads vel[3..5] only
+      const int ko[3] = {0, 7, 8};
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        const long p = c0 + (ko[e] - 4);
+        pe[ko[e]] = D[D_PE * g.plane + p];
+        pi[ko[e]] = D[D_PI * g.plane + p];
+        pr[ko[e]] = D[D_PR * g.plane + p];
+      }
+      noncon_fast(vel, pe, pi, pr, g.rdy, ncy);
+    }
+  }
   double ft[6];
 #pragma unroll
   for (int f = 0; f < 6; ++f) ft[f] = __shfl_down(fb[f], 1, blockDim.x);
-  const double m = ((int)threadIdx.x < cw && i < nx && j < ny) ? assemble_core<FAST, false>(sd, ph, sa, i, j, fb, ft)
-                                                                : INFINITY;
+  const double m = ((int)threadIdx.x < cw && i < nx && j < ny)
+                       ? assemble_core<FAST, false>(sd, ph, sa, i, j, fb, ft, ncy, uc, src)
+                       : INFINITY;
   if (sa.mode == 3) put_group_min(m, partial, t);
 }
 
