@@ -259,7 +259,9 @@ constexpr int kOpenLG = 64 / kOpenPerWave;
 // data rows of 57 doubles (55 E_r points, u_inf, one pad): a stride of 56
 // put rows c and c + 2 on the same LDS banks
 constexpr int kRow = 57;
-constexpr int kOpenLds = 6 * kRow + 90 + kFrameLen;  // doubles per ghost
+// per-ghost scalars read by the later phases (see ghost_grp)
+constexpr int kScal = 12;
+constexpr int kOpenLds = 6 * kRow + 90 + kFrameLen + kScal;  // doubles per ghost
 // Wall ghosts keep the L rows for their ILW systems: alpha gets its own 30
 constexpr int grp_lds(bool wall) { return kOpenLds + (wall ? 30 : 0); }
 static_assert(kOpenLG == 32, "the swizzled table schedule is for 32-lane groups");
@@ -280,78 +282,87 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
   // k <= r), the frame (rows of L, rows of R, eigenvalues).  Once read, the
   // L rows hold alpha[6][5], and data slots that are no longer read hold the
   // normalised weights om[6][5] and V^(k) v[5][6] (slot 55 of each row is
-  // kept).  504 doubles per ghost.
+  // kept).  The per-ghost scalars s_sc (12 doubles): the distance D, the
+  // destination slot, the Outflow constants, the weights d_r, the state
+  // plane pointer and the plane size.  522 doubles per ghost.
   extern __shared__ double s_dyn[];
   double* const s_data = s_dyn + ((long)w * kOpenPerWave + grp) * grp_lds(WALL);
   double* const s_part = s_data;
   double* const s_vk = s_data + 6 * kRow;
   double* const s_F = s_vk + 90;
   double* const s_alpha = WALL ? s_F + kFrameLen : s_F;  // Wall keeps the L rows
+  double* const s_sc = s_F + kFrameLen + (WALL ? 30 : 0);
   double* const s_v = s_data;
   double* const s_om = s_data + 60;
   const bool active = hl < count;
   const int h = h0 + (active ? hl : 0);
+  // Every global input of the ghost is requested before the first barrier
+  // in two dependent rounds, with no branch between the loads: first what
+  // the ghost index gives (table row, solve, frame, kind, normal, the packed
+  // indices of its E_r points, D, the destination slot, the Outflow
+  // constants), then what those give (the packed E_r states -- 3 x 16 bytes
+  // per point, packed by k_ghost_prep -- its frame's eigensystem, the table
+  // entries of the first segment, d_r, the state plane).  Their latencies
+  // overlap, and the scalars that later phases use wait in LDS instead of
+  // being read from global memory once the arithmetic has started.  Loads of
+  // lanes or slots that are not used read valid entries (index 54 for the
+  // slots past the 55 E_r points, ghost h0 for an idle group).
+  constexpr int NP = (56 + LG - 1) / LG;
   const long row = hg.tab[h];
-  const SolveDesc& sd = descs[hg.sid[h]];
-  double* S = sd.U[buf];
-  const long plane = sd.g.plane;
+  const int sid = hg.sid[h];
+  const int frm = hg.frame[h];
   const double nx = hg.normal[2L * h], ny = hg.normal[2L * h + 1];
   const bool far = !WALL && hg.kind[h] == RH3T_BC_FARFIELD;
-  // The ghost's loads are issued together: the state gathers at its E_r
-  // points (packed by k_ghost_prep: 3 x 16 bytes per point), its frame's
-  // eigensystem (k_ghost_prep) and the table entries of the first
-  // segment, so that their latencies overlap.
+  int gp[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) gp[p] = hg.gpk[55L * h + min(l + p * LG, 54)];
+  using u64 = unsigned long long;
+  // lanes 0..4: D, the destination slot, p_inf, sigma, l_domain
+  const u64* sc1p = l == 0   ? reinterpret_cast<const u64*>(hg.dist + h)
+                    : l == 1 ? reinterpret_cast<const u64*>(hg.pad + h)
+                             : reinterpret_cast<const u64*>(hg.outflow + 3L * h + min(max(l - 2, 0), 2));
+  const u64 sc1 = *sc1p;
+  // lanes 0..4: d_0 .. d_4 of the row; 5: the state plane pointer; 6: the plane size
+  const SolveDesc* sdp = descs + sid;
+  const u64* sc2p = l < 5 ? reinterpret_cast<const u64*>(hg.d5 + 5L * row + l)
+                    : l == 5 ? reinterpret_cast<const u64*>(sdp->U + buf)
+                             : reinterpret_cast<const u64*>(&sdp->g.plane);
+  const double* Fg = hg.F + (long)kFrameLen * frm;
+  double fv[(kFrameLen + LG - 1) / LG];
+#pragma unroll
+  for (int q = 0; q < (kFrameLen + LG - 1) / LG; ++q) fv[q] = Fg[min(l + q * LG, kFrameLen - 1)];
+  const u64 sc2 = *sc2p;
   const double2* T = reinterpret_cast<const double2*>(hg.T + (long)kSwzLen * row) + l;
   double tvA[kSwzA];
   // L . u at the E_r points (a < 55) and, for a FarField ghost, L . u_inf
   // (src/bc1.rs:829-924) in the free slot 55: point a = l + p LG, p = 0, 1
   {
-    constexpr int NP = (56 + LG - 1) / LG;
     double u6[NP][6];
     bool has[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
       const int a = l + p * LG;
       has[p] = active && (a < 55 || (a == 55 && far));
-      double s6[6] = {0, 0, 0, 0, 0, 0};
-      if (has[p]) {
-        if (a < 55) {
-          const double2* pp = reinterpret_cast<const double2*>(hg.P + 6L * hg.gpk[55L * h + a]);
-          const double2 x0 = pp[0], x1 = pp[1], x2 = pp[2];
-          s6[0] = x0.x;
-          s6[1] = x0.y;
-          s6[2] = x1.x;
-          s6[3] = x1.y;
-          s6[4] = x2.x;
-          s6[5] = x2.y;
-        } else {
-#pragma unroll
-          for (int f = 0; f < 6; ++f) s6[f] = hg.bcstate[6L * h + f];
-        }
-      }
-      u6[p][0] = s6[0];
-      u6[p][1] = s6[1] * nx + s6[2] * ny;
-      u6[p][2] = -s6[1] * ny + s6[2] * nx;
-      u6[p][3] = s6[3];
-      u6[p][4] = s6[4];
-      u6[p][5] = s6[5];
+      const double2* pp = reinterpret_cast<const double2*>(a == 55 ? hg.bcstate + 6L * h : hg.P + 6L * gp[p]);
+      const double2 x0 = pp[0], x1 = pp[1], x2 = pp[2];
+      u6[p][0] = x0.x;
+      u6[p][1] = x0.y * nx + x1.x * ny;
+      u6[p][2] = -x0.y * ny + x1.x * nx;
+      u6[p][3] = x1.y;
+      u6[p][4] = x2.x;
+      u6[p][5] = x2.y;
     }
-    if (active) {
-      const double* Fg = hg.F + (long)kFrameLen * hg.frame[h];
-      double fv[(kFrameLen + LG - 1) / LG];
 #pragma unroll
-      for (int q = 0; q < (kFrameLen + LG - 1) / LG; ++q)
-        if (l + q * LG < kFrameLen) fv[q] = Fg[l + q * LG];
-#pragma unroll
-      for (int q = 0; q < kSwzPairsA; ++q) {
-        const double2 t2 = T[32 * q];
-        tvA[2 * q] = t2.x;
-        if (2 * q + 1 < kSwzA) tvA[2 * q + 1] = t2.y;
-      }
-#pragma unroll
-      for (int q = 0; q < (kFrameLen + LG - 1) / LG; ++q)
-        if (l + q * LG < kFrameLen) s_F[l + q * LG] = fv[q];
+    for (int q = 0; q < kSwzPairsA; ++q) {
+      const double2 t2 = T[32 * q];
+      tvA[2 * q] = t2.x;
+      if (2 * q + 1 < kSwzA) tvA[2 * q + 1] = t2.y;
     }
+#pragma unroll
+    for (int q = 0; q < (kFrameLen + LG - 1) / LG; ++q)
+      if (l + q * LG < kFrameLen) s_F[l + q * LG] = fv[q];
+    if (l < 5) s_sc[l] = __longlong_as_double((long long)sc1);
+    if (l < 7) s_sc[5 + l] = __longlong_as_double((long long)sc2);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -444,7 +455,7 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
       const double* p = &s_part[c * kRow + r * (r + 1) * (2 * r + 1) / 6];
       double beta = 0.0;
       for (int a = 0; a < m; ++a) beta += p[a];
-      s_alpha[c * 5 + r] = hg.d5[5L * row + r] / weno_pow(ph.eps + beta, ph.weno_q);
+      s_alpha[c * 5 + r] = s_sc[5 + r] / weno_pow(ph.eps + beta, ph.weno_q);
     }
   }
   __syncthreads();
@@ -471,7 +482,7 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
     }
   }
   __syncthreads();
-  const double dist = hg.dist[h];
+  const double dist = s_sc[0];
   double* const s_uk = s_vk;        // [k][i]
   double* const s_ug = s_vk + 30;   // Taylor sums
   double* const s_fix = s_vk + 42;  // Outflow: flag, rhs_1[0]; Wall: solved
@@ -549,7 +560,7 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
       double lambda1 = un0 - a;
       s_fix[0] = 0.0;
       if (lambda1 < 0.0) {
-        const double p_inf = hg.outflow[3L * h], sigma = hg.outflow[3L * h + 1], l_dom = hg.outflow[3L * h + 2];
+        const double p_inf = s_sc[2], sigma = s_sc[3], l_dom = s_sc[4];
         double ml = fabs(un0 / a);
         double l1 = sigma * (1.0 - sq(ml)) * a / l_dom * (p0 - p_inf);
         s_fix[0] = 1.0;
@@ -591,11 +602,13 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
   if (active && l < 6) {  // rotation back to (x, y)
     const int f = l;
     const double out = f == 1 ? s_ug[1] * nx - s_ug[2] * ny : (f == 2 ? s_ug[1] * ny + s_ug[2] * nx : s_ug[f]);
-    S[f * plane + hg.pad[h]] = out;
+    double* const S = reinterpret_cast<double*>(__double_as_longlong(s_sc[10]));
+    const long plane = (long)__double_as_longlong(s_sc[11]), pad = (long)__double_as_longlong(s_sc[1]);
+    S[f * plane + pad] = out;
     if (WALL && s_fix[0] == 0.0) {
-      if (f == 0) raise_error(sd.sc, phase, RH3T_ERR_SINGULAR, (unsigned)hg.gid[h]);
+      if (f == 0) raise_error(descs[sid].sc, phase, RH3T_ERR_SINGULAR, (unsigned)hg.gid[h]);
     } else if (!isfinite(out)) {
-      raise_error(sd.sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
+      raise_error(descs[sid].sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
     }
   }
 }
@@ -1147,7 +1160,8 @@ __global__ __launch_bounds__(256) void k_gather_ghosts(SolveDesc sd, int n, doub
 namespace rh3t_eng {
 const char* const kKernelNames[K_COUNT] = {"ghost_cheap", "ghost_heavy", "derived", "flux_x", "flux_y",
                                            "assemble", "dt_partial", "dt_final", "capture", "adj_iface",
-                                           "adj_gather", "adj_ghost", "adj_param", "flux_y_assemble"};
+                                           "adj_gather", "adj_ghost", "adj_param", "flux_y_assemble",
+                                           "ghost_prep"};
 }  // namespace rh3t_eng
 
 namespace {
@@ -1214,7 +1228,7 @@ void ghosts_on(Engine& E, const SolveDesc* desc, int buf, unsigned phase, int gt
   if (E.heavy.n > 0) {
     // the frames and packed points, with the cheap ghosts riding along
     // when nothing the first two read is a ghost point
-    Launcher L(E, K_GHOST_HEAVY);
+    Launcher L(E, K_GHOST_PREP);
     const int nbf = (E.frames.n + 63) / 64, nbp = nbf + (E.packed.n + 63) / 64;
     const int nbc = with_prep ? (E.cheap.n + 63) / 64 : 0;
     k_ghost_prep<<<nbp + nbc, 64, 0, E.stream>>>(desc, E.frames, E.packed, E.cheap, E.phys, buf, phase, gt, nbf,

@@ -129,7 +129,7 @@ struct StageSpec {
 
 enum KernelId {
   K_GHOST_CHEAP, K_GHOST_HEAVY, K_DERIVED, K_FLUX_X, K_FLUX_Y, K_ASSEMBLE, K_DT_PARTIAL, K_DT_FINAL, K_CAPTURE,
-  K_ADJ_IFACE, K_ADJ_GATHER, K_ADJ_GHOST, K_ADJ_PARAM, K_FY_ASSEMBLE, K_COUNT
+  K_ADJ_IFACE, K_ADJ_GATHER, K_ADJ_GHOST, K_ADJ_PARAM, K_FY_ASSEMBLE, K_GHOST_PREP, K_COUNT
 };
 extern const char* const kKernelNames[K_COUNT];
 
