@@ -149,15 +149,17 @@ __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict
 }
 
 // Derived::from_state of every ghost slot of buffer buf (the cheap ghosts,
-// then the heavy ones), its first wd planes: after a ghost update whose
-// fluid cells' Derived planes are current.
+// then the heavy ones in solve and slot order, so that a wavefront's lanes
+// touch neighbouring slots of one solve), its first wd planes: after a ghost
+// update whose fluid cells' Derived planes are current.
 __global__ __launch_bounds__(256) void k_derived_ghosts(const SolveDesc* __restrict__ descs, CheapGhosts cg,
                                                         HeavyGhosts hg, Phys ph, int buf, int wd) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= cg.n + hg.n) return;
   const bool ch = k < cg.n;
-  const SolveDesc& sd = descs[ch ? cg.sid[k] : hg.sid[k - cg.n]];
-  const long p = ch ? cg.pad[k] : hg.pad[k - cg.n];
+  const int hk = ch ? 0 : hg.dord[k - cg.n];
+  const SolveDesc& sd = descs[ch ? cg.sid[k] : hg.sid[hk]];
+  const long p = ch ? cg.pad[k] : hg.pad[hk];
   const double* __restrict__ S = sd.U[buf];
   double st[6];
 #pragma unroll
@@ -1762,7 +1764,10 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
     permute(h_of, 3);
   }
   // heavy ghosts of one solve that share a nearest point and a normal share
-  // a frame
+  // a frame.  Frames are numbered in (solve, nearest slot) order, so that
+  // the lanes of a k_ghost_prep wavefront read neighbouring slots of one
+  // solve (the ghost order puts the clones of a ghost, in other solves, next
+  // to each other).
   std::vector<int> h_frame(E.heavy.n, 0), f_sid;
   std::vector<long> f_near;
   std::vector<double> f_n;
@@ -1772,35 +1777,42 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
       uint64_t bx, by;
       std::memcpy(&bx, &h_n[2 * (size_t)k], sizeof(bx));
       std::memcpy(&by, &h_n[2 * (size_t)k + 1], sizeof(by));
-      const auto key = std::make_tuple(h_sid[k], h_near[k], bx, by);
-      auto it = fid.find(key);
-      if (it == fid.end()) {
-        it = fid.emplace(key, (int)f_sid.size()).first;
-        f_sid.push_back(h_sid[k]);
-        f_near.push_back(h_near[k]);
-        f_n.push_back(h_n[2 * (size_t)k]);
-        f_n.push_back(h_n[2 * (size_t)k + 1]);
-      }
-      h_frame[k] = it->second;
+      fid.emplace(std::make_tuple(h_sid[k], h_near[k], bx, by), 0);
+    }
+    for (auto& kv : fid) {  // key order
+      kv.second = (int)f_sid.size();
+      f_sid.push_back(std::get<0>(kv.first));
+      f_near.push_back(std::get<1>(kv.first));
+      double nx_, ny_;
+      const uint64_t bx = std::get<2>(kv.first), by = std::get<3>(kv.first);
+      std::memcpy(&nx_, &bx, sizeof(nx_));
+      std::memcpy(&ny_, &by, sizeof(ny_));
+      f_n.push_back(nx_);
+      f_n.push_back(ny_);
+    }
+    for (int k = 0; k < E.heavy.n; ++k) {
+      uint64_t bx, by;
+      std::memcpy(&bx, &h_n[2 * (size_t)k], sizeof(bx));
+      std::memcpy(&by, &h_n[2 * (size_t)k + 1], sizeof(by));
+      h_frame[k] = fid.at(std::make_tuple(h_sid[k], h_near[k], bx, by));
     }
   }
-  // the distinct E_r points of the heavy ghosts, numbered in the order the
-  // ghosts first read them
+  // the distinct E_r points of the heavy ghosts, numbered in (solve, slot)
+  // order: k_ghost_prep's lanes copy neighbouring slots of one solve
   std::vector<int> h_gpk((size_t)E.heavy.n * 55), p_sid;
   std::vector<long> p_slot;
   {
     std::map<std::pair<int, long>, int> pid;
     for (int k = 0; k < E.heavy.n; ++k)
-      for (int a = 0; a < 55; ++a) {
-        const auto key = std::make_pair(h_sid[k], h_st[55 * (size_t)h_tab[k] + a]);
-        auto it = pid.find(key);
-        if (it == pid.end()) {
-          it = pid.emplace(key, (int)p_sid.size()).first;
-          p_sid.push_back(key.first);
-          p_slot.push_back(key.second);
-        }
-        h_gpk[55 * (size_t)k + a] = it->second;
-      }
+      for (int a = 0; a < 55; ++a) pid.emplace(std::make_pair(h_sid[k], h_st[55 * (size_t)h_tab[k] + a]), 0);
+    for (auto& kv : pid) {  // key order
+      kv.second = (int)p_sid.size();
+      p_sid.push_back(kv.first.first);
+      p_slot.push_back(kv.first.second);
+    }
+    for (int k = 0; k < E.heavy.n; ++k)
+      for (int a = 0; a < 55; ++a)
+        h_gpk[55 * (size_t)k + a] = pid.at(std::make_pair(h_sid[k], h_st[55 * (size_t)h_tab[k] + a]));
   }
   E.h_heavy_kind = h_kind;
   E.h_heavy_tab = h_tab;
@@ -1852,6 +1864,14 @@ int build_engine(Engine& E, const rh3t::Problem* const* P, int n, const rh3t_phy
   if ((st = dev_alloc(E, &E.frames.out, (size_t)kFrameLen * (f_sid.empty() ? 1 : f_sid.size())))) return st;
   E.heavy.F = E.frames.out;
   UP(h_gpk, ip) E.heavy.gpk = ip;
+  {
+    std::vector<int> dord(E.heavy.n);
+    for (int k = 0; k < E.heavy.n; ++k) dord[k] = k;
+    std::sort(dord.begin(), dord.end(), [&](int a, int b) {
+      return h_sid[a] != h_sid[b] ? h_sid[a] < h_sid[b] : h_pad[a] < h_pad[b];
+    });
+    UP(dord, ip) E.heavy.dord = ip;
+  }
   E.packed.n = (int)p_sid.size();
   UP(p_sid, ip) E.packed.sid = ip;
   UP(p_slot, lp) E.packed.slot = lp;

@@ -77,6 +77,7 @@ struct HeavyGhosts {
   const double* F;       // kFrameLen per frame, formed by k_ghost_prep each update
   const int* gpk;        // FarField / Outflow: 55 per ghost, PackedPoints index of each E_r point
   const double* P;       // PackedPoints values, formed by k_ghost_prep each update
+  const int* dord;       // the ghosts in (solve, slot) order, for k_derived_ghosts
 };
 
 // The E_r points read by the FarField / Outflow ghosts, each distinct
