@@ -55,7 +55,7 @@ TRAFFIC_SOURCE = ("not measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_
                   "copy-calibrated (profiles/flux_x_counters_2048.md)")
 # the same for one k_fy_assemble<true> launch (profiles/hbm_per_stage_2048.md);
 # it writes the new state's 8 Derived planes beside the state
-FY_AS_FETCH_KIB, FY_AS_WRITE_KIB = 460304, 460009
+FY_AS_FETCH_KIB, FY_AS_WRITE_KIB = 400295, 459561
 FY_AS_TRAFFIC_BYTES = (FY_AS_FETCH_KIB * COPY_FETCH_SCALE + FY_AS_WRITE_KIB * COPY_WRITE_SCALE) * 1024
 FY_AS_TRAFFIC_SOURCE = ("not measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of k_fy_assemble<true> "
                         "at 2048^2, copy-calibrated (profiles/hbm_per_stage_2048.md)")
