@@ -148,18 +148,17 @@ __global__ __launch_bounds__(256) void k_ghost_cheap(const SolveDesc* __restrict
   if (k < cg.n) ghost_cheap(descs, cg, buf, phase, gt, k, ph, wd);
 }
 
-// Derived::from_state of every ghost slot of buffer buf (the cheap ghosts,
-// then the heavy ones in solve and slot order, so that a wavefront's lanes
-// touch neighbouring slots of one solve), its first wd planes: after a ghost
-// update whose fluid cells' Derived planes are current.
-__global__ __launch_bounds__(256) void k_derived_ghosts(const SolveDesc* __restrict__ descs, CheapGhosts cg,
-                                                        HeavyGhosts hg, Phys ph, int buf, int wd) {
+// Derived::from_state of the heavy ghost slots of buffer buf, their first
+// wd planes, in (solve, slot) order so that a wavefront's lanes touch
+// neighbouring slots of one solve: used instead of forming them in the
+// ghost kernels when the heavy ghosts are many (see ghosts_on).
+__global__ __launch_bounds__(256) void k_derived_ghosts(const SolveDesc* __restrict__ descs, HeavyGhosts hg,
+                                                        Phys ph, int buf, int wd) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cg.n + hg.n) return;
-  const bool ch = k < cg.n;
-  const int hk = ch ? 0 : hg.dord[k - cg.n];
-  const SolveDesc& sd = descs[ch ? cg.sid[k] : hg.sid[hk]];
-  const long p = ch ? cg.pad[k] : hg.pad[hk];
+  if (k >= hg.n) return;
+  const int hk = hg.dord[k];
+  const SolveDesc& sd = descs[hg.sid[hk]];
+  const long p = hg.pad[hk];
   const double* __restrict__ S = sd.U[buf];
   double st[6];
 #pragma unroll
@@ -167,11 +166,13 @@ __global__ __launch_bounds__(256) void k_derived_ghosts(const SolveDesc* __restr
   put_derived(sd, ph, st, p, wd);
 }
 
-
 // y = R x of the X eigensystem with the entries formed on the fly (one dot6
 // per row, rows in order).
 // Heavy ghosts per workgroup: at most this many wavefronts (two ghosts each)
 constexpr int kMaxGhostsPerBlock = 4;
+// from this many heavy ghosts on, their Derived values get a pass of their
+// own (ghosts_on)
+constexpr int kHeavyDerivedPass = 130000;
 
 // One frame: build_local_plain (src/bc1.rs:294-312), the nearest
 // state rotated to the (n, t) frame and the X-direction eigensystem of its
@@ -229,7 +230,7 @@ __device__ __forceinline__ void pack_point(const SolveDesc* __restrict__ descs, 
 // others take the cheap ghosts (which no frame or E_r point reads).
 __global__ __launch_bounds__(64) void k_ghost_prep(const SolveDesc* __restrict__ descs, GhostFrames fr,
                                                    PackedPoints pk, CheapGhosts cg, Phys ph, int buf,
-                                                   unsigned phase, int gt, int nbf, int nbp) {
+                                                   unsigned phase, int gt, int nbf, int nbp, int wd) {
   const int b = blockIdx.x;
   if (b < nbf) {
     const int k = b * 64 + threadIdx.x;
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(64) void k_ghost_prep(const SolveDesc* __restrict__
     if (e < pk.n) pack_point(descs, pk, buf, e);
   } else {
     const int k = (b - nbp) * 64 + threadIdx.x;
-    if (k < cg.n) ghost_cheap(descs, cg, buf, phase, gt, k, ph, 0);
+    if (k < cg.n) ghost_cheap(descs, cg, buf, phase, gt, k, ph, wd);
   }
 }
 
@@ -262,7 +263,7 @@ constexpr int kOpenLG = 64 / kOpenPerWave;
 // put rows c and c + 2 on the same LDS banks
 constexpr int kRow = 57;
 // per-ghost scalars read by the later phases (see ghost_grp)
-constexpr int kScal = 12;
+constexpr int kScal = 14;
 constexpr int kOpenLds = 6 * kRow + 90 + kFrameLen + kScal;  // doubles per ghost
 // Wall ghosts keep the L rows for their ILW systems: alpha gets its own 30
 constexpr int grp_lds(bool wall) { return kOpenLds + (wall ? 30 : 0); }
@@ -270,9 +271,10 @@ static_assert(kOpenLG == 32, "the swizzled table schedule is for 32-lane groups"
 // slot of v_k,r (k <= r; v_k,r = 0 for k > r is never stored)
 __device__ __forceinline__ int vk_slot(int k, int r) { return r * (r + 1) / 2 + k; }
 
+// wd > 0: also Derived of each new ghost value, its first wd planes
 template <bool WALL>
 __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, const HeavyGhosts& hg,
-                                          const Phys& ph, int buf, unsigned phase, int h0, int count) {
+                                          const Phys& ph, int buf, unsigned phase, int h0, int count, int wd) {
   constexpr int LG = kOpenLG;
   const int lane = threadIdx.x;
   const int grp = lane / LG, l = lane % LG;
@@ -284,9 +286,10 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
   // k <= r), the frame (rows of L, rows of R, eigenvalues).  Once read, the
   // L rows hold alpha[6][5], and data slots that are no longer read hold the
   // normalised weights om[6][5] and V^(k) v[5][6] (slot 55 of each row is
-  // kept).  The per-ghost scalars s_sc (12 doubles): the distance D, the
+  // kept).  The per-ghost scalars s_sc (14 doubles): the distance D, the
   // destination slot, the Outflow constants, the weights d_r, the state
-  // plane pointer and the plane size.  522 doubles per ghost.
+  // plane pointer, the plane size and the Derived plane pointer.  524
+  // doubles per ghost.
   extern __shared__ double s_dyn[];
   double* const s_data = s_dyn + ((long)w * kOpenPerWave + grp) * grp_lds(WALL);
   double* const s_part = s_data;
@@ -324,11 +327,13 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
                     : l == 1 ? reinterpret_cast<const u64*>(hg.pad + h)
                              : reinterpret_cast<const u64*>(hg.outflow + 3L * h + min(max(l - 2, 0), 2));
   const u64 sc1 = *sc1p;
-  // lanes 0..4: d_0 .. d_4 of the row; 5: the state plane pointer; 6: the plane size
+  // lanes 0..4: d_0 .. d_4 of the row; 5: the state plane pointer; 6: the
+  // plane size; 7: the Derived plane pointer
   const SolveDesc* sdp = descs + sid;
-  const u64* sc2p = l < 5 ? reinterpret_cast<const u64*>(hg.d5 + 5L * row + l)
+  const u64* sc2p = l < 5    ? reinterpret_cast<const u64*>(hg.d5 + 5L * row + l)
                     : l == 5 ? reinterpret_cast<const u64*>(sdp->U + buf)
-                             : reinterpret_cast<const u64*>(&sdp->g.plane);
+                    : l == 6 ? reinterpret_cast<const u64*>(&sdp->g.plane)
+                             : reinterpret_cast<const u64*>(&sdp->D);
   const double* Fg = hg.F + (long)kFrameLen * frm;
   double fv[(kFrameLen + LG - 1) / LG];
 #pragma unroll
@@ -364,7 +369,7 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
     for (int q = 0; q < (kFrameLen + LG - 1) / LG; ++q)
       if (l + q * LG < kFrameLen) s_F[l + q * LG] = fv[q];
     if (l < 5) s_sc[l] = __longlong_as_double((long long)sc1);
-    if (l < 7) s_sc[5 + l] = __longlong_as_double((long long)sc2);
+    if (l < 8) s_sc[5 + l] = __longlong_as_double((long long)sc2);
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -613,16 +618,59 @@ __device__ __forceinline__ void ghost_grp(const SolveDesc* __restrict__ descs, c
       raise_error(descs[sid].sc, phase, RH3T_ERR_GHOST, (unsigned)hg.gid[h]);
     }
   }
+  if (wd > 0) {
+    // Derived::from_state of the new ghost value (derived_from_state's
+    // expressions), its first wd planes, spread over lanes 1..5 of the group:
+    // lane f forms field f's quotient by rho (u, v, then s_3 / rho .. s_5 /
+    // rho), shares it through LDS, then lanes 3..5 form e_k, p_k and the
+    // temperature of their energy.  One quotient per lane instead of five
+    // in sequence on one lane.
+    double* const s_q = s_vk + 44;
+    double q = 0.0;
+    if (active && l >= 1 && l < 6) {
+      const double num = l == 1 ? s_ug[1] * nx - s_ug[2] * ny : (l == 2 ? s_ug[1] * ny + s_ug[2] * nx : s_ug[l]);
+      q = num / s_ug[0];
+      s_q[l] = q;
+    }
+    __syncthreads();
+    if (active && l >= 1 && l < 6) {
+      const double rho = s_ug[0], u = s_q[1], v = s_q[2];
+      const double w2 = sq(u) + sq(v);
+      double* const Dp = reinterpret_cast<double*>(__double_as_longlong(s_sc[12]));
+      const long plane = (long)__double_as_longlong(s_sc[11]), pad = (long)__double_as_longlong(s_sc[1]);
+      if (l < 3) {
+        Dp[(l == 1 ? D_U : D_V) * plane + pad] = q;
+      } else {
+        const int k = l - 3;
+        const double e = q - w2 / 6.0;
+        const double g = k == 0 ? ph.ge : (k == 1 ? ph.gi : ph.gr);
+        Dp[(D_EE + k) * plane + pad] = e;
+        if (D_PE + k < wd) Dp[(D_PE + k) * plane + pad] = g * (s_ug[l] - rho * w2 / 6.0);
+        if (wd > 8) {
+          double t;
+          if (k < 2) {
+            const double cv = k == 0 ? ph.cve : ph.cvi;
+            t = s_ug[l] / (rho * cv) - w2 / (6.0 * cv);
+          } else {
+            t = pow(e * rho / ph.a_rad, 0.25);
+          }
+          Dp[(D_TE + k) * plane + pad] = t;
+        }
+      }
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ghost_open(
-    const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
-  ghost_grp<false>(descs, hg, ph, buf, phase, h0, count);
+    const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count,
+    int wd) {
+  ghost_grp<false>(descs, hg, ph, buf, phase, h0, count, wd);
 }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ghost_wall(
-    const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count) {
-  ghost_grp<true>(descs, hg, ph, buf, phase, h0, count);
+    const SolveDesc* __restrict__ descs, HeavyGhosts hg, Phys ph, int buf, unsigned phase, int h0, int count,
+    int wd) {
+  ghost_grp<true>(descs, hg, ph, buf, phase, h0, count, wd);
 }
 
 // ----------------------------------------------------------- derived
@@ -1217,15 +1265,21 @@ int build_tables(Engine& E) {
 namespace rh3t_eng {
 
 // The ghost update of buffer buf with descriptors desc; wd > 0: also the
-// ghost slots' Derived planes (their first wd), in k_ghost_cheap when there
-// are no heavy ghosts, else in one pass over every ghost slot afterwards.
+// ghost slots' Derived planes (their first wd), formed by the kernel that
+// writes each ghost value -- except for the heavy ghosts of a context with
+// many of them, whose Derived values a pass of their own forms.  In the
+// heavy-ghost kernels the Derived values take a serial tail of a few lanes
+// of each two-ghost wavefront (~0.1 us per 1,000 ghosts and launch); the
+// separate pass takes a launch (~6 us) plus ~0.055 us per 1,000 slots at
+// full lane use: below ~130k heavy ghosts the tail costs less (a 121x481
+// cylinder, 4,480 ghosts: the RK3 step 7% shorter), above it the pass (the
+// 64-solve batch, 434k ghosts: 0.13 against 0.10 ms per step).
 void ghosts_on(Engine& E, const SolveDesc* desc, int buf, unsigned phase, int gt, int wd) {
   const int n_open = E.heavy.n_open;
   const bool with_prep = E.heavy.n > 0 && E.cheap_with_prep;
   if (E.cheap.n > 0 && !with_prep) {
     Launcher L(E, K_GHOST_CHEAP);
-    k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(desc, E.cheap, buf, phase, gt, E.phys,
-                                                                 E.heavy.n > 0 ? 0 : wd);
+    k_ghost_cheap<<<(E.cheap.n + 255) / 256, 256, 0, E.stream>>>(desc, E.cheap, buf, phase, gt, E.phys, wd);
   }
   if (E.heavy.n > 0) {
     // the frames and packed points, with the cheap ghosts riding along
@@ -1234,26 +1288,27 @@ void ghosts_on(Engine& E, const SolveDesc* desc, int buf, unsigned phase, int gt
     const int nbf = (E.frames.n + 63) / 64, nbp = nbf + (E.packed.n + 63) / 64;
     const int nbc = with_prep ? (E.cheap.n + 63) / 64 : 0;
     k_ghost_prep<<<nbp + nbc, 64, 0, E.stream>>>(desc, E.frames, E.packed, E.cheap, E.phys, buf, phase, gt, nbf,
-                                                 nbp);
+                                                 nbp, wd);
   }
+  const bool heavy_pass = E.heavy.n >= kHeavyDerivedPass;
+  const int wdh = heavy_pass ? 0 : wd;
   if (E.heavy.n > 0) {
     Launcher L(E, K_GHOST_HEAVY);
     const int g = E.ghosts_per_block, n_wall = E.heavy.n - n_open;
     if (n_open > 0) {
       const int per = g * kOpenPerWave;  // ghosts per workgroup
       k_ghost_open<<<(n_open + per - 1) / per, dim3(64, g), (size_t)per * kOpenLds * sizeof(double), E.stream>>>(
-          desc, E.heavy, E.phys, buf, phase, 0, n_open);
+          desc, E.heavy, E.phys, buf, phase, 0, n_open, wdh);
     }
     if (n_wall > 0) {
       const int per = g * kOpenPerWave;
       k_ghost_wall<<<(n_wall + per - 1) / per, dim3(64, g), (size_t)per * grp_lds(true) * sizeof(double),
-                     E.stream>>>(desc, E.heavy, E.phys, buf, phase, n_open, n_wall);
+                     E.stream>>>(desc, E.heavy, E.phys, buf, phase, n_open, n_wall, wdh);
     }
   }
-  if (wd > 0 && E.heavy.n > 0) {
+  if (wd > 0 && heavy_pass) {
     Launcher L(E, K_DERIVED);
-    const int n = E.cheap.n + E.heavy.n;
-    k_derived_ghosts<<<(n + 255) / 256, 256, 0, E.stream>>>(desc, E.cheap, E.heavy, E.phys, buf, wd);
+    k_derived_ghosts<<<(E.heavy.n + 255) / 256, 256, 0, E.stream>>>(desc, E.heavy, E.phys, buf, wd);
   }
 }
 
