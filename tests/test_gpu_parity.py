@@ -566,8 +566,10 @@ def test_kept_derived_planes_across_calls(rh3t, box64, cyl, case):
     those calls (steps, l() right after a step, set_state mid-run, new
     physics with the temperature planes, a VJP between steps) must match the
     oracle at every point: bitwise in EXACT at default physics.  box64 forms
-    the ghosts' Derived values in k_ghost_cheap, the cylinder in the pass
-    after its heavy ghosts."""
+    the ghosts' Derived values in k_ghost_cheap, the cylinder in its
+    heavy-ghost kernels (the temperature planes too, after the physics
+    change); the 64-solve batch test covers the pass that replaces them for
+    130k heavy ghosts or more."""
     f, u, p = box64 if case == "box64" else cyl
     if case == "cylinder":
         u = _developed(f, u, p, 1)
