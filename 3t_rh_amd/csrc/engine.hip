@@ -1070,10 +1070,7 @@ __global__ __launch_bounds__(256, RH3T_FLUX_WAVES) void k_fy_assemble(const Solv
 // (U[0] for an l()), phase 2k + 1 (stage k) in its destination, U[1], U[2]
 // and U[3] (the alias of U[0], or the tape buffer the adjoint's forward
 // points it at) -- so the failing values are still there.
-__global__ void k_capture(const SolveDesc* __restrict__ descs, int nsolve, unsigned lo, unsigned hi) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsolve) return;
-  const SolveDesc& sd = descs[s];
+__device__ __forceinline__ void capture_failure(const SolveDesc& sd, unsigned lo, unsigned hi) {
   Scalars* sc = sd.sc;
   unsigned long long key = sc->err;
   if (key == ~0ULL) return;
@@ -1084,6 +1081,11 @@ __global__ void k_capture(const SolveDesc* __restrict__ descs, int nsolve, unsig
   long p = (phase % 2 == 0) ? sd.gpad[idx] : pad_index(sd.g, idx / sd.g.ny, idx % sd.g.ny);
   const double* b = sd.U[buf];
   for (int f = 0; f < 6; ++f) sc->err_state[f] = b[f * sd.g.plane + p];
+}
+
+__global__ void k_capture(const SolveDesc* __restrict__ descs, int nsolve, unsigned lo, unsigned hi) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsolve) capture_failure(descs[s], lo, hi);
 }
 
 // ------------------------------------------------------------- dt
@@ -1121,6 +1123,10 @@ __global__ __launch_bounds__(256) void k_dt_partial(const SolveDesc* __restrict_
 // calc_global_dt (src/main.rs:375-408) and, when advancing,
 // dt = cfl_factor * dt_cfl (src/main.rs:461) or the fixed dt, the step
 // counter and Field.time.
+// advance: a time-loop step begins (step, dt and time move on); it first
+// captures the previous step's first failure, phases 2..7 (the previous
+// step's buffers are intact until this step's stages run), which spares the
+// time loop a capture launch per step.
 __global__ __launch_bounds__(1024) void k_dt_final(const SolveDesc* __restrict__ descs,
                                                    const int* __restrict__ tiles,
                                                    const double* __restrict__ partial, int advance) {
@@ -1150,6 +1156,7 @@ __global__ __launch_bounds__(1024) void k_dt_final(const SolveDesc* __restrict__
     Scalars* sc = descs[s].sc;
     sc->dt_cfl = dt_cfl;
     if (advance) {
+      capture_failure(descs[s], 2, 7);
       sc->step = sc->step + 1;
       double dt = sc->dt_fixed > 0.0 ? sc->dt_fixed : sc->cfl_factor * dt_cfl;
       sc->dt = dt;
@@ -1419,11 +1426,11 @@ void launch_dt(Engine& E, int advance, bool fresh) {
 //   stage 2: U2 = 3/4 U0 + 1/4 U1 + dt/4 L(U1)
 //   stage 3: U0 = 1/3 U0 + 2/3 U2 + 2dt/3 L(U2)   (in place: each cell
 //            reads its own U0 value before writing it)
-void launch_rk3(Engine& E) {
+void launch_rk3(Engine& E, bool capture) {
   launch_stage(E, 0, StageSpec{1, 0, 0, 1, 3}, true);
   launch_stage(E, 1, StageSpec{2, 0, 1, 2, 5}, true);
   launch_stage(E, 2, StageSpec{3, 0, 2, 0, 7}, true);
-  launch_capture(E, 2, 7);
+  if (capture) launch_capture(E, 2, 7);
 }
 
 // Reads every solve's first-failure key; returns the first failing status.
@@ -2002,10 +2009,12 @@ int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
   // the captured step starts from current Derived planes of U0 (as every
   // step leaves them), so that its first stage forms only the ghosts'
   ensure_derived(E, 0);
+  // each step's dt kernel captures the previous step's failure; the last
+  // step's is captured after the loop
   if (E.profiling) {
     for (int s = 0; s < nsteps; ++s) {
       launch_dt(E, 1, false);
-      launch_rk3(E);
+      launch_rk3(E, false);
     }
     HIP_TRY(hipGetLastError());
   } else {
@@ -2013,7 +2022,7 @@ int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
       hipGraph_t gr = nullptr;
       HIP_TRY(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
       launch_dt(E, 1, false);
-      launch_rk3(E);
+      launch_rk3(E, false);
       // end the capture whatever the launches reported, so that the stream
       // never stays in capture mode
       hipError_t le = hipGetLastError();
@@ -2029,6 +2038,8 @@ int advance(Engine& E, int nsteps, double cfl_factor, double dt_fixed) {
     }
     for (int s = 0; s < nsteps; ++s) HIP_TRY(hipGraphLaunch(E.graph, E.stream));
   }
+  launch_capture(E, 2, 7);
+  HIP_TRY(hipGetLastError());
   // the last stage 3 left the wavefront minima of the new state
   E.dt_partials_valid = nsteps > 0 || E.dt_partials_valid;
   return check_errors(E);

@@ -285,8 +285,9 @@ void launch_stage(Engine& E, int src, const StageSpec& sa, bool kept = false);
 // phase is in [lo, hi], into the solves' scalars (after the stages of those
 // phases; a recompute of stages that already ran without error skips it)
 void launch_capture(Engine& E, unsigned lo, unsigned hi);
-// one SSP-RK3 step with the dt held in the solves' scalars, then the capture
-void launch_rk3(Engine& E);
+// one SSP-RK3 step with the dt held in the solves' scalars, then (capture)
+// the capture
+void launch_rk3(Engine& E, bool capture = true);
 int check_errors(Engine& E);
 int reset_errors(Engine& E);
 // releases the adjoint workspace (host side; device memory is in allocs)

@@ -539,6 +539,32 @@ def test_nonpositive_dt_error(rh3t, box64):
     s.close()
 
 
+def test_advance_captures_failure_state(rh3t, box64):
+    """The time loop reports the state of its first failure: a negative
+    density fails stage 1 of the first step; advance(3) then runs steps 2
+    and 3 over the stage buffers, yet the reported state is the failing
+    cell's stage-1 output (taken by the dt kernel that starts step 2), as
+    the oracle's; advance(1) takes it after its only step."""
+    f, u, p = box64
+    bad = u.copy()
+    bad[10, 20, 0] = -1.0
+    for nsteps in (3, 1):
+        _say(f"error path: advance({nsteps}) over an injected negative density")
+        s = f.solver()
+        s.set_state(bad, 0.0)
+        dt = 0.05 * s.calc_global_dt()
+        with pytest.raises(rh3t.RH3TError) as ei:
+            s.advance(nsteps, 0.05)
+        with pytest.raises(ob.OracleError) as eo:
+            ob.rk3_step(p, bad, dt)
+        g, o = ei.value.error, eo.value.info
+        print("gpu", g, "oracle", o)
+        assert ei.value.status == rh3t.ERR_NONPHYSICAL
+        assert (g["stage"], g["i"], g["j"]) == (o["stage"], o["i"], o["j"])
+        assert np.allclose(g["state"], o["state"], rtol=1e-12, atol=0.0, equal_nan=True)
+        s.close()
+
+
 def test_set_physics_recomputes_dt(rh3t, box64):
     """After a step, new physics constants (a stiff exchange) must take part
     in the next dt: advance, set_physics, advance one step; the step's dt is
