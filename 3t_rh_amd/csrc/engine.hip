@@ -27,7 +27,8 @@
 //                   admissibility check (src/main.rs:307-349) in one pass
 //                   (k_flux<1> + k_assemble for launch shapes it does not
 //                   take).
-//   k_capture       the state of a step's first failure (one per step).
+//   k_capture       the state of a call's first failure (in the time loop
+//                   k_dt_final takes it for the previous step).
 //   k_dt_final      calc_global_dt (src/main.rs:375-408, src/dt.rs:5-34) as
 //                   wavefront fmin reductions: the last RK stage's assembly
 //                   leaves one minimum of get_local_dt of the new state per
