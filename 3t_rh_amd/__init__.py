@@ -652,6 +652,16 @@ class Batch:
             raise RH3TError(st, f"batch advance: {errs[:3]}", errs)
         return times
 
+    def last_error(self, k):
+        """Member k's first failure of the last call (the fields of
+        rh3t_error), or None."""
+        e = c_error()
+        _check(lib().rh3t_batch_last_error(self._h, k, ctypes.byref(e)), "batch last_error")
+        if not e.kind:
+            return None
+        return {"kind": e.kind, "stage": e.stage, "i": e.i, "j": e.j, "ghost_id": e.ghost_id,
+                "state": list(e.state), "message": e.message.decode(errors="replace")}
+
     def set_launch(self, bx, by, ghosts_per_block=1, x_shape=None):
         _check(lib().rh3t_batch_set_launch(self._h, bx, by, ghosts_per_block), "batch set_launch")
         if x_shape is not None:

@@ -143,6 +143,32 @@ def test_batch_member_against_oracle(rh3t, members):
     b.close()
 
 
+def test_batch_failure_state(rh3t, members):
+    """A negative density in the box member fails its stage 1 of the first
+    step; the batch's time loop runs two more steps over the stage buffers,
+    yet that member reports the failing cell's stage-1 state (the oracle's)
+    and the cylinders report no failure."""
+    fields, states = members
+    bad = states[0].copy()
+    bad[10, 20, 0] = -1.0
+    _say("batch error path: negative density in member 0, advance(3)")
+    b = rh3t.Batch(fields)
+    for k, u in enumerate(states):
+        b.set_state(k, bad if k == 0 else u, 0.0)
+    with pytest.raises(rh3t.RH3TError) as ei:
+        b.advance(3, 0.05)
+    assert ei.value.status == rh3t.ERR_NONPHYSICAL
+    p = ob.OracleProblem(fields[0])
+    with pytest.raises(ob.OracleError) as eo:
+        ob.rk3_step(p, bad, 0.05 * ob.global_dt(p, bad))
+    g, o = b.last_error(0), eo.value.info
+    print("gpu", g, "oracle", o)
+    assert (g["stage"], g["i"], g["j"]) == (o["stage"], o["i"], o["j"])
+    assert np.allclose(g["state"], o["state"], rtol=1e-12, atol=0.0, equal_nan=True)
+    assert all(b.last_error(k) is None for k in range(1, len(fields)))
+    b.close()
+
+
 def test_batched_workload_64(rh3t, workloads):
     """The benchmark's batched workload itself: 64 half-cylinder solves drawn
     with seed 4 over all four grid classes (121x481 ... 241x961) and Mach
