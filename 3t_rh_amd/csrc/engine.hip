@@ -1133,17 +1133,25 @@ __global__ __launch_bounds__(1024) void k_dt_final(const SolveDesc* __restrict__
                                                    const double* __restrict__ partial, int advance) {
   __shared__ double red[16];
   const int s = blockIdx.x;
-  // eight independent loads in flight per thread (fmin is exact and order
-  // independent: the same minimum in any grouping)
+  // sixteen independent loads in flight per thread, the last partial
+  // round predicated rather than walked one load at a time (fmin is exact
+  // and order independent: the same minimum in any grouping)
   const int k1 = tiles[s + 1];
   double mm[8];
 #pragma unroll
   for (int u = 0; u < 8; ++u) mm[u] = INFINITY;
   int k = tiles[s] + threadIdx.x;
-  for (; k + 7 * 1024 < k1; k += 8 * 1024)
+  for (; k + 15 * 1024 < k1; k += 16 * 1024)
 #pragma unroll
-    for (int u = 0; u < 8; ++u) mm[u] = fmin(mm[u], partial[k + u * 1024]);
-  for (; k < k1; k += 1024) mm[0] = fmin(mm[0], partial[k]);
+    for (int u = 0; u < 16; ++u) mm[u & 7] = fmin(mm[u & 7], partial[k + u * 1024]);
+  // the last fewer than 16 of this thread's partials, loaded together
+  if (k < k1) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = k + u * 1024 < k1 ? partial[k + u * 1024] : INFINITY;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) mm[u & 7] = fmin(mm[u & 7], v[u]);
+  }
   double m = mm[0];
 #pragma unroll
   for (int u = 1; u < 8; ++u) m = fmin(m, mm[u]);
